@@ -1,0 +1,161 @@
+/*
+ * C ABI of the MI355X Arnoldi exp(t*A)*v library (libmek.so).
+ *
+ * The reference (kol-di/matrix-exp-krylov) has no FFI: its interface is the
+ * C++ class surface of main.cu (CSRHost, DeviceContext, ArnoldiRunner) plus
+ * the cuSPARSE/cuBLAS calls its hot loop makes.  Each entry point below names
+ * the reference interface it stands for.  Plain C types only; no exit() and no
+ * C++ exception crosses this boundary.
+ *
+ * Conventions
+ *   - Every function returns MEK_OK (0) or a negative MEK_E_* status; the
+ *     message of the last failure on the calling thread is mek_last_error().
+ *   - Pointers named d_* are device pointers (hipMalloc'd, or from
+ *     mek_malloc); everything else is host memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Kernel-level entry points are asynchronous on that stream.
+ */
+#ifndef MEK_H
+#define MEK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MEK_OK 0
+#define MEK_E_INVALID (-1)   /* bad argument                         */
+#define MEK_E_LOAD (-2)      /* matrix file failed validation / I/O  */
+#define MEK_E_HIP (-3)       /* HIP runtime error                    */
+#define MEK_E_SOLVER (-4)    /* solver refused the problem           */
+#define MEK_E_INTERNAL (-5)
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mek_last_error(void);
+/* Library version string. */
+const char* mek_version(void);
+
+/* ---------------------------------------------------------------- device */
+
+int mek_device_count(int* count);           /* cudaGetDeviceCount, main.cu:2146 */
+int mek_device_synchronize(void);
+int mek_malloc(void** d_ptr, size_t bytes);
+int mek_free(void* d_ptr);
+int mek_memcpy_htod(void* d_dst, const void* src, size_t bytes);
+int mek_memcpy_dtoh(void* dst, const void* d_src, size_t bytes);
+int mek_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+
+/* ---------------------------------------------------------------- CSRHost */
+
+typedef struct mek_csr mek_csr;
+
+/* CSRHost built from caller arrays (copied).  reference: CSRHost fields,
+ * main.cu:86-95. */
+int mek_csr_create(int rows, int cols, long long nnz, const int* row_ptr, const int* col_idx,
+                   const double* values, mek_csr** out);
+/* reference: CSRHost::from_binary_csr, main.cu:265-523 (docs/binary_csr_v1.md) */
+int mek_csr_load_bincsr(const char* path, mek_csr** out);
+/* reference: CSRHost::from_matrix_market, main.cu:97-245 (also reads .mtx.gz) */
+int mek_csr_load_mtx(const char* path, mek_csr** out);
+/* Dimensions; value_dtype is 1 (f32) or 2 (f64) as stored in the source file. */
+int mek_csr_info(const mek_csr* A, int* rows, int* cols, long long* nnz, int* value_dtype);
+/* Copy the arrays out (row_ptr: rows+1, col_idx/values: nnz entries). */
+int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values);
+void mek_csr_destroy(mek_csr* A);
+
+/* ---------------------------------------------------------------- solver */
+
+/* reference: ArnoldiParams(m, t, tol, max_restarts, reorthogonalize),
+ * main.cu:1221-1230 */
+typedef struct {
+    int m;
+    double t;
+    double tol;
+    int max_restarts;
+    int reorthogonalize;
+} mek_arnoldi_params;
+
+/* Additive options (all-zero except gs_mode = -1, verbose = 1 reproduces the
+ * reference; see mek_solver_options_default). */
+typedef struct {
+    int step_control;   /* 0 reference controller, 1 Expokit heuristic       */
+    int gs_mode;        /* -1 follow reorthogonalize, 0 never, 1 always,
+                           2 conditional (DGKS, ||w'|| < dgks_eta ||w||)      */
+    double dgks_eta;
+    int values_f32;     /* store matrix values in FP32 on the device          */
+    int verbose;        /* print the reference's per-column / [STEP] / METRIC */
+    int timing;         /* HIP-event timing of every kernel launch            */
+    int fuse_spmv_dot;  /* first-pass dot products inside the SpMV kernel     */
+} mek_solver_options;
+
+void mek_solver_options_default(mek_solver_options* opt);
+
+typedef struct {
+    int converged;      /* ArnoldiRunner::converged      */
+    int restarts_done;  /* ArnoldiRunner::restarts_done  */
+    double last_residual;
+    double t_done;
+    int attempts;
+    int accepted;
+    int rejected;
+    int spmv_count;
+    int extra_spmv;
+    int reorth_columns;
+    int aborted;
+    double solve_ms;
+} mek_solve_stats;
+
+typedef struct mek_runner mek_runner;
+
+/* ArnoldiRunner(1, params, rows, cols, nnz) + init_all_devices on device
+ * `device_id` with the whole matrix (one GPU).  reference: main.cu:1286-1409 */
+int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* params,
+                      const mek_solver_options* options, int device_id, mek_runner** out);
+/* y = exp(t A) v, host vectors of n doubles.  reference:
+ * ArnoldiRunner::compute_expmv, main.cu:1411-1571 */
+int mek_runner_expmv(mek_runner* R, const double* v, double* y);
+/* Same with device-resident v and y (n doubles each). */
+int mek_runner_expmv_device(mek_runner* R, const double* d_v, double* d_y);
+/* Change t (and the derived TimeSteppingParams) between solves. */
+int mek_runner_set_t(mek_runner* R, double t);
+int mek_runner_stats(const mek_runner* R, mek_solve_stats* out);
+/* Per-attempt [STEP] records of the last solve: t, dt, err (3 doubles) and
+ * m_used, accepts, rejects (3 ints) per record; returns the count in *n. */
+int mek_runner_steps(const mek_runner* R, double* tde, int* mar, int cap, int* n);
+/* Event-timed kernel classes of the last solve (options.timing):
+ * kind 0 SpMV(+dots), 1 update+dot, 2 update+norm, 3 dots, 4 finalisers,
+ * 5 expm, 6 lift.  Summed ms, launches and algorithmic bytes. */
+int mek_runner_kernel_time(const mek_runner* R, int kind, double* ms, long long* launches,
+                           double* bytes);
+/* One Arnoldi decomposition from v: H ((m+1) x m, column-major).  Returns the
+ * number of columns built (< m after a breakdown) in *k_used. */
+int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used);
+/* The runner's stream (hipStream_t). */
+int mek_runner_stream(const mek_runner* R, void** stream);
+void mek_runner_destroy(mek_runner* R);
+
+/* ---------------------------------------------------------- kernel level */
+
+typedef struct mek_spmv_plan mek_spmv_plan;
+
+/* Adaptive CSR SpMV over device arrays; h_row_ptr (host copy of row_ptr) is
+ * used once to build the row blocks.  Replaces cusparseCreateCsr +
+ * cusparseSpMV_bufferSize (main.cu:987-1053).  vals_f32: values are float. */
+int mek_spmv_plan_create(int n, long long nnz, const int* d_row_ptr, const int* d_col_idx,
+                         const void* d_vals, int vals_f32, const int* h_row_ptr,
+                         mek_spmv_plan** out);
+/* d_y = A * d_x (beta = 0).  Replaces cusparseSpMV, main.cu:1743-1745. */
+int mek_spmv_execute(mek_spmv_plan* P, const double* d_x, double* d_y, void* stream);
+void mek_spmv_plan_destroy(mek_spmv_plan* P);
+
+/* d_E = exp(d_A) for a dense k x k column-major matrix on the device (one
+ * workgroup, FP64 Pade scaling and squaring).  Replaces the host Eigen
+ * A.exp() of main.cu:33-36.  Synchronous. */
+int mek_expm_dense(int k, const double* d_A, double* d_E);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEK_H */

@@ -1,0 +1,153 @@
+// ArnoldiRunner: y = exp(t*A)*v by adaptive time-stepping Arnoldi.
+// reference: ArnoldiParams / ExpmvMode / TimeSteppingParams / ArnoldiRunner,
+// main.cu:1220-2081.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "mek/csr_host.hpp"
+#include "mek/device_context.hpp"
+
+// reference: main.cu:81-82
+constexpr double ARNOLDI_TOL = 1e-8;
+constexpr int MAX_RESTARTS = 10;
+
+// reference: main.cu:1221-1230
+struct ArnoldiParams {
+  int m;
+  double t;
+  double tol;
+  int max_restarts;
+  bool reorthogonalize;
+  ArnoldiParams(int m_val = 100, double t_val = 1.0, double tol_val = ARNOLDI_TOL,
+                int max_restarts_val = MAX_RESTARTS, bool reortho = true)
+      : m(m_val), t(t_val), tol(tol_val), max_restarts(max_restarts_val), reorthogonalize(reortho) {}
+};
+
+// reference: main.cu:1232-1235
+enum class ExpmvMode { AdaptiveTimeStepping, RestartedKrylov };
+
+// reference: main.cu:1237-1250
+struct TimeSteppingParams {
+  double dt_init{0.0};
+  double dt_min{0.0};
+  double dt_max{0.0};
+  double tol_step{0.0};
+  int max_steps{0};
+  int max_rejects{20};
+  double safety{0.8};
+  double grow{1.5};
+  double shrink{0.5};
+  bool adapt_m{false};
+  int m_max{0};
+  int m_step{0};
+};
+
+namespace mek {
+
+// Additive: how sub-steps are chosen.
+enum class StepControl {
+  Reference = 0,  // the reference controller (main.cu:1425-1518)
+  Expokit = 1,    // Sidje 1998 (ACM TOMS 24(1)) step-size heuristic
+};
+
+// Additive solver options (defaults reproduce the reference exactly).
+struct SolverOptions {
+  StepControl control = StepControl::Reference;
+  // Second Gram-Schmidt pass: -1 follows ArnoldiParams::reorthogonalize
+  // (always / never); 2 = conditional (DGKS test ||w'|| < eta ||w||).
+  int gs_mode = -1;
+  double dgks_eta = 0.1;
+  bool values_f32 = false;  // store matrix values in FP32 on the device
+  bool verbose = true;      // print the reference's per-column and [STEP] lines
+  bool timing = false;      // HIP-event timing of every kernel launch
+  bool fuse_spmv_dot = true;  // first-pass dots inside the SpMV kernel
+};
+
+struct StepRecord {
+  double t, dt, err;
+  int m_used, accepts, rejects;
+};
+
+// Kernel classes for event timing.
+enum KernelKind : int { kKSpmv = 0, kKUpdateDot, kKUpdateNorm, kKDots, kKFinal, kKExpm, kKLift, kKOther, kNumKinds };
+
+struct KernelTiming {
+  double ms = 0.0;        // summed event time
+  long long launches = 0;
+  double bytes = 0.0;     // summed algorithmic bytes
+};
+
+struct SolveStats {
+  int attempts = 0;
+  int accepted = 0;
+  int rejected = 0;
+  int spmv_count = 0;       // Krylov columns built (SpMV launches of the basis)
+  int extra_spmv = 0;       // Expokit ||A v_{m+1}|| products
+  int reorth_columns = 0;
+  double t_done = 0.0;
+  double solve_ms = 0.0;    // the METRIC total_compute_expmv_ms span
+  double err_sum = 0.0;     // Expokit: summed local error estimates
+  bool aborted = false;
+  std::vector<StepRecord> steps;
+  KernelTiming kernels[kNumKinds];
+};
+
+}  // namespace mek
+
+struct ArnoldiRunner {
+  int num_gpus;
+  std::vector<DeviceContext> device_contexts;
+  ArnoldiParams params;
+  ExpmvMode mode{ExpmvMode::AdaptiveTimeStepping};
+  TimeSteppingParams ts_params;
+  bool converged = false;
+  int restarts_done = 0;
+  double last_residual = 0.0;
+  double current_segment_t = 0.0;
+  std::vector<CSRHost::GhostMap> ghost_maps;
+  double v_norm = 0.0;
+  int global_rows;
+  int global_cols;
+  long long global_nnz;
+
+  // Additive
+  mek::SolverOptions options;
+  mek::SolveStats stats;
+  double anorm_inf = 0.0;   // ||A||_inf, for the Expokit initial step
+
+  ArnoldiRunner(int n_gpus, const ArnoldiParams& arnoldi_params, int total_rows, int total_cols,
+                long long total_nnz);
+
+  // reference: main.cu:1308-1409.  One GPU: device_ids.size() must be 1.
+  void init_all_devices(const std::vector<CSRHost>& partitions, const std::vector<int>& device_ids,
+                        const std::vector<CSRHost::GhostMap>& ghost_maps_in);
+  // reference: main.cu:1411-1571.  Host v in, host y out (y pre-sized).
+  void compute_expmv(const std::vector<double>& v_host, std::vector<double>& y_host);
+  void destroy();
+
+  // Additive: device-resident input and output (device pointers of n doubles).
+  void compute_expmv_device(const double* d_v, double* d_y);
+  // Additive: one Arnoldi decomposition from v (no time stepping): fills
+  // H ((m+1) x m, column-major) and returns the columns built.
+  int arnoldi_decomposition(const std::vector<double>& v_host, std::vector<double>& H_out);
+
+ private:
+  void solve_on_device(const double* d_v_or_null, const double* h_v_or_null);
+  void run_columns(int m_cols);
+  int read_status();
+  void begin_timed(int kind);
+  void end_timed(int kind, double bytes);
+  void flush_timing();
+  void init_q1_device(const double* d_v, const double* h_v);
+  void launch_expm(double dt, int mode);
+  void lift(int ncoef, double skip_below);
+  void restart_from_y();
+  void reference_controller();
+  void expokit_controller();
+
+  struct EventPair { hipEvent_t a, b; int kind; double bytes; };
+  std::vector<EventPair> ev_pool_;
+  size_t ev_used_ = 0;
+};

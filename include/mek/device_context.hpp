@@ -1,0 +1,72 @@
+// DeviceContext: one GPU's resources for the Arnoldi solve.
+// reference: struct DeviceContext, main.cu:747-1125 (handles, streams, device
+// CSR, V_m, work vectors, scalar scratch, SpMV buffer, destroy()).
+//
+// MI355X layout: the basis V is column-major with a leading dimension padded
+// to 64 doubles (512 B), holding m+2 columns; values are stored as FP64 or, in
+// the mixed form, FP32 (x, y and every accumulation stay FP64).  One HIP
+// stream.  There are no vendor BLAS/sparse handles: every kernel is this
+// project's own.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mek/csr_host.hpp"
+
+namespace mek {
+struct ArnoldiBufs;
+}
+
+struct DeviceContext {
+  int device_id;
+  hipStream_t stream_compute = nullptr;
+
+  // device CSR
+  int* d_row_ptr = nullptr;
+  int* d_col_idx = nullptr;
+  void* d_values = nullptr;       // double, or float when values_f32
+  bool values_f32 = false;
+  int* d_row_blocks = nullptr;    // SpMV row blocks (the "SpMV buffer")
+  int n_row_blocks = 0;
+
+  // Krylov basis and work vectors
+  double* d_V_m = nullptr;        // ld x (allocated_m_max + 2)
+  long long ld = 0;
+  double* d_w = nullptr;
+  double* d_y = nullptr;
+  double* d_small = nullptr;      // colscale, h1, h2, H, wH, scalars (one block)
+  double* d_partials = nullptr;   // per-block partial sums
+  double* d_expm_ws = nullptr;    // global expm workspace for large m
+  double* h_status = nullptr;     // pinned host copy of the scalar slots
+  int grid = 0;                   // blocks of the streaming kernels
+
+  int local_rows = 0;
+  long long local_nnz = 0;
+  int allocated_m_max = 0;
+
+  explicit DeviceContext(int dev_id) : device_id(dev_id) {}
+
+  void init_handles();
+  void create_streams();
+  // reference: alloc_from (main.cu:862-897).  Additive last argument: store
+  // the values in FP32 (the caller decides whether that is lossless).
+  void alloc_from(const CSRHost& part, int global_cols, int m_alloc_max, bool store_f32 = false);
+  // reference: create_descriptors (main.cu:900-1088).  On one GPU the matrix
+  // has no off-diagonal block; this builds the adaptive SpMV row blocks.
+  void create_descriptors(const CSRHost& part, int global_cols, const CSRHost::GhostMap& ghost_map,
+                          int global_row_offset, int m);
+  void destroy();
+
+  // Additive: device buffer bundle handed to the kernels.
+  mek::ArnoldiBufs bufs() const;
+  size_t small_doubles() const;  // size of d_small
+};
+
+namespace mek {
+// Adaptive CSR row blocks: consecutive rows whose nnz sum to at most nnz_cap
+// (and at most row_cap rows); a row longer than nnz_cap is a block of its own.
+// Returns the n_blocks+1 block start rows.
+std::vector<int> build_row_blocks(const int* row_ptr, int n, int nnz_cap, int row_cap);
+}  // namespace mek

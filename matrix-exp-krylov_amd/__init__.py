@@ -1,0 +1,365 @@
+"""Python access to libmek.so (the MI355X Arnoldi exp(t*A)*v library) over its C ABI.
+
+The product is the C++/HIP library and the ``matrix_exp`` command line; this
+module is a thin ctypes layer used by the tests, ``bench.py`` and
+``__graft_entry__``.  It never falls back to a CPU path: if ``lib/libmek.so``
+is missing, :func:`lib` raises.
+
+The directory name is not a Python identifier, so load it with :func:`load`
+from ``__graft_entry__`` (or ``importlib`` by path).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmek.so")
+CLI_PATH = os.path.join(HERE, "bin", "matrix_exp")
+
+# kernel classes of mek_runner_kernel_time
+KERNEL_KINDS = {"spmv": 0, "update_dot": 1, "update_norm": 2, "dots": 3, "final": 4,
+                "expm": 5, "lift": 6}
+
+
+class MekError(RuntimeError):
+    pass
+
+
+def build(jobs: int = 8) -> None:
+    """Compile libmek.so and matrix_exp for gfx950 in-tree (hipcc)."""
+    subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}"], check=True)
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [("step_control", ctypes.c_int), ("gs_mode", ctypes.c_int),
+                ("dgks_eta", ctypes.c_double), ("values_f32", ctypes.c_int),
+                ("verbose", ctypes.c_int), ("timing", ctypes.c_int),
+                ("fuse_spmv_dot", ctypes.c_int)]
+
+
+class ArnoldiParams(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int), ("t", ctypes.c_double), ("tol", ctypes.c_double),
+                ("max_restarts", ctypes.c_int), ("reorthogonalize", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("restarts_done", ctypes.c_int),
+                ("last_residual", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("attempts", ctypes.c_int), ("accepted", ctypes.c_int),
+                ("rejected", ctypes.c_int), ("spmv_count", ctypes.c_int),
+                ("extra_spmv", ctypes.c_int), ("reorth_columns", ctypes.c_int),
+                ("aborted", ctypes.c_int), ("solve_ms", ctypes.c_double)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# exported entry points and their (restype, argtypes); the header include/mek.h
+# declares exactly these
+SIGNATURES = {
+    "mek_last_error": (ctypes.c_char_p, []),
+    "mek_version": (ctypes.c_char_p, []),
+    "mek_device_count": (ctypes.c_int, [_ip]),
+    "mek_device_synchronize": (ctypes.c_int, []),
+    "mek_malloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "mek_free": (ctypes.c_int, [_vp]),
+    "mek_memcpy_htod": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "mek_memcpy_dtoh": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "mek_memcpy_dtod": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "mek_csr_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _ip, _ip, _dp,
+                                      ctypes.POINTER(_vp)]),
+    "mek_csr_load_bincsr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "mek_csr_load_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "mek_csr_info": (ctypes.c_int, [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
+    "mek_csr_export": (ctypes.c_int, [_vp, _ip, _ip, _dp]),
+    "mek_csr_destroy": (None, [_vp]),
+    "mek_solver_options_default": (None, [ctypes.POINTER(SolverOptions)]),
+    "mek_runner_create": (ctypes.c_int, [_vp, ctypes.POINTER(ArnoldiParams),
+                                         ctypes.POINTER(SolverOptions), ctypes.c_int,
+                                         ctypes.POINTER(_vp)]),
+    "mek_runner_expmv": (ctypes.c_int, [_vp, _dp, _dp]),
+    "mek_runner_expmv_device": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mek_runner_set_t": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "mek_runner_stats": (ctypes.c_int, [_vp, ctypes.POINTER(SolveStats)]),
+    "mek_runner_steps": (ctypes.c_int, [_vp, _dp, _ip, ctypes.c_int, _ip]),
+    "mek_runner_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, _dp,
+                                              ctypes.POINTER(ctypes.c_longlong), _dp]),
+    "mek_runner_arnoldi": (ctypes.c_int, [_vp, _dp, _dp, _ip]),
+    "mek_runner_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "mek_runner_destroy": (None, [_vp]),
+    "mek_spmv_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp,
+                                            ctypes.c_int, _ip, ctypes.POINTER(_vp)]),
+    "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "mek_spmv_plan_destroy": (None, [_vp]),
+    "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
+}
+
+
+def lib():
+    """The loaded libmek.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MekError(f"{LIB_PATH} is missing: build it first (make -C matrix-exp-krylov_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise MekError(f"mek status {status}: {lib().mek_last_error().decode()}")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+# ------------------------------------------------------------------ device
+
+class DeviceBuffer:
+    """Raw device allocation (hipMalloc) owned by Python."""
+
+    def __init__(self, nbytes: int):
+        p = _vp()
+        _check(lib().mek_malloc(ctypes.byref(p), int(nbytes)))
+        self.ptr = p
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _check(lib().mek_memcpy_htod(b.ptr, a.ctypes.data_as(_vp), a.nbytes))
+        return b
+
+    def to_array(self, dtype=np.float64, count=None) -> np.ndarray:
+        count = self.nbytes // np.dtype(dtype).itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        _check(lib().mek_memcpy_dtoh(out.ctypes.data_as(_vp), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mek_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(lib().mek_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def synchronize() -> None:
+    _check(lib().mek_device_synchronize())
+
+
+# ------------------------------------------------------------------ CSR
+
+class CSR:
+    """CSRHost handle (host memory)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def from_arrays(cls, row_ptr, col_idx, values, rows=None, cols=None):
+        rp, ci, va = _i32(row_ptr), _i32(col_idx), _f64(values)
+        rows = rp.size - 1 if rows is None else rows
+        cols = rows if cols is None else cols
+        h = _vp()
+        _check(lib().mek_csr_create(rows, cols, ci.size, _pi(rp), _pi(ci), _pd(va), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_bincsr(cls, path):
+        h = _vp()
+        _check(lib().mek_csr_load_bincsr(os.fsencode(path), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_mtx(cls, path):
+        h = _vp()
+        _check(lib().mek_csr_load_mtx(os.fsencode(path), ctypes.byref(h)))
+        return cls(h)
+
+    def info(self):
+        r, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        z = ctypes.c_longlong()
+        _check(lib().mek_csr_info(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), ctypes.byref(d)))
+        return {"rows": r.value, "cols": c.value, "nnz": z.value, "value_dtype": d.value}
+
+    def arrays(self):
+        i = self.info()
+        rp = np.empty(i["rows"] + 1, np.int32)
+        ci = np.empty(i["nnz"], np.int32)
+        va = np.empty(i["nnz"], np.float64)
+        _check(lib().mek_csr_export(self.h, _pi(rp), _pi(ci), _pd(va)))
+        return rp, ci, va
+
+    def close(self):
+        if self.h:
+            lib().mek_csr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ runner
+
+class Runner:
+    """ArnoldiRunner on one GPU: y = exp(t*A)*v."""
+
+    def __init__(self, A: CSR, m=30, t=1.0, tol=1e-8, max_restarts=10, reorthogonalize=True,
+                 step_control="reference", gs_mode=-1, dgks_eta=0.1, values_f32=False,
+                 verbose=False, timing=False, fuse_spmv_dot=True, device=0):
+        p = ArnoldiParams(int(m), float(t), float(tol), int(max_restarts), 1 if reorthogonalize else 0)
+        o = SolverOptions()
+        lib().mek_solver_options_default(ctypes.byref(o))
+        o.step_control = {"reference": 0, "expokit": 1}[step_control]
+        o.gs_mode = int(gs_mode)
+        o.dgks_eta = float(dgks_eta)
+        o.values_f32 = 1 if values_f32 else 0
+        o.verbose = 1 if verbose else 0
+        o.timing = 1 if timing else 0
+        o.fuse_spmv_dot = 1 if fuse_spmv_dot else 0
+        h = _vp()
+        _check(lib().mek_runner_create(A.h, ctypes.byref(p), ctypes.byref(o), int(device), ctypes.byref(h)))
+        self.h = h
+        self.n = A.info()["rows"]
+        self.m = int(m)
+
+    def expmv(self, v):
+        v = _f64(v)
+        y = np.zeros(self.n, np.float64)
+        _check(lib().mek_runner_expmv(self.h, _pd(v), _pd(y)))
+        return y
+
+    def expmv_device(self, d_v: DeviceBuffer, d_y: DeviceBuffer):
+        _check(lib().mek_runner_expmv_device(self.h, d_v.ptr, d_y.ptr))
+
+    def set_t(self, t):
+        _check(lib().mek_runner_set_t(self.h, float(t)))
+
+    def stats(self):
+        s = SolveStats()
+        _check(lib().mek_runner_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in SolveStats._fields_}
+
+    def steps(self, cap=100000):
+        n = ctypes.c_int()
+        _check(lib().mek_runner_steps(self.h, None, None, 0, ctypes.byref(n)))
+        k = min(n.value, cap)
+        tde = np.zeros(3 * max(k, 1), np.float64)
+        mar = np.zeros(3 * max(k, 1), np.int32)
+        _check(lib().mek_runner_steps(self.h, _pd(tde), _pi(mar), k, ctypes.byref(n)))
+        return [{"t": tde[3 * i], "dt": tde[3 * i + 1], "err": tde[3 * i + 2], "m_used": int(mar[3 * i]),
+                 "accepts": int(mar[3 * i + 1]), "rejects": int(mar[3 * i + 2])} for i in range(k)]
+
+    def kernel_time(self, kind):
+        ms, nb = ctypes.c_double(), ctypes.c_double()
+        nl = ctypes.c_longlong()
+        _check(lib().mek_runner_kernel_time(self.h, KERNEL_KINDS[kind], ctypes.byref(ms), ctypes.byref(nl),
+                                            ctypes.byref(nb)))
+        return {"ms": ms.value, "launches": nl.value, "bytes": nb.value}
+
+    def arnoldi(self, v):
+        v = _f64(v)
+        H = np.zeros((self.m + 1) * self.m, np.float64)
+        k = ctypes.c_int()
+        _check(lib().mek_runner_arnoldi(self.h, _pd(v), _pd(H), ctypes.byref(k)))
+        return H.reshape(self.m, self.m + 1).T.copy(), k.value
+
+    def close(self):
+        if self.h:
+            lib().mek_runner_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ kernel level
+
+class SpmvPlan:
+    """Device CSR + adaptive row blocks; execute() computes y = A x."""
+
+    def __init__(self, row_ptr, col_idx, values, values_f32=False):
+        rp = _i32(row_ptr)
+        self.n = rp.size - 1
+        self.nnz = int(rp[-1])
+        vals = np.ascontiguousarray(values, dtype=np.float32 if values_f32 else np.float64)
+        self.d_rp = DeviceBuffer.from_array(rp)
+        self.d_ci = DeviceBuffer.from_array(_i32(col_idx))
+        self.d_va = DeviceBuffer.from_array(vals)
+        h = _vp()
+        _check(lib().mek_spmv_plan_create(self.n, self.nnz, self.d_rp.ptr, self.d_ci.ptr, self.d_va.ptr,
+                                          1 if values_f32 else 0, _pi(rp), ctypes.byref(h)))
+        self.h = h
+
+    def execute(self, d_x: DeviceBuffer, d_y: DeviceBuffer, stream=None):
+        _check(lib().mek_spmv_execute(self.h, d_x.ptr, d_y.ptr, stream))
+
+    def __call__(self, x):
+        d_x = DeviceBuffer.from_array(_f64(x))
+        d_y = DeviceBuffer(8 * self.n)
+        self.execute(d_x, d_y)
+        synchronize()
+        return d_y.to_array()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mek_spmv_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def expm_dense(A):
+    """exp(A) of a small dense matrix on the device (one workgroup)."""
+    A = np.asarray(A, dtype=np.float64)
+    k = A.shape[0]
+    d_a = DeviceBuffer.from_array(np.asfortranarray(A).ravel(order="F"))
+    d_e = DeviceBuffer(8 * k * k)
+    _check(lib().mek_expm_dense(k, d_a.ptr, d_e.ptr))
+    return d_e.to_array().reshape(k, k, order="F")

@@ -1,0 +1,390 @@
+// extern "C" boundary of libmek.so (declared in include/mek.h).  Every entry
+// point catches exceptions and returns a status; the message is kept per
+// thread for mek_last_error().
+#include "mek.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host/hip_check.hpp"
+#include "kernels/launch.hpp"
+#include "mek/arnoldi_runner.hpp"
+#include "mek/csr_host.hpp"
+
+struct mek_csr {
+  CSRHost A;
+};
+
+struct mek_runner {
+  std::unique_ptr<ArnoldiRunner> R;
+};
+
+struct mek_spmv_plan {
+  mek::ArnoldiBufs b{};
+  int* d_blocks = nullptr;
+  double* d_partials = nullptr;
+  int* d_iscal = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    g_err.clear();
+    f();
+    return MEK_OK;
+  } catch (const mek::LoadError& e) {
+    return set_err(MEK_E_LOAD, e.what());
+  } catch (const mek::Error& e) {
+    const std::string m = e.what();
+    return set_err(m.rfind("HIP error", 0) == 0 ? MEK_E_HIP : MEK_E_SOLVER, m);
+  } catch (const std::bad_alloc&) {
+    return set_err(MEK_E_INTERNAL, "out of host memory");
+  } catch (const std::exception& e) {
+    return set_err(MEK_E_INTERNAL, e.what());
+  } catch (...) {
+    return set_err(MEK_E_INTERNAL, "unknown error");
+  }
+}
+
+void need(bool ok, const char* what) {
+  if (!ok) throw mek::Error(std::string("invalid argument: ") + what);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mek_last_error(void) { return g_err.c_str(); }
+const char* mek_version(void) { return "mek 0.1 (gfx950)"; }
+
+int mek_device_count(int* count) {
+  return guarded([&] {
+    need(count, "count");
+    MEK_HIP(hipGetDeviceCount(count));
+  });
+}
+int mek_device_synchronize(void) { return guarded([&] { MEK_HIP(hipDeviceSynchronize()); }); }
+int mek_malloc(void** p, size_t bytes) {
+  return guarded([&] {
+    need(p, "ptr");
+    MEK_HIP(hipMalloc(p, bytes ? bytes : 1));
+  });
+}
+int mek_free(void* p) { return guarded([&] { if (p) MEK_HIP(hipFree(p)); }); }
+int mek_memcpy_htod(void* d, const void* s, size_t n) {
+  return guarded([&] { MEK_HIP(hipMemcpy(d, s, n, hipMemcpyHostToDevice)); });
+}
+int mek_memcpy_dtoh(void* d, const void* s, size_t n) {
+  return guarded([&] { MEK_HIP(hipMemcpy(d, s, n, hipMemcpyDeviceToHost)); });
+}
+int mek_memcpy_dtod(void* d, const void* s, size_t n) {
+  return guarded([&] { MEK_HIP(hipMemcpy(d, s, n, hipMemcpyDeviceToDevice)); });
+}
+
+// ------------------------------------------------------------------ CSR
+
+int mek_csr_create(int rows, int cols, long long nnz, const int* row_ptr, const int* col_idx,
+                   const double* values, mek_csr** out) {
+  return guarded([&] {
+    need(out && row_ptr && rows >= 0 && cols >= 0 && nnz >= 0, "csr arrays");
+    need(nnz == 0 || (col_idx && values), "csr arrays");
+    need(row_ptr[0] == 0 && row_ptr[rows] == nnz, "row_ptr[0] == 0 and row_ptr[rows] == nnz");
+    for (int r = 0; r < rows; ++r) need(row_ptr[r] <= row_ptr[r + 1], "row_ptr non-decreasing");
+    for (long long k = 0; k < nnz; ++k) need(col_idx[k] >= 0 && col_idx[k] < cols, "column index in range");
+    auto h = std::make_unique<mek_csr>();
+    h->A.rows = rows;
+    h->A.cols = cols;
+    h->A.nnz = nnz;
+    h->A.row_ptr.assign(row_ptr, row_ptr + rows + 1);
+    h->A.col_idx.assign(col_idx, col_idx + nnz);
+    h->A.values.assign(values, values + nnz);
+    *out = h.release();
+  });
+}
+
+int mek_csr_load_bincsr(const char* path, mek_csr** out) {
+  return guarded([&] {
+    need(path && out, "path");
+    auto h = std::make_unique<mek_csr>();
+    h->A.load_binary_csr(path);
+    *out = h.release();
+  });
+}
+
+int mek_csr_load_mtx(const char* path, mek_csr** out) {
+  return guarded([&] {
+    need(path && out, "path");
+    auto h = std::make_unique<mek_csr>();
+    h->A.load_matrix_market(path);
+    *out = h.release();
+  });
+}
+
+int mek_csr_info(const mek_csr* A, int* rows, int* cols, long long* nnz, int* value_dtype) {
+  return guarded([&] {
+    need(A, "csr");
+    if (rows) *rows = A->A.rows;
+    if (cols) *cols = A->A.cols;
+    if (nnz) *nnz = A->A.nnz;
+    if (value_dtype) *value_dtype = A->A.source_value_dtype;
+  });
+}
+
+int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values) {
+  return guarded([&] {
+    need(A, "csr");
+    if (row_ptr) std::memcpy(row_ptr, A->A.row_ptr.data(), sizeof(int) * A->A.row_ptr.size());
+    if (col_idx) std::memcpy(col_idx, A->A.col_idx.data(), sizeof(int) * A->A.col_idx.size());
+    if (values) std::memcpy(values, A->A.values.data(), sizeof(double) * A->A.values.size());
+  });
+}
+
+void mek_csr_destroy(mek_csr* A) { delete A; }
+
+// --------------------------------------------------------------- runner
+
+void mek_solver_options_default(mek_solver_options* o) {
+  if (!o) return;
+  const mek::SolverOptions d;
+  o->step_control = static_cast<int>(d.control);
+  o->gs_mode = d.gs_mode;
+  o->dgks_eta = d.dgks_eta;
+  o->values_f32 = d.values_f32;
+  o->verbose = d.verbose;
+  o->timing = d.timing;
+  o->fuse_spmv_dot = d.fuse_spmv_dot;
+}
+
+int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_solver_options* o,
+                      int device_id, mek_runner** out) {
+  return guarded([&] {
+    need(A && p && out, "runner arguments");
+    need(device_id >= 0, "device_id");
+    ArnoldiParams params(p->m, p->t, p->tol, p->max_restarts, p->reorthogonalize != 0);
+    auto h = std::make_unique<mek_runner>();
+    h->R = std::make_unique<ArnoldiRunner>(1, params, A->A.rows, A->A.cols, A->A.nnz);
+    if (o) {
+      mek::SolverOptions& so = h->R->options;
+      need(o->step_control == 0 || o->step_control == 1, "step_control");
+      need(o->gs_mode >= -1 && o->gs_mode <= 2, "gs_mode");
+      so.control = static_cast<mek::StepControl>(o->step_control);
+      so.gs_mode = o->gs_mode;
+      so.dgks_eta = o->dgks_eta;
+      so.values_f32 = o->values_f32 != 0;
+      so.verbose = o->verbose != 0;
+      so.timing = o->timing != 0;
+      so.fuse_spmv_dot = o->fuse_spmv_dot != 0;
+    }
+    std::vector<CSRHost> parts{A->A};
+    std::vector<int> ids{device_id};
+    std::vector<CSRHost::GhostMap> maps(1);
+    try {
+      h->R->init_all_devices(parts, ids, maps);
+    } catch (...) {
+      h->R->destroy();
+      throw;
+    }
+    *out = h.release();
+  });
+}
+
+int mek_runner_expmv(mek_runner* R, const double* v, double* y) {
+  return guarded([&] {
+    need(R && v && y, "runner, v, y");
+    const int n = R->R->global_rows;
+    std::vector<double> vh(v, v + n), yh(n, 0.0);
+    R->R->compute_expmv(vh, yh);
+    std::memcpy(y, yh.data(), sizeof(double) * n);
+  });
+}
+
+int mek_runner_expmv_device(mek_runner* R, const double* d_v, double* d_y) {
+  return guarded([&] {
+    need(R && d_v && d_y, "runner, d_v, d_y");
+    R->R->compute_expmv_device(d_v, d_y);
+  });
+}
+
+int mek_runner_set_t(mek_runner* R, double t) {
+  return guarded([&] {
+    need(R, "runner");
+    ArnoldiRunner& r = *R->R;
+    r.params.t = t;
+    r.ts_params.dt_max = t;
+    r.ts_params.dt_min = std::max(1e-12, t * 1e-6);
+    const double denom = (r.params.max_restarts > 0) ? r.params.max_restarts : 1;
+    r.ts_params.dt_init = t / denom;
+  });
+}
+
+int mek_runner_stats(const mek_runner* R, mek_solve_stats* o) {
+  return guarded([&] {
+    need(R && o, "runner, out");
+    const ArnoldiRunner& r = *R->R;
+    o->converged = r.converged ? 1 : 0;
+    o->restarts_done = r.restarts_done;
+    o->last_residual = r.last_residual;
+    o->t_done = r.stats.t_done;
+    o->attempts = r.stats.attempts;
+    o->accepted = r.stats.accepted;
+    o->rejected = r.stats.rejected;
+    o->spmv_count = r.stats.spmv_count;
+    o->extra_spmv = r.stats.extra_spmv;
+    o->reorth_columns = r.stats.reorth_columns;
+    o->aborted = r.stats.aborted ? 1 : 0;
+    o->solve_ms = r.stats.solve_ms;
+  });
+}
+
+int mek_runner_steps(const mek_runner* R, double* tde, int* mar, int cap, int* n) {
+  return guarded([&] {
+    need(R && n, "runner, n");
+    const auto& st = R->R->stats.steps;
+    *n = static_cast<int>(st.size());
+    for (int i = 0; i < cap && i < static_cast<int>(st.size()); ++i) {
+      if (tde) {
+        tde[3 * i] = st[i].t;
+        tde[3 * i + 1] = st[i].dt;
+        tde[3 * i + 2] = st[i].err;
+      }
+      if (mar) {
+        mar[3 * i] = st[i].m_used;
+        mar[3 * i + 1] = st[i].accepts;
+        mar[3 * i + 2] = st[i].rejects;
+      }
+    }
+  });
+}
+
+int mek_runner_kernel_time(const mek_runner* R, int kind, double* ms, long long* launches, double* bytes) {
+  return guarded([&] {
+    need(R && kind >= 0 && kind < mek::kNumKinds, "runner, kind");
+    const mek::KernelTiming& k = R->R->stats.kernels[kind];
+    if (ms) *ms = k.ms;
+    if (launches) *launches = k.launches;
+    if (bytes) *bytes = k.bytes;
+  });
+}
+
+int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used) {
+  return guarded([&] {
+    need(R && v && H, "runner, v, H");
+    const int n = R->R->global_rows;
+    std::vector<double> vh(v, v + n), Hh;
+    const int k = R->R->arnoldi_decomposition(vh, Hh);
+    std::memcpy(H, Hh.data(), sizeof(double) * Hh.size());
+    if (k_used) *k_used = k;
+  });
+}
+
+int mek_runner_stream(const mek_runner* R, void** stream) {
+  return guarded([&] {
+    need(R && stream, "runner, stream");
+    *stream = R->R->device_contexts[0].stream_compute;
+  });
+}
+
+void mek_runner_destroy(mek_runner* R) {
+  if (!R) return;
+  if (R->R) R->R->destroy();
+  delete R;
+}
+
+// ----------------------------------------------------------- kernel level
+
+int mek_spmv_plan_create(int n, long long nnz, const int* d_row_ptr, const int* d_col_idx,
+                         const void* d_vals, int vals_f32, const int* h_row_ptr, mek_spmv_plan** out) {
+  return guarded([&] {
+    need(out && d_row_ptr && h_row_ptr && n >= 0 && nnz >= 0, "plan arguments");
+    need(nnz == 0 || (d_col_idx && d_vals), "plan arrays");
+    need(h_row_ptr[n] == nnz, "h_row_ptr[n] == nnz");
+    auto P = std::make_unique<mek_spmv_plan>();
+    const std::vector<int> blocks = mek::build_row_blocks(h_row_ptr, n, mek::kSpmvNnzCap, mek::kSpmvRowCap);
+    int grid = static_cast<int>(std::min<long long>(1024, std::max<long long>(8, (n + 1023) / 1024)));
+    grid = (grid + 7) / 8 * 8;
+    MEK_HIP(hipMalloc(&P->d_blocks, sizeof(int) * blocks.size()));
+    MEK_HIP(hipMemcpy(P->d_blocks, blocks.data(), sizeof(int) * blocks.size(), hipMemcpyHostToDevice));
+    MEK_HIP(hipMalloc(&P->d_partials, sizeof(double) * grid));
+    MEK_HIP(hipMalloc(&P->d_iscal, sizeof(int) * mek::kNumInts));
+    MEK_HIP(hipMemset(P->d_iscal, 0, sizeof(int) * mek::kNumInts));
+    mek::ArnoldiBufs& b = P->b;
+    b.n = n;
+    b.row_ptr = d_row_ptr;
+    b.col_idx = d_col_idx;
+    b.vals = d_vals;
+    b.vals_f32 = vals_f32 ? 1 : 0;
+    b.row_blocks = P->d_blocks;
+    b.n_row_blocks = static_cast<int>(blocks.size()) - 1;
+    b.iscal = P->d_iscal;
+    b.partials = P->d_partials;
+    b.grid = grid;
+    *out = P.release();
+  });
+}
+
+int mek_spmv_execute(mek_spmv_plan* P, const double* d_x, double* d_y, void* stream) {
+  return guarded([&] {
+    need(P && d_x && d_y, "plan, x, y");
+    mek::launch_spmv_plain(P->b, d_x, nullptr, d_y, static_cast<hipStream_t>(stream));
+    MEK_HIP_LAUNCHED();
+  });
+}
+
+void mek_spmv_plan_destroy(mek_spmv_plan* P) {
+  if (!P) return;
+  if (P->d_blocks) (void)hipFree(P->d_blocks);
+  if (P->d_partials) (void)hipFree(P->d_partials);
+  if (P->d_iscal) (void)hipFree(P->d_iscal);
+  delete P;
+}
+
+int mek_expm_dense(int k, const double* d_A, double* d_E) {
+  return guarded([&] {
+    need(k >= 1 && k <= mek::kMaxKrylov + 2 && d_A && d_E, "k in [1, 502], A, E");
+    double* d_scr = nullptr;
+    double* d_ws = nullptr;
+    MEK_HIP(hipMalloc(&d_scr, sizeof(double) * (32 + k + 4)));
+    if (k > mek::kExpmMaxKLds) {
+      const hipError_t e = hipMalloc(&d_ws, sizeof(double) * 6 * static_cast<size_t>(k) * k);
+      if (e != hipSuccess) {
+        (void)hipFree(d_scr);
+        MEK_HIP(e);
+      }
+    }
+    mek::ExpmArgs a{};
+    a.H = d_A;
+    a.ldh = k;
+    a.iscal = reinterpret_cast<int*>(d_scr + 16);
+    a.iscal_out = reinterpret_cast<int*>(d_scr + 16);
+    a.scal = d_scr;
+    a.out = d_scr + 24;
+    a.out_len = k;
+    a.dt = 1.0;
+    a.mode = 2;  // plain exponential: no residual
+    a.ws = d_ws;
+    a.k_override = k;
+    a.breakdown_override = 0;
+    a.full_out = d_E;
+    mek::launch_expm(a, k, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d_scr);
+    if (d_ws) (void)hipFree(d_ws);
+    MEK_HIP(e);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,499 @@
+// ArnoldiRunner: the adaptive time-stepping Arnoldi exp(t*A)*v solve on one
+// MI355X.  reference: main.cu:1252-2081 (controller main.cu:1411-1571).
+//
+// Per step attempt the host enqueues all m columns (three sweeps and three
+// finalisers each, see kernels/arnoldi.hip), the on-device Pade exp(dt*H)e1
+// with the residual estimate, and the lift, then reads back one small status
+// block (err, k_used, breakdown, ||y||) and runs the reference's accept /
+// reject logic.  A rejected attempt keeps its Krylov basis: the basis depends
+// only on the start vector, so recomputing it (as the reference does) would
+// reproduce the same V and H; only exp(dt*H) and the lift are redone.
+#include "mek/arnoldi_runner.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+#include <limits>
+
+#include "../kernels/launch.hpp"
+#include "hip_check.hpp"
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// Round up to two significant digits, as Expokit does for step sizes.
+double round_step(double x) {
+  const double s = std::pow(10.0, std::floor(std::log10(x)) - 1.0);
+  return std::ceil(x / s) * s;
+}
+
+}  // namespace
+
+ArnoldiRunner::ArnoldiRunner(int n_gpus, const ArnoldiParams& p, int total_rows, int total_cols,
+                             long long total_nnz)
+    : num_gpus(n_gpus), params(p), global_rows(total_rows), global_cols(total_cols),
+      global_nnz(total_nnz) {
+  device_contexts.reserve(num_gpus);
+  for (int i = 0; i < num_gpus; ++i) device_contexts.emplace_back(i);
+  // reference: main.cu:1292-1305
+  ts_params.dt_max = params.t;
+  ts_params.dt_min = std::max(1e-12, params.t * 1e-6);
+  const double denom = (params.max_restarts > 0) ? params.max_restarts : 1;
+  ts_params.dt_init = params.t / denom;
+  ts_params.tol_step = params.tol;
+  ts_params.max_steps = (params.max_restarts > 0) ? params.max_restarts : 1000;
+  ts_params.max_rejects = 20;
+  ts_params.safety = 0.8;
+  ts_params.grow = 1.5;
+  ts_params.shrink = 0.5;
+  ts_params.adapt_m = false;
+  ts_params.m_max = params.m;
+  ts_params.m_step = 0;
+}
+
+void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
+                                     const std::vector<int>& device_ids,
+                                     const std::vector<CSRHost::GhostMap>& ghost_maps_in) {
+  ghost_maps = ghost_maps_in;
+  if (static_cast<int>(device_ids.size()) != num_gpus)
+    throw mek::Error("ERROR: device_ids size (" + std::to_string(device_ids.size()) +
+                     ") must match num_gpus (" + std::to_string(num_gpus) + ")");
+  if (num_gpus != 1)
+    throw mek::Error("ERROR: this build runs the solve on one GPU (num_gpus=" +
+                     std::to_string(num_gpus) + ")");
+  if (params.m >= global_rows)
+    throw mek::Error("ERROR: Requested Arnoldi m=" + std::to_string(params.m) +
+                     " must be less than global_rows=" + std::to_string(global_rows));
+  if (global_rows != global_cols)
+    throw mek::Error("ERROR: Matrix must be square. rows=" + std::to_string(global_rows) +
+                     " cols=" + std::to_string(global_cols));
+  if (params.m < 1 || params.m > mek::kMaxKrylov)
+    throw mek::Error("ERROR: Arnoldi m=" + std::to_string(params.m) + " outside [1, " +
+                     std::to_string(mek::kMaxKrylov) + "]");
+  DeviceContext& dc = device_contexts[0];
+  dc.device_id = device_ids[0];
+  dc.init_handles();
+  dc.create_streams();
+  const int alloc_m = std::max(params.m, ts_params.m_max > 0 ? ts_params.m_max : params.m);
+  dc.alloc_from(partitions[0], global_cols, alloc_m, options.values_f32);
+  static const CSRHost::GhostMap kEmpty;
+  dc.create_descriptors(partitions[0], global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0,
+                        alloc_m);
+  anorm_inf = partitions[0].norm_inf();
+}
+
+// ------------------------------------------------------------ timing
+
+void ArnoldiRunner::begin_timed(int kind) {
+  if (!options.timing) return;
+  if (ev_used_ == ev_pool_.size()) {
+    EventPair p{};
+    MEK_HIP(hipEventCreate(&p.a));
+    MEK_HIP(hipEventCreate(&p.b));
+    ev_pool_.push_back(p);
+  }
+  EventPair& p = ev_pool_[ev_used_];
+  p.kind = kind;
+  MEK_HIP(hipEventRecord(p.a, device_contexts[0].stream_compute));
+}
+
+void ArnoldiRunner::end_timed(int kind, double bytes) {
+  if (!options.timing) return;
+  EventPair& p = ev_pool_[ev_used_];
+  p.kind = kind;
+  p.bytes = bytes;
+  MEK_HIP(hipEventRecord(p.b, device_contexts[0].stream_compute));
+  ++ev_used_;
+}
+
+void ArnoldiRunner::flush_timing() {
+  if (!options.timing || ev_used_ == 0) return;
+  MEK_HIP(hipStreamSynchronize(device_contexts[0].stream_compute));
+  for (size_t i = 0; i < ev_used_; ++i) {
+    float ms = 0.f;
+    MEK_HIP(hipEventElapsedTime(&ms, ev_pool_[i].a, ev_pool_[i].b));
+    mek::KernelTiming& kt = stats.kernels[ev_pool_[i].kind];
+    kt.ms += ms;
+    kt.launches += 1;
+    kt.bytes += ev_pool_[i].bytes;
+  }
+  ev_used_ = 0;
+}
+
+// ------------------------------------------------------------ device steps
+
+int ArnoldiRunner::read_status() {
+  DeviceContext& dc = device_contexts[0];
+  const size_t bytes = sizeof(double) * (20 + dc.allocated_m_max + 4);
+  MEK_HIP(hipMemcpyAsync(dc.h_status, dc.d_small, bytes, hipMemcpyDeviceToHost, dc.stream_compute));
+  MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+  flush_timing();
+  return reinterpret_cast<const int*>(dc.h_status + 16)[mek::kKUsed];
+}
+
+void ArnoldiRunner::init_q1_device(const double* d_v, const double* h_v) {
+  DeviceContext& dc = device_contexts[0];
+  const mek::ArnoldiBufs b = dc.bufs();
+  hipStream_t s = dc.stream_compute;
+  const size_t bytes = sizeof(double) * dc.local_rows;
+  if (d_v) MEK_HIP(hipMemcpyAsync(dc.d_V_m, d_v, bytes, hipMemcpyDeviceToDevice, s));
+  else MEK_HIP(hipMemcpyAsync(dc.d_V_m, h_v, bytes, hipMemcpyHostToDevice, s));
+  mek::launch_norm2(b, dc.d_V_m, s);
+  mek::launch_fin_start(b, s);
+  MEK_HIP_LAUNCHED();
+}
+
+void ArnoldiRunner::run_columns(int m_cols) {
+  DeviceContext& dc = device_contexts[0];
+  const mek::ArnoldiBufs b = dc.bufs();
+  hipStream_t s = dc.stream_compute;
+  const double n = dc.local_rows;
+  const double bspmv = static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
+                       4.0 * (n + 1) + 16.0 * n;
+  const int gs_mode = (options.gs_mode >= 0) ? options.gs_mode : (params.reorthogonalize ? 1 : 0);
+  for (int j = 0; j < m_cols; ++j) {
+    const int nq = j + 1;
+    if (nq <= mek::kMaxFusedDots) {
+      if (options.fuse_spmv_dot) {
+        begin_timed(mek::kKSpmv);
+        mek::launch_spmv_dot(b, j, nq, nq, s);
+        end_timed(mek::kKSpmv, bspmv + 8.0 * n * nq);
+      } else {
+        begin_timed(mek::kKSpmv);
+        mek::launch_spmv_dot(b, j, 0, nq, s);
+        end_timed(mek::kKSpmv, bspmv);
+        begin_timed(mek::kKDots);
+        mek::launch_dots(b, dc.d_w, 0, nq, 0, 0, s);
+        end_timed(mek::kKDots, 8.0 * n * (nq + 1));
+      }
+      mek::launch_fin_h1(b, nq, s);
+      begin_timed(mek::kKUpdateDot);
+      mek::launch_update_dot(b, j, s);
+      end_timed(mek::kKUpdateDot, 8.0 * n * (nq + 2));
+    } else {
+      // generic path for j+1 > 64: dots in chunks of 64 columns
+      begin_timed(mek::kKSpmv);
+      mek::launch_spmv_dot(b, j, mek::kMaxFusedDots, nq, s);
+      end_timed(mek::kKSpmv, bspmv + 8.0 * n * mek::kMaxFusedDots);
+      for (int c0 = mek::kMaxFusedDots; c0 < nq; c0 += mek::kMaxFusedDots)
+        mek::launch_dots(b, dc.d_w, c0, std::min(mek::kMaxFusedDots, nq - c0), 0, c0, s);
+      mek::launch_fin_h1(b, nq, s);
+      double* vn = dc.d_V_m + static_cast<long long>(j + 1) * dc.ld;
+      mek::launch_update(b, dc.d_w, vn, b.h1, nq, s);
+      for (int c0 = 0; c0 < nq; c0 += mek::kMaxFusedDots) {
+        const int cnt = std::min(mek::kMaxFusedDots, nq - c0);
+        mek::launch_dots(b, vn, c0, cnt, (c0 + cnt == nq) ? 1 : 0, c0, s);
+      }
+    }
+    mek::launch_fin_h2(b, j, gs_mode, options.dgks_eta, s);
+    begin_timed(mek::kKUpdateNorm);
+    mek::launch_update_norm(b, j, s);
+    end_timed(mek::kKUpdateNorm, 8.0 * n * (nq + 2));
+    mek::launch_fin_col(b, j, s);
+    MEK_HIP_LAUNCHED();
+  }
+  stats.spmv_count += m_cols;  // launched; columns after a breakdown exit at once
+}
+
+void ArnoldiRunner::launch_expm(double dt, int mode) {
+  DeviceContext& dc = device_contexts[0];
+  const mek::ArnoldiBufs b = dc.bufs();
+  mek::ExpmArgs a{};
+  a.H = b.H;
+  a.ldh = b.ldh;
+  a.iscal = b.iscal;
+  a.iscal_out = b.iscal;
+  a.scal = b.scal;
+  a.out = b.wH;
+  a.out_len = dc.allocated_m_max + 4;
+  a.dt = dt;
+  a.mode = mode;
+  a.ws = dc.d_expm_ws;
+  a.k_override = 0;
+  a.breakdown_override = 0;
+  a.full_out = nullptr;
+  begin_timed(mek::kKExpm);
+  mek::launch_expm(a, dc.allocated_m_max + 2, dc.stream_compute);
+  end_timed(mek::kKExpm, 0.0);
+  MEK_HIP_LAUNCHED();
+}
+
+void ArnoldiRunner::lift(int ncoef, double skip_below) {
+  DeviceContext& dc = device_contexts[0];
+  const mek::ArnoldiBufs b = dc.bufs();
+  begin_timed(mek::kKLift);
+  mek::launch_lift(b, ncoef, skip_below, dc.stream_compute);
+  end_timed(mek::kKLift, 8.0 * dc.local_rows * (ncoef + 1));
+  mek::launch_fin_norm(b, mek::kYNorm, dc.stream_compute);
+  MEK_HIP_LAUNCHED();
+}
+
+void ArnoldiRunner::restart_from_y() {
+  DeviceContext& dc = device_contexts[0];
+  mek::launch_restart(dc.bufs(), dc.stream_compute);
+  MEK_HIP_LAUNCHED();
+}
+
+// ------------------------------------------------------------ controllers
+
+// reference: compute_expmv's adaptive time stepping, main.cu:1425-1518.
+void ArnoldiRunner::reference_controller() {
+  DeviceContext& dc = device_contexts[0];
+  const double total_t = params.t;
+  double dt = std::min(std::max(ts_params.dt_init, ts_params.dt_min), ts_params.dt_max);
+  double t_done = 0.0;
+  int accepted = 0;
+  converged = false;
+  bool basis_ready = false;
+  while (t_done < total_t && accepted < ts_params.max_steps) {
+    dt = std::min(dt, total_t - t_done);
+    int rejects = 0;
+    int m_current = params.m;
+    for (;;) {
+      if (m_current > dc.allocated_m_max)
+        throw mek::Error("ERROR: m_current exceeds allocated_m_max on GPU 0");
+      if (!basis_ready) {
+        run_columns(m_current);
+        basis_ready = true;
+      }
+      current_segment_t = dt;
+      launch_expm(dt, 0);
+      lift(m_current + 1, 1e-15);
+      const int k_used = read_status();
+      const double* sc = dc.h_status;
+      const int breakdown = reinterpret_cast<const int*>(sc + 16)[mek::kBreakdown];
+      const double err = sc[mek::kErr];  // 0 on breakdown (set on the device)
+      if (options.verbose) {
+        for (int j = 0; j < k_used; ++j)
+          std::cout << "    H_m column " << j << " stored. Current H_m size: " << (m_current + 1)
+                    << "x" << m_current << "\n";
+        std::cout << "[STEP] t=" << t_done << " dt=" << dt << " err=" << err << " m_used=" << k_used
+                  << " accepts=" << accepted << " rejects=" << rejects << std::endl;
+      }
+      stats.steps.push_back({t_done, dt, err, k_used, accepted, rejects});
+      stats.attempts++;
+      if (err <= ts_params.tol_step || breakdown) {
+        t_done += dt;
+        accepted++;
+        last_residual = err;
+        if (t_done >= total_t) {
+          converged = true;
+          break;
+        }
+        restart_from_y();
+        basis_ready = false;
+        if (err < 0.1 * ts_params.tol_step) dt = std::min(dt * ts_params.grow * ts_params.safety, ts_params.dt_max);
+        break;
+      }
+      rejects++;
+      stats.rejected++;
+      if (rejects >= ts_params.max_rejects) {
+        std::cerr << "[STEP] rejected too many times, aborting" << std::endl;
+        converged = false;
+        last_residual = err;
+        stats.aborted = true;
+        t_done = total_t;
+        break;
+      }
+      if (ts_params.adapt_m && (m_current + ts_params.m_step) <= ts_params.m_max && ts_params.m_step > 0) {
+        m_current += ts_params.m_step;
+        basis_ready = false;
+      } else {
+        dt = std::max(dt * ts_params.shrink * ts_params.safety, ts_params.dt_min);
+        m_current = params.m;
+      }
+    }
+  }
+  restarts_done = accepted;
+  stats.accepted = accepted;
+  stats.t_done = t_done;
+}
+
+// Sidje 1998 (Expokit, ACM TOMS 24(1):130-156) step-size control: local error
+// estimate from the augmented (m+2) x (m+2) exponential, err_loc <=
+// delta * t_step * tol per step, step sizes rounded to two digits.  The
+// Krylov basis is built once per step; a rejection only recomputes the small
+// exponential with a shorter step.
+void ArnoldiRunner::expokit_controller() {
+  DeviceContext& dc = device_contexts[0];
+  const mek::ArnoldiBufs b = dc.bufs();
+  const int m = params.m;
+  const double tol = params.tol;
+  const double t_out = std::abs(params.t);
+  const double anorm = (anorm_inf > 0.0) ? anorm_inf : 1.0;
+  const int mxrej = 10;
+  const double btol = 1.0e-7, gamma = 0.9, delta = 1.2;
+  const double rndoff = anorm * std::numeric_limits<double>::epsilon();
+  read_status();
+  double beta = dc.h_status[mek::kVNorm];
+  const double fact = std::pow((m + 1) / std::exp(1.0), m + 1) * std::sqrt(2.0 * M_PI * (m + 1));
+  double t_new = (1.0 / anorm) * std::pow((fact * tol) / (4.0 * beta * anorm), 1.0 / m);
+  t_new = round_step(t_new);
+  double t_now = 0.0, s_error = 0.0, err_loc = 0.0;
+  int nstep = 0;
+  converged = false;
+  const int max_steps = (params.max_restarts > 0) ? std::max(params.max_restarts, 100000) : 100000;
+  while (t_now < t_out && nstep < max_steps) {
+    ++nstep;
+    double t_step = std::min(t_out - t_now, t_new);
+    run_columns(m);
+    int k_used = read_status();
+    const int breakdown = reinterpret_cast<const int*>(dc.h_status + 16)[mek::kBreakdown];
+    int k1 = 2, mb = m;
+    double avnorm = 0.0;
+    if (breakdown) {
+      k1 = 0;
+      mb = k_used;
+      t_step = t_out - t_now;
+    } else {
+      // ||A v_{m+1}||
+      begin_timed(mek::kKSpmv);
+      mek::launch_spmv_plain(b, dc.d_V_m + static_cast<long long>(m) * dc.ld, b.colscale + m, dc.d_w,
+                             dc.stream_compute);
+      end_timed(mek::kKSpmv, static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
+                                 4.0 * (dc.local_rows + 1) + 16.0 * dc.local_rows);
+      mek::launch_fin_norm(b, mek::kAvNorm, dc.stream_compute);
+      MEK_HIP_LAUNCHED();
+      stats.extra_spmv++;
+    }
+    int ireject = 0;
+    double xm = 1.0 / m;
+    for (;;) {
+      launch_expm(t_step, 1);
+      read_status();
+      const double* F = dc.h_status + 20;  // first column of exp(t_step * H_aug)
+      if (!breakdown) avnorm = dc.h_status[mek::kAvNorm];
+      stats.attempts++;
+      if (k1 == 0) {
+        err_loc = btol;
+        break;
+      }
+      const double phi1 = std::abs(beta * F[m]);
+      const double phi2 = std::abs(beta * F[m + 1] * avnorm);
+      if (phi1 > 10.0 * phi2) {
+        err_loc = phi2;
+        xm = 1.0 / m;
+      } else if (phi1 > phi2) {
+        err_loc = (phi1 * phi2) / (phi1 - phi2);
+        xm = 1.0 / m;
+      } else {
+        err_loc = phi1;
+        xm = 1.0 / (m - 1);
+      }
+      if (err_loc <= delta * t_step * tol) break;
+      if (ireject == mxrej) throw mek::Error("ERROR: The requested tolerance is too high.");
+      t_step = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+      ++ireject;
+      stats.rejected++;
+    }
+    const int mx = mb + std::max(0, k1 - 1);
+    lift(mx, -1.0);
+    restart_from_y();
+    read_status();
+    beta = dc.h_status[mek::kVNorm];
+    if (options.verbose) {
+      std::cout << "[STEP] t=" << t_now << " dt=" << t_step << " err=" << err_loc << " m_used=" << mb
+                << " accepts=" << (nstep - 1) << " rejects=" << ireject << std::endl;
+    }
+    stats.steps.push_back({t_now, t_step, err_loc, mb, nstep - 1, ireject});
+    t_now += t_step;
+    t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+    err_loc = std::max(err_loc, rndoff);
+    s_error += err_loc;
+  }
+  converged = (t_now >= t_out);
+  stats.err_sum = s_error;
+  restarts_done = nstep;
+  last_residual = err_loc;
+  stats.accepted = nstep;
+  stats.t_done = t_now;
+}
+
+// ------------------------------------------------------------ entry points
+
+void ArnoldiRunner::solve_on_device(const double* d_v, const double* h_v) {
+  DeviceContext& dc = device_contexts[0];
+  MEK_HIP(hipSetDevice(dc.device_id));
+  stats = mek::SolveStats();
+  ev_used_ = 0;
+  if (mode != ExpmvMode::AdaptiveTimeStepping) {
+    std::cerr << "Expmv mode not implemented" << std::endl;
+    return;
+  }
+  init_q1_device(d_v, h_v);
+  if (options.control == mek::StepControl::Expokit) expokit_controller();
+  else reference_controller();
+  MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+  flush_timing();
+  stats.reorth_columns = reinterpret_cast<const int*>(dc.h_status + 16)[mek::kReorthCount];
+  v_norm = dc.h_status[mek::kVNorm];
+}
+
+void ArnoldiRunner::compute_expmv(const std::vector<double>& v_host, std::vector<double>& y_host) {
+  const double t0 = now_ms();
+  DeviceContext& dc = device_contexts[0];
+  if (static_cast<int>(v_host.size()) < dc.local_rows || static_cast<int>(y_host.size()) < dc.local_rows)
+    throw mek::Error("ERROR: v and y must hold " + std::to_string(dc.local_rows) + " entries");
+  solve_on_device(nullptr, v_host.data());
+  MEK_HIP(hipMemcpyAsync(y_host.data(), dc.d_y, sizeof(double) * dc.local_rows, hipMemcpyDeviceToHost,
+                         dc.stream_compute));
+  MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+  if (options.verbose) {
+    // reference's analytic FLOP model (main.cu:1538-1565)
+    const double N = global_rows, nnz = static_cast<double>(global_nnz), mv = params.m,
+                 S = restarts_done;
+    const double seg = 2.0 * nnz * mv + 4.0 * N * mv * (mv + 1.0) + 5.0 * N * mv + 2.0 * N;
+    const double tot = 3.0 * N * S + S * seg;
+    const double cseg = 2.0 * 1.0 * mv * mv * mv + 2.0 * mv * mv;
+    std::cout << "[FLOPs] N=" << N << " nnz=" << nnz << " m=" << mv << " steps=" << S << std::endl;
+    std::cout << "[FLOPs] GPU segment: " << seg << " | GPU total: " << tot << std::endl;
+    std::cout << "[FLOPs] CPU segment: " << cseg << " | CPU total: " << S * cseg << " (C_exp=" << 1.0
+              << ")" << std::endl;
+  }
+  stats.solve_ms = now_ms() - t0;
+  if (options.verbose) std::cout << "METRIC total_compute_expmv_ms=" << stats.solve_ms << std::endl;
+}
+
+void ArnoldiRunner::compute_expmv_device(const double* d_v, double* d_y) {
+  const double t0 = now_ms();
+  DeviceContext& dc = device_contexts[0];
+  solve_on_device(d_v, nullptr);
+  if (d_y != dc.d_y)
+    MEK_HIP(hipMemcpyAsync(d_y, dc.d_y, sizeof(double) * dc.local_rows, hipMemcpyDeviceToDevice,
+                           dc.stream_compute));
+  MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+  stats.solve_ms = now_ms() - t0;
+}
+
+int ArnoldiRunner::arnoldi_decomposition(const std::vector<double>& v_host, std::vector<double>& H_out) {
+  DeviceContext& dc = device_contexts[0];
+  MEK_HIP(hipSetDevice(dc.device_id));
+  stats = mek::SolveStats();
+  init_q1_device(nullptr, v_host.data());
+  run_columns(params.m);
+  const int k_used = read_status();
+  const mek::ArnoldiBufs b = dc.bufs();
+  std::vector<double> Hd(static_cast<size_t>(b.ldh) * b.ldh);
+  MEK_HIP(hipMemcpy(Hd.data(), b.H, sizeof(double) * Hd.size(), hipMemcpyDeviceToHost));
+  const int m = params.m;
+  H_out.assign(static_cast<size_t>(m + 1) * m, 0.0);
+  for (int c = 0; c < k_used; ++c)
+    for (int r = 0; r <= std::min(c + 1, m); ++r)
+      H_out[static_cast<size_t>(c) * (m + 1) + r] = Hd[static_cast<size_t>(c) * b.ldh + r];
+  return k_used;
+}
+
+void ArnoldiRunner::destroy() {
+  for (auto& p : ev_pool_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  ev_pool_.clear();
+  for (auto& dc : device_contexts) dc.destroy();
+}
