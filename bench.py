@@ -1,0 +1,213 @@
+"""Throughput of the MI355X Arnoldi exp(t*A)*v solver on one GPU.
+
+A step is one full solve y = exp(t*A)*v (ArnoldiRunner::compute_expmv on
+device-resident v and y).  Default workload: 2-D 5-point Laplacian on a
+4096 x 4096 grid (n = 16,777,216, nnz = 83,869,696, FP64 values, i32
+indices), v ~ N(0,1) normalised (seed 0), t = 10, m = 40, tol 1e-8, sub-steps
+by the Expokit heuristic.
+
+Prints one JSON line with solves/s, the relative error against the DST-I closed
+form, a roofline object for the dominant kernel (algorithmic bytes per launch /
+HIP-event launch time, on the solver's own stream), the SpMV kernel's HBM
+GB/s, and a CPU baseline: the oracle's single-thread Arnoldi column loop timed
+on a bounded sample of the same matrix and extrapolated to the same solve.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM_GBS = 8000.0
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+WORKLOADS = {
+    "laplacian4096": dict(kind="laplacian", N=4096, t=10.0, m=40, tol=1e-8, f32=False),
+    "convdiff256_f32": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=True),
+    "convdiff256_f64": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=False),
+}
+
+
+def reference_solution(w, v, gen):
+    import scipy.fft  # noqa: F401
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from closed_forms import kron3_expmv, laplacian_2d_expmv
+    if w["kind"] == "laplacian":
+        return laplacian_2d_expmv(w["N"], w["t"], v)
+    (lx, dx, ux), (ly, dy, uy), (lz, dz, uz) = gen.convdiff_3d_coeffs()
+    N = w["N"]
+    return kron3_expmv(gen.tridiag_1d(N, lz, dz, uz), gen.tridiag_1d(N, ly, dy, uy),
+                       gen.tridiag_1d(N, lx, dx, ux), w["t"], v)
+
+
+def cpu_baseline(rp, ci, va, v, columns_per_solve, m, n, seconds_budget):
+    """Time the oracle's reference Arnoldi loop (SpMV + MGS2 + normalise, one
+    thread) on the first columns of the same matrix and extrapolate linearly in
+    the basis size to the columns the GPU solve built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    # pick a column count that fits the time budget (first call is a probe)
+    probe = pyoracle.arnoldi_columns(rp, ci, va, v, 2)
+    per_col0 = probe.mean()
+    ncols = int(max(3, min(12, seconds_budget * 1000.0 / max(per_col0 * 3.0, 1e-3))))
+    t0 = time.time()
+    col_ms = pyoracle.arnoldi_columns(rp, ci, va, v, ncols)
+    wall = time.time() - t0
+    j = np.arange(1, col_ms.size + 1, dtype=float)
+    b, a = np.polyfit(j, col_ms, 1)  # ms per column = a + b * (basis vectors)
+    per_step_ms = sum(a + b * (k + 1) for k in range(m))
+    lift_ms = (m + 1) * b / 4.0  # one axpy per basis vector vs. two dots + two axpys
+    solve_ms = columns_per_solve / m * (per_step_ms + lift_ms)
+    return {
+        "value": 1000.0 / solve_ms,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle Arnoldi loop (SpMV + MGS2 + normalise, main_cpu.cpp:486-546 restated in C) "
+                   f"for the first {col_ms.size} columns of the same matrix and start vector "
+                   f"({wall:.1f} s wall), fitted as {a:.1f} + {b:.1f}*(j+1) ms per column and "
+                   f"extrapolated to the {columns_per_solve} columns (+ lifts) of one GPU solve"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS))
+    ap.add_argument("--step-control", default="expokit", choices=["expokit", "reference"])
+    ap.add_argument("--gs", default="always", choices=["always", "dgks", "never"])
+    ap.add_argument("--no-fuse", action="store_true", help="SpMV and first-pass dots as two kernels")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this solver runs on one GPU; multi-GPU runs are not part of this build "
+              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    mek = _load("mek", os.path.join(ROOT, "matrix-exp-krylov_amd", "__init__.py"))
+    gen = _load("mek_generate", os.path.join(ROOT, "matrix-exp-krylov_amd", "generate.py"))
+    w = WORKLOADS[args.workload]
+    log(f"building {args.workload}")
+    if w["kind"] == "laplacian":
+        rp, ci, va = gen.laplacian_2d(w["N"])
+        n = w["N"] ** 2
+    else:
+        rp, ci, va = gen.convdiff_3d(w["N"])
+        n = w["N"] ** 3
+    nnz = int(rp[-1])
+    v = gen.normal_vector(n, seed=0)
+    gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+    A = mek.CSR.from_arrays(rp, ci, va)
+    R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
+                   gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=True,
+                   fuse_spmv_dot=not args.no_fuse)
+    A.close()
+    d_v = mek.DeviceBuffer.from_array(v)
+    d_y = mek.DeviceBuffer(8 * n)
+    mek.synchronize()
+
+    for i in range(args.warmup):
+        log(f"warmup {i + 1}/{args.warmup}")
+        R.expmv_device(d_v, d_y)
+    mek.synchronize()
+    kern = {k: {"ms": 0.0, "launches": 0, "bytes": 0.0} for k in mek.KERNEL_KINDS}
+    log(f"timing {args.steps} solves")
+    t0 = time.perf_counter()
+    stats = None
+    for i in range(args.steps):
+        R.expmv_device(d_v, d_y)
+        for k in kern:  # per-solve event totals (each solve resets them)
+            kt = R.kernel_time(k)
+            for f in kern[k]:
+                kern[k][f] += kt[f]
+        stats = R.stats()
+    mek.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1000.0 / args.steps
+    log(f"{ms_per_step:.1f} ms per solve; checking against the closed form")
+
+    y = d_y.to_array()
+    ref = reference_solution(w, v, gen)
+    s = max(np.abs(ref).max(), 1e-300)
+    rel = float(np.linalg.norm((y - ref) / s) / np.linalg.norm(ref / s))
+    log(f"relative error {rel:.3e}")
+
+    def roof(k):
+        kt = kern[k]
+        if kt["launches"] == 0 or kt["ms"] <= 0:
+            return None
+        achieved = kt["bytes"] / (kt["ms"] * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": None, "kernel": k,
+                "bytes_per_launch": kt["bytes"] / kt["launches"],
+                "avg_launch_us": kt["ms"] * 1e3 / kt["launches"]}
+
+    streaming = [k for k in ("spmv", "update_dot", "update_norm", "dots", "lift") if kern[k]["launches"]]
+    dominant = max(streaming, key=lambda k: kern[k]["ms"])
+    roofline = roof(dominant)
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        if roofline is not None and dominant in tr:
+            roofline["traffic"] = tr[dominant]
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("CPU baseline (oracle, one thread, bounded sample)")
+        cpu = cpu_baseline(rp, ci, va, v, stats["spmv_count"], w["m"], n, args.cpu_seconds)
+
+    solves_per_s = 1000.0 / ms_per_step
+    out = {
+        "metric": "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64",
+        "value": round(solves_per_s, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-values/f64" if w["f32"] else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n": n, "nnz": nnz, "t": w["t"], "m": w["m"],
+                   "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
+                   "fused_spmv_dot": not args.no_fuse, "seed": 0},
+        "rel_err": rel,
+        "solve": {"substeps": stats["accepted"], "expm_attempts": stats["attempts"],
+                  "krylov_columns": stats["spmv_count"], "extra_spmv": stats["extra_spmv"],
+                  "converged": bool(stats["converged"])},
+        "roofline": roofline,
+        "spmv": roof("spmv"),
+        "kernels_ms_per_solve": {k: round(kern[k]["ms"] / args.steps, 3) for k in kern if kern[k]["launches"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    R.close()
+
+
+if __name__ == "__main__":
+    main()

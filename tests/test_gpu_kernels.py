@@ -1,0 +1,120 @@
+"""Kernel-level parity on the GPU, through the C ABI of libmek.so.
+
+SpMV: bit-exact against the oracle's sequential CSR loop for rows that fit a
+short-row block (same products, same summation order); long rows (> 2048
+nonzeros, reduced by a whole workgroup) within 1e-14 relative.
+expm: against the oracle's Pade restatement (Higham 2005).
+Gram-Schmidt: the Hessenberg matrix of one Arnoldi decomposition (CGS2 on the
+device) against the oracle's MGS2.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import gen, mek
+
+pytestmark = pytest.mark.gpu
+
+
+def _long_row_matrix(n=3000, seed=3):
+    """random_rows plus one dense row (row 7: n nonzeros) and two empty rows."""
+    rp, ci, va = gen.random_rows(n, 5, 30, seed=seed)
+    rows = [list(zip(ci[rp[r]:rp[r + 1]], va[rp[r]:rp[r + 1]])) for r in range(n)]
+    rng = np.random.default_rng(seed)
+    rows[7] = [(c, float(x)) for c, x in zip(range(n), rng.uniform(-1, 1, n))]
+    rows[11] = []
+    rows[n - 1] = []
+    new_rp = np.zeros(n + 1, np.int32)
+    new_rp[1:] = np.cumsum([len(r) for r in rows])
+    new_ci = np.array([c for r in rows for c, _ in r], np.int32)
+    new_va = np.array([x for r in rows for _, x in r], np.float64)
+    return new_rp, new_ci, new_va
+
+
+@pytest.mark.parametrize("which", ["laplacian", "random", "rows"])
+def test_spmv_bitexact(gpu, oracle, which):
+    print(f"[spmv] {which}", flush=True)
+    if which == "laplacian":
+        rp, ci, va = gen.laplacian_2d(64)
+    elif which == "random":
+        rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
+    else:
+        rp, ci, va = gen.random_rows(5000, 5, 30, seed=1)
+    x = np.random.default_rng(7).standard_normal(rp.size - 1)
+    plan = mek.SpmvPlan(rp, ci, va)
+    y = plan(x)
+    ref = oracle.spmv(rp, ci, va, x)
+    np.testing.assert_array_equal(y, ref)
+
+
+def test_spmv_long_and_empty_rows(gpu, oracle):
+    rp, ci, va = _long_row_matrix()
+    x = np.random.default_rng(9).standard_normal(rp.size - 1)
+    y = mek.SpmvPlan(rp, ci, va)(x)
+    ref = oracle.spmv(rp, ci, va, x)
+    short = np.ones(rp.size - 1, bool)
+    short[7] = False
+    np.testing.assert_array_equal(y[short], ref[short])
+    assert abs(y[7] - ref[7]) <= 1e-13 * np.abs(va[rp[7]:rp[8]] * x).sum()
+    assert y[11] == 0.0 and y[-1] == 0.0
+
+
+def test_spmv_f32_values_exact_stencil(gpu, oracle):
+    # stencil coefficients are exact in FP32, so FP32 storage changes nothing
+    rp, ci, va = gen.convdiff_3d(20)
+    x = np.random.default_rng(3).standard_normal(rp.size - 1)
+    y = mek.SpmvPlan(rp, ci, va, values_f32=True)(x)
+    np.testing.assert_array_equal(y, oracle.spmv(rp, ci, va, x))
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 17, 40, 58, 60, 61, 80])
+def test_expm_dense(gpu, oracle, k):
+    rng = np.random.default_rng(100 + k)
+    for scale in [1e-3, 0.2, 0.9, 2.0, 5.0, 30.0, 300.0]:
+        print(f"[expm] k={k} scale={scale}", flush=True)
+        H = np.triu(rng.standard_normal((k, k)), -1)
+        H *= scale / max(np.abs(H).sum(0).max(), 1e-300)
+        if scale > 5:
+            H -= np.eye(k) * scale / 4  # keep exp(H) representable
+        E = mek.expm_dense(H)
+        R, _ = oracle.expm(H)
+        err = np.linalg.norm(E - R) / np.linalg.norm(R)
+        assert err < 1e-12 * max(1.0, scale), (k, scale, err)
+
+
+def test_expm_closed_forms(gpu):
+    # Jordan block exp(t[[a,1],[0,a]]) = e^{at}[[1,t],[0,1]]
+    a, t = -0.7, 3.0
+    E = mek.expm_dense(np.array([[a * t, t], [0.0, a * t]]))
+    np.testing.assert_allclose(E, np.exp(a * t) * np.array([[1, t], [0, 1]]), rtol=1e-14, atol=1e-15)
+    # rotation
+    w = 2.5
+    E = mek.expm_dense(np.array([[0.0, -w], [w, 0.0]]))
+    np.testing.assert_allclose(E, [[np.cos(w), -np.sin(w)], [np.sin(w), np.cos(w)]], atol=1e-14)
+    # nilpotent shift: finite series
+    k = 6
+    N = np.diag(np.ones(k - 1), 1) * 1.5
+    ref = sum(np.linalg.matrix_power(N, p) / math.factorial(p) for p in range(k))
+    np.testing.assert_allclose(mek.expm_dense(N), ref, rtol=1e-14, atol=1e-14)
+    # diagonal
+    d = np.linspace(-20, 3, 9)
+    np.testing.assert_allclose(mek.expm_dense(np.diag(d)), np.diag(np.exp(d)), rtol=1e-13)
+
+
+@pytest.mark.parametrize("which,m", [("random", 30), ("laplacian", 40), ("rows", 70)])
+def test_arnoldi_hessenberg(gpu, oracle, which, m):
+    print(f"[arnoldi] {which} m={m}", flush=True)
+    if which == "random":
+        rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
+    elif which == "laplacian":
+        rp, ci, va = gen.laplacian_2d(48)
+    else:
+        rp, ci, va = gen.random_rows(3000, 5, 30, seed=5)
+    v = gen.normal_vector(rp.size - 1, seed=11, normalise=False)
+    R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=m)
+    H, k = R.arnoldi(v)
+    Href, kref = oracle.arnoldi(rp, ci, va, v, m)
+    assert k == kref == m
+    err = np.linalg.norm(H - Href) / np.linalg.norm(Href)
+    assert err < 1e-10, err

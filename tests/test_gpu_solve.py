@@ -1,0 +1,146 @@
+"""End-to-end exp(t*A)*v parity on the GPU (C ABI), at sizes the oracle finishes
+in seconds.  Bar: relative 2-norm error <= 1e-8 against the FP64 oracle or a
+closed form; the controller's step sequence must match the oracle's.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from closed_forms import kron3_expmv, laplacian_2d_eigvec, laplacian_2d_expmv, rel_err
+from conftest import gen, mek
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-8  # relative 2-norm error gate for the FP64 path
+
+
+def _runner(rp, ci, va, **kw):
+    return mek.Runner(mek.CSR.from_arrays(rp, ci, va), **kw)
+
+
+def test_cfg1_random_matches_oracle(gpu, oracle):
+    """n=2000 random Gershgorin-negative matrix, v=0.1, t=1, m=30, tol 1e-6, R=10."""
+    rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
+    v = np.full(2000, 0.1)
+    R = _runner(rp, ci, va, m=30, t=1.0, tol=1e-6, max_restarts=10)
+    y = R.expmv(v)
+    yref, st, log = oracle.expmv(rp, ci, va, v, 1.0, 1e-6, 30, 10)
+    assert rel_err(y, yref) <= TOL
+    s = R.stats()
+    assert s["converged"] == st["converged"]
+    assert s["restarts_done"] == st["restarts_done"]
+    steps = R.steps()
+    assert len(steps) == len(log)
+    for a, b in zip(steps, log):
+        assert a["m_used"] == b["m_used"] and a["accepts"] == b["accepts"] and a["rejects"] == b["rejects"]
+        assert a["dt"] == pytest.approx(b["dt"], rel=1e-15)
+
+
+def test_rejections_and_small_m(gpu, oracle):
+    """Small m and tight tol force rejections (the basis is reused on reject)."""
+    rp, ci, va = gen.random_negative(1500, 5e-3, seed=3)
+    v = gen.normal_vector(1500, seed=4)
+    R = _runner(rp, ci, va, m=8, t=2.0, tol=1e-10, max_restarts=0)
+    y = R.expmv(v)
+    yref, st, log = oracle.expmv(rp, ci, va, v, 2.0, 1e-10, 8, 0)
+    assert st["attempts"] > st["restarts_done"]  # there were rejections
+    steps = R.steps()
+    assert [s["rejects"] for s in steps] == [s["rejects"] for s in log]
+    assert R.stats()["converged"] == st["converged"]
+    assert rel_err(y, yref) <= TOL
+
+
+def test_builtin_diagonal_closed_form(gpu):
+    """The CLI's built-in matrix a_ii = 1 + i/n: y_i = 0.1 e^{t(1+i/n)}."""
+    n = 1000
+    d = 1.0 + np.arange(n) / n
+    rp = np.arange(n + 1, dtype=np.int32)
+    R = _runner(rp, np.arange(n, dtype=np.int32), d, m=30, t=1.0, tol=1e-6, max_restarts=10)
+    y = R.expmv(np.full(n, 0.1))
+    assert rel_err(y, 0.1 * np.exp(d)) <= TOL
+
+
+def test_eigenvector_happy_breakdown(gpu):
+    N = 40
+    rp, ci, va = gen.laplacian_2d(N)
+    u, lam = laplacian_2d_eigvec(N, 3, 5)
+    R = _runner(rp, ci, va, m=20, t=1.5, tol=1e-8, max_restarts=10)
+    y = R.expmv(2.0 * u)
+    assert rel_err(y, 2.0 * np.exp(1.5 * lam) * u) <= 1e-12
+    assert R.steps()[0]["m_used"] == 1  # breakdown at the first column
+
+
+@pytest.mark.parametrize("control", ["reference", "expokit"])
+def test_laplacian_dst(gpu, control):
+    N, t = 128, 10.0
+    rp, ci, va = gen.laplacian_2d(N)
+    v = gen.normal_vector(N * N, seed=1)
+    R = _runner(rp, ci, va, m=40, t=t, tol=1e-10, max_restarts=0, step_control=control)
+    y = R.expmv(v)
+    assert rel_err(y, laplacian_2d_expmv(N, t, v)) <= TOL
+    assert R.stats()["converged"] == 1
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_convdiff_kronecker(gpu, f32):
+    """Nonsymmetric 3-D upwind convection-diffusion (Pe=50); FP32 storage is
+    lossless for these coefficients, so both forms meet the FP64 gate."""
+    N, t = 24, 0.5
+    rp, ci, va = gen.convdiff_3d(N)
+    (lx, dx, ux), (ly, dy, uy), (lz, dz, uz) = gen.convdiff_3d_coeffs()
+    v = gen.normal_vector(N ** 3, seed=2)
+    R = _runner(rp, ci, va, m=50, t=t, tol=1e-10, max_restarts=0, step_control="expokit",
+                values_f32=f32)
+    y = R.expmv(v)
+    ref = kron3_expmv(gen.tridiag_1d(N, lz, dz, uz), gen.tridiag_1d(N, ly, dy, uy),
+                      gen.tridiag_1d(N, lx, dx, ux), t, v)
+    assert rel_err(y, ref) <= TOL
+
+
+def test_dgks_conditional_reorth(gpu, oracle):
+    rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
+    v = gen.normal_vector(2000, seed=8)
+    R = _runner(rp, ci, va, m=30, t=1.0, tol=1e-10, max_restarts=0, gs_mode=2, dgks_eta=0.1)
+    y = R.expmv(v)
+    yref, _, _ = oracle.expmv(rp, ci, va, v, 1.0, 1e-10, 30, 0)
+    assert rel_err(y, yref) <= TOL
+
+
+def test_large_m_generic_path(gpu, oracle):
+    """m > 64 takes the chunked dot path and the global-memory expm."""
+    rp, ci, va = gen.random_rows(3000, 5, 30, seed=5)
+    v = gen.normal_vector(3000, seed=6)
+    R = _runner(rp, ci, va, m=70, t=0.5, tol=1e-10, max_restarts=0)
+    y = R.expmv(v)
+    yref, st, _ = oracle.expmv(rp, ci, va, v, 0.5, 1e-10, 70, 0)
+    assert rel_err(y, yref) <= TOL
+
+
+def test_device_resident_entry(gpu, oracle):
+    rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
+    v = gen.normal_vector(2000, seed=9)
+    R = _runner(rp, ci, va, m=30, t=1.0, tol=1e-8, max_restarts=10)
+    d_v = mek.DeviceBuffer.from_array(v)
+    d_y = mek.DeviceBuffer(8 * 2000)
+    R.expmv_device(d_v, d_y)
+    yref, _, _ = oracle.expmv(rp, ci, va, v, 1.0, 1e-8, 30, 10)
+    assert rel_err(d_y.to_array(), yref) <= TOL
+
+
+def test_cli_builtin_matrix(gpu, tmp_path, oracle):
+    out = tmp_path / "y.txt"
+    env = dict(os.environ, OUTPUT_Y_FILE=str(out))
+    p = subprocess.run([mek.CLI_PATH, "500", "--t", "1.0"], env=env, capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "Arnoldi summary: m=30, max_restarts=10" in p.stdout
+    assert "METRIC total_compute_expmv_ms=" in p.stdout
+    lines = out.read_text().splitlines()
+    assert len(lines) == 500 and lines[0].count("e") == 1
+    y = np.array([float(s) for s in lines])
+    d = 1.0 + np.arange(500) / 500
+    rp = np.arange(501, dtype=np.int32)
+    yref, st, _ = oracle.expmv(rp, np.arange(500, dtype=np.int32), d, np.full(500, 0.1), 1.0, 1e-6, 30, 10)
+    assert rel_err(y, yref) <= TOL
+    assert ("converged=yes" in p.stdout) == bool(st["converged"])

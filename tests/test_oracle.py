@@ -1,0 +1,168 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign or
+index, transposed operand) fails it: SciPy golden fixtures, closed forms,
+invariants and brute force on tiny inputs.  The reference holds no fixtures
+of its own for this path.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from closed_forms import laplacian_2d_eigvec, laplacian_2d_expmv, rel_err
+from conftest import gen
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ------------------------------------------------------------------ SpMV
+
+def test_spmv_brute_force(oracle):
+    rng = np.random.default_rng(0)
+    D = rng.standard_normal((13, 13)) * (rng.random((13, 13)) < 0.3)
+    A = sp.csr_matrix(D)
+    x = rng.standard_normal(13)
+    y = oracle.spmv(A.indptr, A.indices, A.data, x)
+    np.testing.assert_allclose(y, D @ x, rtol=1e-14, atol=1e-14)
+
+
+def test_spmv_row_sums_and_integer_stencil(oracle):
+    rp, ci, va = gen.laplacian_2d(32)
+    ones = np.ones(32 * 32)
+    A = sp.csr_matrix((va, ci, rp))
+    np.testing.assert_array_equal(oracle.spmv(rp, ci, va, ones), A.sum(1).A1)
+    xi = np.random.default_rng(1).integers(-1000, 1000, 32 * 32).astype(float)
+    np.testing.assert_array_equal(oracle.spmv(rp, ci, va, xi), A @ xi)  # exact in FP64
+
+
+def test_spmv_laplacian_eigenmode(oracle):
+    N = 20
+    rp, ci, va = gen.laplacian_2d(N)
+    u, lam = laplacian_2d_eigvec(N, 2, 7)
+    np.testing.assert_allclose(oracle.spmv(rp, ci, va, u), lam * u, atol=1e-13)
+
+
+# ------------------------------------------------------------------ expm
+
+def test_pade_coefficients_known_values(oracle):
+    b = oracle.pade_coeffs(13)
+    assert b[13] == 1 and b[12] == 182 and b[11] == 16380 and b[0] == 64764752532480000
+    assert oracle.pade_coeffs(3).tolist() == [120, 60, 12, 1]
+
+
+def test_expm_scipy_fixtures(oracle):
+    z = np.load(os.path.join(GOLDEN, "expm_cases.npz"))
+    for k, H, E in zip(z["k"], z["H"], z["E"]):
+        H, E = H[:k, :k], E[:k, :k]
+        got, _ = oracle.expm(H)
+        assert rel_err(got, E) < 1e-12 * max(1.0, np.abs(H).sum(0).max()), k
+        if k > 2:  # a transposed operand would fail the same check
+            bad, _ = oracle.expm(H.T)
+            assert rel_err(bad, E) > 1e-6
+
+
+def test_expm_closed_forms(oracle):
+    a, t = -0.7, 3.0
+    E, _ = oracle.expm(np.array([[a * t, t], [0.0, a * t]]))
+    np.testing.assert_allclose(E, np.exp(a * t) * np.array([[1, t], [0, 1]]), rtol=1e-14)
+    w = 2.5
+    E, _ = oracle.expm(np.array([[0.0, -w], [w, 0.0]]))
+    np.testing.assert_allclose(E, [[np.cos(w), -np.sin(w)], [np.sin(w), np.cos(w)]], atol=1e-14)
+    N = np.diag(np.ones(5), 1) * 1.5
+    ref = sum(np.linalg.matrix_power(N, p) / math.factorial(p) for p in range(6))
+    np.testing.assert_allclose(oracle.expm(N)[0], ref, rtol=1e-14, atol=1e-14)
+    d = np.linspace(-20, 3, 9)
+    E, s = oracle.expm(np.diag(d))
+    np.testing.assert_allclose(E, np.diag(np.exp(d)), rtol=1e-13)
+    assert s == 2  # ||A||_1 = 20 > theta_13: ceil(log2(20 / 5.3719...)) = 2
+    E, s = oracle.expm(np.array([[1e-4]]))
+    assert s == 0 and abs(E[0, 0] - math.exp(1e-4)) <= 2.3e-16
+
+
+def test_expm_scaling_count(oracle):
+    theta13 = 5.371920351148152
+    for nrm, s_exp in [(theta13, 0), (2 * theta13, 1), (2 * theta13 * 1.0001, 2), (1000.0, 8)]:
+        _, s = oracle.expm(np.array([[-nrm]]))
+        assert s == s_exp, (nrm, s)
+
+
+def test_expm_nonfinite(oracle):
+    E, s = oracle.expm(np.array([[1.0, np.nan], [0.0, 1.0]]))
+    assert s == -1 and np.isnan(E).all()
+
+
+# ------------------------------------------------------------------ solve
+
+def test_expmv_builtin_diagonal(oracle):
+    n = 1000
+    d = 1.0 + np.arange(n) / n
+    y, st, _ = oracle.expmv(np.arange(n + 1), np.arange(n), d, np.full(n, 0.1), 1.0, 1e-6, 30, 10)
+    assert rel_err(y, 0.1 * np.exp(d * st["t_done"])) < 1e-12
+
+
+def test_expmv_eigenvector_breakdown(oracle):
+    N = 30
+    rp, ci, va = gen.laplacian_2d(N)
+    u, lam = laplacian_2d_eigvec(N, 4, 1)
+    y, st, log = oracle.expmv(rp, ci, va, 3 * u, 2.0, 1e-8, 20, 10)
+    assert log[0]["m_used"] == 1 and log[0]["err"] == 0.0
+    assert rel_err(y, 3 * np.exp(lam * st["t_done"]) * u) < 1e-13
+
+
+def test_expmv_laplacian_dst(oracle):
+    N, t = 40, 5.0
+    rp, ci, va = gen.laplacian_2d(N)
+    v = gen.normal_vector(N * N, seed=3)
+    y, st, _ = oracle.expmv(rp, ci, va, v, t, 1e-10, 30, 0)
+    assert st["converged"] == 1
+    assert rel_err(y, laplacian_2d_expmv(N, t, v)) < 1e-9
+
+
+def test_expmv_scipy_fixtures(oracle):
+    z = np.load(os.path.join(GOLDEN, "expmv_cases.npz"))
+    for i in range(int(z["count"])):
+        rp, ci, va = z[f"c{i}_indptr"], z[f"c{i}_indices"], z[f"c{i}_data"]
+        v, t, yref = z[f"c{i}_v"], float(z[f"c{i}_t"]), z[f"c{i}_y"]
+        y, st, _ = oracle.expmv(rp, ci, va, v, t, 1e-10, 30, 0)
+        assert st["converged"] == 1
+        assert rel_err(y, yref) < 1e-9, i
+
+
+def test_controller_abort_after_20_rejects(oracle):
+    # m = 1 with an unreachable tolerance: every attempt is rejected
+    rp, ci, va = gen.random_negative(200, 5e-2, seed=2)
+    v = gen.normal_vector(200, seed=2)
+    y, st, log = oracle.expmv(rp, ci, va, v, 1.0, 1e-300, 1, 10)
+    assert st["aborted"] == 1 and st["converged"] == 0 and st["attempts"] == 20
+    assert [r["rejects"] for r in log] == list(range(20))
+    dts = [r["dt"] for r in log]
+    assert all(abs(b - a * 0.4) <= 1e-15 * a or b == max(1e-6, 1e-12) for a, b in zip(dts, dts[1:]))
+
+
+def test_controller_step_cap_leaves_t_short(oracle):
+    # max_restarts caps the accepted steps: after rejections shrink dt, two
+    # accepted steps cannot reach t, so the run ends with converged = no and
+    # y = exp(t_done A) v for t_done < t
+    import scipy.sparse.linalg as spla
+    rp, ci, va = gen.random_negative(300, 2e-2, seed=5)
+    v = gen.normal_vector(300, seed=5)
+    y, st, log = oracle.expmv(rp, ci, va, v, 1.0, 1e-10, 6, 2)
+    assert st["restarts_done"] == 2 and st["converged"] == 0
+    assert st["t_done"] < 1.0 and st["attempts"] > 2
+    A = sp.csr_matrix((va, ci, rp), shape=(300, 300))
+    assert rel_err(y, spla.expm_multiply(st["t_done"] * A, v)) < 1e-9
+
+
+def test_nan_start_vector_is_accepted(oracle):
+    # a NaN column norm counts as breakdown (err = 0, step accepted); the NaN
+    # coefficient fails the lift's |wH_i| > 1e-15 test, so y = 0 and the run
+    # reports converged (main_cpu.cpp:529, :564, :645)
+    rp, ci, va = gen.laplacian_2d(10)
+    v = np.ones(100)
+    v[5] = np.nan
+    y, st, log = oracle.expmv(rp, ci, va, v, 1.0, 1e-6, 5, 10)
+    assert log[0]["err"] == 0.0 and log[0]["m_used"] == 1
+    assert st["converged"] == 1 and np.all(y == 0.0)
