@@ -17,10 +17,19 @@
 #include <iostream>
 #include <limits>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../kernels/launch.hpp"
 #include "hip_check.hpp"
 
 namespace {
+
+// roctx range with the reference's NVTX names (main.cu:38-41, :1413, :1437,
+// :1575, :1770, :1955), visible in rocprofv3 --marker-trace.
+struct Range {
+  explicit Range(const char* name) { roctxRangePush(name); }
+  ~Range() { roctxRangePop(); }
+};
 
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
@@ -138,6 +147,7 @@ int ArnoldiRunner::read_status() {
 }
 
 void ArnoldiRunner::init_q1_device(const double* d_v, const double* h_v) {
+  Range r("init_q1");
   DeviceContext& dc = device_contexts[0];
   const mek::ArnoldiBufs b = dc.bufs();
   hipStream_t s = dc.stream_compute;
@@ -150,6 +160,7 @@ void ArnoldiRunner::init_q1_device(const double* d_v, const double* h_v) {
 }
 
 void ArnoldiRunner::run_columns(int m_cols) {
+  Range r("orthogonalize_mgs");
   DeviceContext& dc = device_contexts[0];
   const mek::ArnoldiBufs b = dc.bufs();
   hipStream_t s = dc.stream_compute;
@@ -202,6 +213,7 @@ void ArnoldiRunner::run_columns(int m_cols) {
 }
 
 void ArnoldiRunner::launch_expm(double dt, int mode) {
+  Range r("small_expm_and_lift");
   DeviceContext& dc = device_contexts[0];
   const mek::ArnoldiBufs b = dc.bufs();
   mek::ExpmArgs a{};
@@ -256,6 +268,7 @@ void ArnoldiRunner::reference_controller() {
     int rejects = 0;
     int m_current = params.m;
     for (;;) {
+      Range r_step("arnoldi_step_attempt");
       if (m_current > dc.allocated_m_max)
         throw mek::Error("ERROR: m_current exceeds allocated_m_max on GPU 0");
       if (!basis_ready) {
@@ -418,6 +431,7 @@ void ArnoldiRunner::expokit_controller() {
 // ------------------------------------------------------------ entry points
 
 void ArnoldiRunner::solve_on_device(const double* d_v, const double* h_v) {
+  Range r("total_compute_expmv");
   DeviceContext& dc = device_contexts[0];
   MEK_HIP(hipSetDevice(dc.device_id));
   stats = mek::SolveStats();
