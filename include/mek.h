@@ -135,6 +135,30 @@ int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used);
 int mek_runner_stream(const mek_runner* R, void** stream);
 void mek_runner_destroy(mek_runner* R);
 
+/* ---------------------------------------------------------------- batch */
+
+/* Many independent problems y_p = exp(t_p A_p) v_p of uneven size, solved in
+ * lockstep on one GPU (one block-diagonal CSR, per-problem reductions, the
+ * reference controller of main.cu:1425-1518 on the device, one hipGraph per
+ * cycle).  No reference counterpart: the reference solves one problem per
+ * process.  m[p] in [1, 62]; substeps[p] > 0 sets the first sub-step to
+ * t/substeps with no step cap (learned selector), 0 keeps max_restarts. */
+typedef struct mek_batch mek_batch;
+int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
+                     const double* tol, const int* max_restarts, const int* substeps,
+                     int values_f32, int gs_mode, double dgks_eta, int use_graph, int device_id,
+                     mek_batch** out);
+/* v[p]: host arrays of each problem's rows, copied to the device. */
+int mek_batch_upload_v(mek_batch* B, const double* const* v);
+/* Solve every problem from the uploaded v; y stays in device memory. */
+int mek_batch_solve(mek_batch* B);
+int mek_batch_download_y(mek_batch* B, double* const* y);
+/* Per-problem statistics (converged, accepted, attempts, spmv_count =
+ * columns built, t_done, last_residual); solve_ms is the whole batch. */
+int mek_batch_stats(const mek_batch* B, int p, mek_solve_stats* out);
+int mek_batch_cycles(const mek_batch* B, int* cycles);
+void mek_batch_destroy(mek_batch* B);
+
 /* ---------------------------------------------------------- kernel level */
 
 typedef struct mek_spmv_plan mek_spmv_plan;

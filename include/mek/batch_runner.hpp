@@ -1,0 +1,83 @@
+// BatchRunner: many independent y_p = exp(t_p A_p) v_p solves of uneven size
+// on one GPU, advanced in lockstep (additive; the reference solves one
+// problem per process).  Each problem follows the reference controller
+// (main.cu:1425-1518) with its own t, m, tol and step cap; the controller runs
+// on the device and one cycle of the solve is replayed from a hipGraph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mek/csr_host.hpp"
+
+namespace mek {
+
+struct BatchProblem {
+  const CSRHost* A;
+  double t;
+  int m;               // Krylov dimension (1..62)
+  double tol;
+  int max_restarts;    // as ArnoldiParams::max_restarts
+  int substeps;        // >0: first sub-step t/substeps and no step cap (learned selector)
+};
+
+struct BatchProblemStats {
+  int converged, accepted, attempts, columns, aborted, reorth_columns;
+  double t_done, last_residual;
+};
+
+class BatchRunner {
+ public:
+  // gs_mode: 0 never, 1 always (reference), 2 DGKS with eta.
+  BatchRunner(int device_id, const std::vector<BatchProblem>& problems, bool values_f32,
+              int gs_mode, double dgks_eta, bool use_graph);
+  ~BatchRunner();
+  BatchRunner(const BatchRunner&) = delete;
+  BatchRunner& operator=(const BatchRunner&) = delete;
+
+  int num_problems() const { return P_; }
+  long long total_rows() const { return n_tot_; }
+  int row_offset(int p) const { return row0_[p]; }
+  int rows(int p) const { return row0_[p + 1] - row0_[p]; }
+
+  // Start vectors (host, one array per problem) into the device copy of v.
+  void upload_v(const std::vector<const double*>& v);
+  // Solve all problems from the uploaded v; y stays on the device.
+  void solve_device();
+  void download_y(const std::vector<double*>& y) const;
+  BatchProblemStats stats(int p) const;
+  int cycles() const { return cycles_; }
+  double solve_ms() const { return solve_ms_; }
+  hipStream_t stream() const { return stream_; }
+  void destroy();
+
+ private:
+  void run_cycle_eager();
+  int P_ = 0;
+  int M_ = 0;
+  long long n_tot_ = 0;
+  std::vector<int> row0_;
+  int device_ = 0;
+  bool use_graph_ = true;
+  int gs_mode_ = 1;
+  double eta_ = 0.1;
+  int spmv_grid_ = 0;
+  hipStream_t stream_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  int cycles_ = 0;
+  double solve_ms_ = 0.0;
+  // device buffers
+  int *d_row_ptr_ = nullptr, *d_col_idx_ = nullptr, *d_rb_start_ = nullptr, *d_rb_prob_ = nullptr;
+  int *d_sb_prob_ = nullptr, *d_sb_row0_ = nullptr, *d_p_sb0_ = nullptr, *d_pint_ = nullptr;
+  int *d_counters_ = nullptr, *h_counters_ = nullptr;
+  void* d_vals_ = nullptr;
+  double *d_V_ = nullptr, *d_w_ = nullptr, *d_y_ = nullptr, *d_v0_ = nullptr, *d_small_ = nullptr;
+  double *d_partials_ = nullptr, *d_expm_ws_ = nullptr, *d_pscal_ = nullptr;
+  std::vector<int> pint_init_;
+  std::vector<double> pscal_init_;
+  void* bufs_ = nullptr;  // BatchBufs (opaque here)
+};
+
+}  // namespace mek

@@ -93,6 +93,15 @@ SIGNATURES = {
     "mek_runner_arnoldi": (ctypes.c_int, [_vp, _dp, _dp, _ip]),
     "mek_runner_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "mek_runner_destroy": (None, [_vp]),
+    "mek_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _dp, _ip, _dp, _ip, _ip,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mek_batch_upload_v": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
+    "mek_batch_solve": (ctypes.c_int, [_vp]),
+    "mek_batch_download_y": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
+    "mek_batch_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(SolveStats)]),
+    "mek_batch_cycles": (ctypes.c_int, [_vp, _ip]),
+    "mek_batch_destroy": (None, [_vp]),
     "mek_spmv_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp,
                                             ctypes.c_int, _ip, ctypes.POINTER(_vp)]),
     "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -306,6 +315,65 @@ class Runner:
     def close(self):
         if self.h:
             lib().mek_runner_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ batch
+
+class Batch:
+    """Many independent exp(t_p A_p) v_p solves in lockstep on one GPU."""
+
+    def __init__(self, mats, t, m, tol, max_restarts=None, substeps=None, values_f32=False,
+                 gs_mode=1, dgks_eta=0.1, use_graph=True, device=0):
+        P = len(mats)
+        self.csrs = [A if isinstance(A, CSR) else CSR.from_arrays(*A) for A in mats]
+        self.rows = [c.info()["rows"] for c in self.csrs]
+        handles = (_vp * P)(*[c.h for c in self.csrs])
+        t = _f64(np.broadcast_to(t, P))
+        m = _i32(np.broadcast_to(m, P))
+        tol = _f64(np.broadcast_to(tol, P))
+        mr = _i32(np.broadcast_to(10 if max_restarts is None else max_restarts, P))
+        ss = _i32(np.broadcast_to(0 if substeps is None else substeps, P))
+        h = _vp()
+        _check(lib().mek_batch_create(P, handles, _pd(t), _pi(m), _pd(tol), _pi(mr), _pi(ss),
+                                      1 if values_f32 else 0, int(gs_mode), float(dgks_eta),
+                                      1 if use_graph else 0, int(device), ctypes.byref(h)))
+        self.h = h
+        self.P = P
+
+    def upload(self, vs):
+        vs = [_f64(v) for v in vs]
+        arr = (_dp * self.P)(*[_pd(v) for v in vs])
+        _check(lib().mek_batch_upload_v(self.h, arr))
+
+    def solve(self):
+        _check(lib().mek_batch_solve(self.h))
+
+    def download(self):
+        ys = [np.zeros(n, np.float64) for n in self.rows]
+        arr = (_dp * self.P)(*[_pd(y) for y in ys])
+        _check(lib().mek_batch_download_y(self.h, arr))
+        return ys
+
+    def stats(self, p):
+        s = SolveStats()
+        _check(lib().mek_batch_stats(self.h, int(p), ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in SolveStats._fields_}
+
+    def cycles(self):
+        c = ctypes.c_int()
+        _check(lib().mek_batch_cycles(self.h, ctypes.byref(c)))
+        return c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mek_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

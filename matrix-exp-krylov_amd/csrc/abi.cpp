@@ -13,6 +13,7 @@
 #include "host/hip_check.hpp"
 #include "kernels/launch.hpp"
 #include "mek/arnoldi_runner.hpp"
+#include "mek/batch_runner.hpp"
 #include "mek/csr_host.hpp"
 
 struct mek_csr {
@@ -21,6 +22,10 @@ struct mek_csr {
 
 struct mek_runner {
   std::unique_ptr<ArnoldiRunner> R;
+};
+
+struct mek_batch {
+  std::unique_ptr<mek::BatchRunner> B;
 };
 
 struct mek_spmv_plan {
@@ -302,6 +307,78 @@ void mek_runner_destroy(mek_runner* R) {
   if (R->R) R->R->destroy();
   delete R;
 }
+
+// ------------------------------------------------------------------ batch
+
+int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
+                     const double* tol, const int* max_restarts, const int* substeps, int values_f32,
+                     int gs_mode, double dgks_eta, int use_graph, int device_id, mek_batch** out) {
+  return guarded([&] {
+    need(nprob >= 1 && A && t && m && tol && out, "batch arguments");
+    need(gs_mode >= 0 && gs_mode <= 2, "gs_mode");
+    std::vector<mek::BatchProblem> probs(nprob);
+    for (int p = 0; p < nprob; ++p) {
+      need(A[p] != nullptr, "matrix");
+      probs[p] = {&A[p]->A, t[p], m[p], tol[p], max_restarts ? max_restarts[p] : 10,
+                  substeps ? substeps[p] : 0};
+    }
+    auto h = std::make_unique<mek_batch>();
+    h->B = std::make_unique<mek::BatchRunner>(device_id, probs, values_f32 != 0, gs_mode, dgks_eta,
+                                             use_graph != 0);
+    *out = h.release();
+  });
+}
+
+int mek_batch_upload_v(mek_batch* B, const double* const* v) {
+  return guarded([&] {
+    need(B && v, "batch, v");
+    std::vector<const double*> vv(v, v + B->B->num_problems());
+    B->B->upload_v(vv);
+  });
+}
+
+int mek_batch_solve(mek_batch* B) {
+  return guarded([&] {
+    need(B, "batch");
+    B->B->solve_device();
+  });
+}
+
+int mek_batch_download_y(mek_batch* B, double* const* y) {
+  return guarded([&] {
+    need(B && y, "batch, y");
+    std::vector<double*> yy(y, y + B->B->num_problems());
+    B->B->download_y(yy);
+  });
+}
+
+int mek_batch_stats(const mek_batch* B, int p, mek_solve_stats* o) {
+  return guarded([&] {
+    need(B && o && p >= 0 && p < B->B->num_problems(), "batch, problem index, out");
+    const mek::BatchProblemStats s = B->B->stats(p);
+    std::memset(o, 0, sizeof(*o));
+    o->converged = s.converged;
+    o->restarts_done = s.accepted;
+    o->accepted = s.accepted;
+    o->attempts = s.attempts;
+    o->rejected = s.attempts - s.accepted;
+    o->spmv_count = s.columns;
+    o->reorth_columns = s.reorth_columns;
+    o->aborted = s.aborted;
+    o->t_done = s.t_done;
+    o->last_residual = s.last_residual;
+    o->solve_ms = B->B->solve_ms();
+  });
+}
+
+int mek_batch_cycles(const mek_batch* B, int* cycles) {
+  return guarded([&] {
+    need(B && cycles, "batch, cycles");
+    *cycles = B->B->cycles();
+  });
+}
+
+void mek_batch_destroy(mek_batch* B) { delete B; }
 
 // ----------------------------------------------------------- kernel level
 

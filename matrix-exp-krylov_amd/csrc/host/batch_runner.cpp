@@ -1,0 +1,248 @@
+// BatchRunner: lockstep batched exp(t_p A_p) v_p on one MI355X (batch.hip).
+#include "mek/batch_runner.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "../kernels/batch.hpp"
+#include "../kernels/launch.hpp"
+#include "hip_check.hpp"
+#include "mek/device_context.hpp"
+
+namespace mek {
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+template <typename T>
+T* upload(const std::vector<T>& h) {
+  T* d = nullptr;
+  MEK_HIP(hipMalloc(&d, sizeof(T) * std::max<size_t>(h.size(), 1)));
+  if (!h.empty()) MEK_HIP(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+}  // namespace
+
+BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs, bool values_f32,
+                         int gs_mode, double dgks_eta, bool use_graph)
+    : P_(static_cast<int>(probs.size())), device_(device_id), use_graph_(use_graph),
+      gs_mode_(gs_mode), eta_(dgks_eta) {
+  if (P_ < 1) throw Error("ERROR: a batch needs at least one problem");
+  MEK_HIP(hipSetDevice(device_));
+  // ---- host assembly: block-diagonal CSR, row blocks, streaming blocks
+  row0_.assign(P_ + 1, 0);
+  long long nnz_tot = 0;
+  for (int p = 0; p < P_; ++p) {
+    const CSRHost& A = *probs[p].A;
+    if (A.rows != A.cols) throw Error("ERROR: Matrix must be square (batch problem " + std::to_string(p) + ")");
+    if (probs[p].m < 1 || probs[p].m > kBatchMaxM || probs[p].m >= A.rows)
+      throw Error("ERROR: batch problem " + std::to_string(p) + ": m=" + std::to_string(probs[p].m) +
+                  " outside [1, min(" + std::to_string(kBatchMaxM) + ", rows-1)]");
+    row0_[p + 1] = row0_[p] + A.rows;
+    nnz_tot += A.nnz;
+    M_ = std::max(M_, probs[p].m);
+  }
+  n_tot_ = row0_[P_];
+  if (n_tot_ > 0x7fffffffLL || nnz_tot > 0x7fffffffLL) throw Error("ERROR: batch exceeds int32 rows/nnz");
+  std::vector<int> row_ptr(static_cast<size_t>(n_tot_) + 1), col_idx(static_cast<size_t>(nnz_tot));
+  std::vector<double> vals(static_cast<size_t>(nnz_tot));
+  std::vector<int> rb_start, rb_prob, sb_row0, sb_prob, p_sb0(P_ + 1);
+  long long k = 0;
+  for (int p = 0; p < P_; ++p) {
+    const CSRHost& A = *probs[p].A;
+    const int r0 = row0_[p];
+    for (int r = 0; r < A.rows; ++r) row_ptr[r0 + r] = static_cast<int>(k + A.row_ptr[r]);
+    for (long long e = 0; e < A.nnz; ++e) {
+      col_idx[k + e] = A.col_idx[e] + r0;
+      vals[k + e] = A.values[e];
+    }
+    k += A.nnz;
+    const std::vector<int> rb = build_row_blocks(A.row_ptr.data(), A.rows, kBatchNnzCap, kSpmvRowCap);
+    for (size_t i = 0; i + 1 < rb.size(); ++i) {
+      rb_start.push_back(r0 + rb[i]);
+      rb_prob.push_back(p);
+    }
+    p_sb0[p] = static_cast<int>(sb_prob.size());
+    for (int r = 0; r < A.rows; r += kBatchRowsPerBlock) {
+      sb_row0.push_back(r0 + r);
+      sb_prob.push_back(p);
+    }
+  }
+  row_ptr[n_tot_] = static_cast<int>(k);
+  rb_start.push_back(static_cast<int>(n_tot_));
+  sb_row0.push_back(static_cast<int>(n_tot_));
+  p_sb0[P_] = static_cast<int>(sb_prob.size());
+  const int nsb = static_cast<int>(sb_prob.size());
+  const int nrb = static_cast<int>(rb_prob.size());
+
+  pint_init_.assign(static_cast<size_t>(P_) * kPInt, 0);
+  pscal_init_.assign(static_cast<size_t>(P_) * kPScal, 0.0);
+  for (int p = 0; p < P_; ++p) {
+    pint_init_[p * kPInt + kBM] = probs[p].m;
+    pint_init_[p * kPInt + kBMaxRestarts] = probs[p].max_restarts;
+    pint_init_[p * kPInt + kBSubsteps] = probs[p].substeps;
+    pscal_init_[p * kPScal + kBT] = probs[p].t;
+    pscal_init_[p * kPScal + kBTol] = probs[p].tol;
+  }
+
+  // ---- device
+  MEK_HIP(hipStreamCreate(&stream_));
+  d_row_ptr_ = upload(row_ptr);
+  d_col_idx_ = upload(col_idx);
+  if (values_f32) {
+    std::vector<float> f(vals.begin(), vals.end());
+    d_vals_ = upload(f);
+  } else {
+    d_vals_ = upload(vals);
+  }
+  d_rb_start_ = upload(rb_start);
+  d_rb_prob_ = upload(rb_prob);
+  d_sb_row0_ = upload(sb_row0);
+  d_sb_prob_ = upload(sb_prob);
+  d_p_sb0_ = upload(p_sb0);
+  const long long ld = (n_tot_ + 63) / 64 * 64;
+  const size_t ncols = static_cast<size_t>(M_) + 2;
+  MEK_HIP(hipMalloc(&d_V_, sizeof(double) * ld * ncols));
+  MEK_HIP(hipMalloc(&d_w_, sizeof(double) * n_tot_));
+  MEK_HIP(hipMalloc(&d_y_, sizeof(double) * n_tot_));
+  MEK_HIP(hipMalloc(&d_v0_, sizeof(double) * n_tot_));
+  const size_t per = 3 * ncols + (ncols) * (ncols) + (M_ + 4);
+  MEK_HIP(hipMalloc(&d_small_, sizeof(double) * per * P_));
+  MEK_HIP(hipMemset(d_small_, 0, sizeof(double) * per * P_));
+  MEK_HIP(hipMalloc(&d_pscal_, sizeof(double) * kPScal * P_));
+  MEK_HIP(hipMalloc(&d_pint_, sizeof(int) * kPInt * P_));
+  MEK_HIP(hipMalloc(&d_partials_, sizeof(double) * (ncols + 1) * nsb));
+  if (M_ > kExpmMaxKLds) MEK_HIP(hipMalloc(&d_expm_ws_, sizeof(double) * 6 * M_ * M_ * P_));
+  MEK_HIP(hipMalloc(&d_counters_, sizeof(int) * 4));
+  MEK_HIP(hipMemset(d_counters_, 0, sizeof(int) * 4));
+  MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), sizeof(int) * 4));
+
+  auto* b = new BatchBufs{};
+  b->P = P_;
+  b->M = M_;
+  b->row_ptr = d_row_ptr_;
+  b->col_idx = d_col_idx_;
+  b->vals = d_vals_;
+  b->vals_f32 = values_f32 ? 1 : 0;
+  b->rb_start = d_rb_start_;
+  b->rb_prob = d_rb_prob_;
+  b->nrb = nrb;
+  b->sb_prob = d_sb_prob_;
+  b->sb_row0 = d_sb_row0_;
+  b->nsb = nsb;
+  b->p_sb0 = d_p_sb0_;
+  b->V = d_V_;
+  b->ld = ld;
+  b->w = d_w_;
+  b->y = d_y_;
+  b->colscale = d_small_;
+  b->h1 = b->colscale + ncols * P_;
+  b->h2 = b->h1 + ncols * P_;
+  b->H = b->h2 + ncols * P_;
+  b->wH = b->H + ncols * ncols * P_;
+  b->pscal = d_pscal_;
+  b->pint = d_pint_;
+  b->partials = d_partials_;
+  b->expm_ws = d_expm_ws_;
+  b->counters = d_counters_;
+  bufs_ = b;
+  spmv_grid_ = static_cast<int>(std::min<long long>(std::max(nrb, 1), 2048));
+}
+
+BatchRunner::~BatchRunner() { destroy(); }
+
+void BatchRunner::upload_v(const std::vector<const double*>& v) {
+  if (static_cast<int>(v.size()) != P_) throw Error("ERROR: upload_v needs one vector per problem");
+  for (int p = 0; p < P_; ++p)
+    MEK_HIP(hipMemcpyAsync(d_v0_ + row0_[p], v[p], sizeof(double) * rows(p), hipMemcpyHostToDevice, stream_));
+  MEK_HIP(hipStreamSynchronize(stream_));
+}
+
+void BatchRunner::run_cycle_eager() {
+  batch_launch_cycle(*static_cast<BatchBufs*>(bufs_), gs_mode_, eta_, spmv_grid_, stream_);
+  MEK_HIP_LAUNCHED();
+}
+
+void BatchRunner::solve_device() {
+  MEK_HIP(hipSetDevice(device_));
+  const double t0 = now_ms();
+  const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
+  MEK_HIP(hipMemcpyAsync(d_V_, d_v0_, sizeof(double) * n_tot_, hipMemcpyDeviceToDevice, stream_));
+  MEK_HIP(hipMemcpyAsync(d_pint_, pint_init_.data(), sizeof(int) * pint_init_.size(), hipMemcpyHostToDevice, stream_));
+  MEK_HIP(hipMemcpyAsync(d_pscal_, pscal_init_.data(), sizeof(double) * pscal_init_.size(),
+                         hipMemcpyHostToDevice, stream_));
+  batch_prepare();
+  batch_launch_init(b, stream_);
+  MEK_HIP_LAUNCHED();
+  if (use_graph_ && !graph_exec_) {
+    MEK_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    batch_launch_cycle(b, gs_mode_, eta_, spmv_grid_, stream_);
+    MEK_HIP(hipStreamEndCapture(stream_, &graph_));
+    MEK_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  }
+  cycles_ = 0;
+  const int max_cycles = 1000000;
+  for (;;) {
+    if (use_graph_) MEK_HIP(hipGraphLaunch(graph_exec_, stream_));
+    else run_cycle_eager();
+    MEK_HIP(hipMemcpyAsync(h_counters_, d_counters_, sizeof(int) * 2, hipMemcpyDeviceToHost, stream_));
+    MEK_HIP(hipStreamSynchronize(stream_));
+    ++cycles_;
+    if (h_counters_[0] == 0) break;
+    if (cycles_ >= max_cycles) throw Error("ERROR: batch did not finish within the cycle limit");
+  }
+  solve_ms_ = now_ms() - t0;
+}
+
+void BatchRunner::download_y(const std::vector<double*>& y) const {
+  if (static_cast<int>(y.size()) != P_) throw Error("ERROR: download_y needs one vector per problem");
+  for (int p = 0; p < P_; ++p)
+    MEK_HIP(hipMemcpyAsync(y[p], d_y_ + row0_[p], sizeof(double) * rows(p), hipMemcpyDeviceToHost, stream_));
+  MEK_HIP(hipStreamSynchronize(stream_));
+}
+
+BatchProblemStats BatchRunner::stats(int p) const {
+  int it[kPInt];
+  double s[kPScal];
+  MEK_HIP(hipMemcpy(it, d_pint_ + static_cast<size_t>(p) * kPInt, sizeof(it), hipMemcpyDeviceToHost));
+  MEK_HIP(hipMemcpy(s, d_pscal_ + static_cast<size_t>(p) * kPScal, sizeof(s), hipMemcpyDeviceToHost));
+  BatchProblemStats r{};
+  r.converged = it[kBConverged];
+  r.accepted = it[kBAccepted];
+  r.attempts = it[kBAttempts];
+  r.columns = it[kBColumns];
+  r.aborted = it[kBAborted];
+  r.reorth_columns = it[kBReorthCount];
+  r.t_done = s[kBTDone];
+  r.last_residual = s[kBLastRes];
+  return r;
+}
+
+void BatchRunner::destroy() {
+  if (!bufs_) return;
+  (void)hipSetDevice(device_);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  void* ptrs[] = {d_row_ptr_, d_col_idx_, d_rb_start_, d_rb_prob_, d_sb_prob_, d_sb_row0_, d_p_sb0_,
+                  d_pint_, d_counters_, d_vals_, d_V_, d_w_, d_y_, d_v0_, d_small_, d_partials_,
+                  d_expm_ws_, d_pscal_};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h_counters_) (void)hipHostFree(h_counters_);
+  h_counters_ = nullptr;
+  if (stream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
+  delete static_cast<BatchBufs*>(bufs_);
+  bufs_ = nullptr;
+}
+
+}  // namespace mek

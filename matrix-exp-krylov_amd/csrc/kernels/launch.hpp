@@ -32,6 +32,6 @@ void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, 
 
 void launch_expm(const ExpmArgs& a, int kk_max, hipStream_t s);
 size_t expm_lds_bytes(int kk);
-constexpr int kExpmMaxKLds = 60;  // must equal kMaxKLds in expm.hip
+constexpr int kExpmMaxKLds = 62;  // must equal kMaxKLds in expm_core.hpp
 
 }  // namespace mek

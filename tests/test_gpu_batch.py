@@ -1,0 +1,73 @@
+"""Batched lockstep solver on the GPU (C ABI) against the oracle, problem by
+problem: every problem follows the reference controller, so its y, accepted
+step count and attempt count must match the oracle's single solve."""
+import numpy as np
+import pytest
+
+from closed_forms import rel_err
+from conftest import gen, mek
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(P=12, seed=0):
+    rng = np.random.default_rng(seed)
+    mats, ts, ms, vs = [], [], [], []
+    for p in range(P):
+        n = int(np.exp(rng.uniform(np.log(300), np.log(6000))))
+        mats.append(gen.random_rows(n, 5, 30, seed=seed * 1000 + p))
+        ts.append(float(rng.uniform(0.1, 2.0)))
+        ms.append(int(rng.integers(8, 41)))
+        vs.append(gen.normal_vector(n, seed=seed * 1000 + p))
+    return mats, ts, ms, vs
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_matches_oracle(gpu, oracle, use_graph):
+    mats, ts, ms, vs = _problems()
+    tol = 1e-9
+    B = mek.Batch(mats, ts, ms, tol, max_restarts=10, use_graph=use_graph)
+    B.upload(vs)
+    B.solve()
+    ys = B.download()
+    print(f"[batch] cycles={B.cycles()} graph={use_graph}", flush=True)
+    for p, ((rp, ci, va), t, m, v, y) in enumerate(zip(mats, ts, ms, vs, ys)):
+        yref, st, log = oracle.expmv(rp, ci, va, v, t, tol, m, 10)
+        s = B.stats(p)
+        assert s["accepted"] == st["restarts_done"], p
+        assert s["attempts"] == st["attempts"], p
+        assert s["converged"] == st["converged"], p
+        assert rel_err(y, yref) <= 1e-8, (p, rel_err(y, yref))
+    B.close()
+
+
+def test_batch_replays_are_identical(gpu):
+    """The captured graph replays must give the eager result on every solve."""
+    mats, ts, ms, vs = _problems(P=6, seed=3)
+    Bg = mek.Batch(mats, ts, ms, 1e-8, max_restarts=0, use_graph=True)
+    Be = mek.Batch(mats, ts, ms, 1e-8, max_restarts=0, use_graph=False)
+    Be.upload(vs)
+    Be.solve()
+    ye = Be.download()
+    Bg.upload(vs)
+    for rep in range(3):
+        Bg.solve()
+        yg = Bg.download()
+        for a, b in zip(yg, ye):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_batch_f32_stencils_and_substeps(gpu, oracle):
+    """FP32-stored stencils (lossless) with a fixed first sub-step count."""
+    mats = [gen.convdiff_3d(12), gen.laplacian_2d(40), gen.convdiff_3d(9)]
+    ts = [0.05, 1.0, 0.02]
+    ms = [30, 20, 25]
+    vs = [gen.normal_vector(r[0].size - 1, seed=i) for i, r in enumerate(mats)]
+    B = mek.Batch(mats, ts, ms, 1e-10, substeps=[4, 2, 1], values_f32=True)
+    B.upload(vs)
+    B.solve()
+    for (rp, ci, va), t, m, v, y in zip(mats, ts, ms, vs, B.download()):
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        A = sp.csr_matrix((va, ci, rp))
+        assert rel_err(y, spla.expm_multiply(t * A, v)) <= 1e-8
