@@ -159,6 +159,30 @@ int mek_batch_stats(const mek_batch* B, int p, mek_solve_stats* out);
 int mek_batch_cycles(const mek_batch* B, int* cycles);
 void mek_batch_destroy(mek_batch* B);
 
+/* ------------------------------------------------------ learned selector */
+
+/* No reference counterpart (the reference fixes m and the first sub-step by
+ * hand, main.cu:1292-1305).  Features: 16 floats per problem computed on the
+ * device (Gershgorin and 1-/inf-norm estimates, nnz-per-row statistics, t,
+ * tol).  MLP: 16 -> 64 -> 64 -> 2, FP32, GELU; forward, backward and Adam on
+ * the device. */
+int mek_features(const mek_csr* A, double t, double tol, float* out16, int device_id);
+/* Features of every problem of a batch (out: nprob x 16). */
+int mek_batch_features(mek_batch* B, float* out);
+typedef struct mek_mlp mek_mlp;
+int mek_mlp_param_count(void);
+int mek_mlp_create(unsigned seed, int device_id, mek_mlp** out);
+int mek_mlp_set_params(mek_mlp* N, const float* params);
+int mek_mlp_get_params(const mek_mlp* N, float* params);
+/* Y (B x 2) = MLP(X (B x 16)) */
+int mek_mlp_forward(mek_mlp* N, const float* X, int B, float* Y);
+/* MSE loss against T (B x 2) and its gradient (param_count floats, may be NULL). */
+int mek_mlp_loss_grad(mek_mlp* N, const float* X, const float* T, int B, float* loss, float* grads);
+/* Mini-batch Adam on the device; *loss = MSE over all N samples afterwards. */
+int mek_mlp_train(mek_mlp* N, const float* X, const float* T, int n, int batch, int epochs, float lr,
+                  unsigned seed, float* loss);
+void mek_mlp_destroy(mek_mlp* N);
+
 /* ---------------------------------------------------------- kernel level */
 
 typedef struct mek_spmv_plan mek_spmv_plan;

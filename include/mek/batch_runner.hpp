@@ -47,6 +47,8 @@ class BatchRunner {
   void solve_device();
   void download_y(const std::vector<double*>& y) const;
   BatchProblemStats stats(int p) const;
+  // The 16 learned-selector features of every problem (P x 16, host).
+  void features(float* out);
   int cycles() const { return cycles_; }
   double solve_ms() const { return solve_ms_; }
   hipStream_t stream() const { return stream_; }

@@ -102,6 +102,22 @@ SIGNATURES = {
     "mek_batch_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(SolveStats)]),
     "mek_batch_cycles": (ctypes.c_int, [_vp, _ip]),
     "mek_batch_destroy": (None, [_vp]),
+    "mek_features": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+    "mek_batch_features": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "mek_mlp_param_count": (ctypes.c_int, []),
+    "mek_mlp_create": (ctypes.c_int, [ctypes.c_uint, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mek_mlp_set_params": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "mek_mlp_get_params": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "mek_mlp_forward": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_float)]),
+    "mek_mlp_loss_grad": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "mek_mlp_train": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_uint, ctypes.POINTER(ctypes.c_float)]),
+    "mek_mlp_destroy": (None, [_vp]),
     "mek_spmv_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp,
                                             ctypes.c_int, _ip, ctypes.POINTER(_vp)]),
     "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -366,6 +382,11 @@ class Batch:
         _check(lib().mek_batch_stats(self.h, int(p), ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in SolveStats._fields_}
 
+    def features(self):
+        out = np.zeros((self.P, 16), np.float32)
+        _check(lib().mek_batch_features(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
     def cycles(self):
         c = ctypes.c_int()
         _check(lib().mek_batch_cycles(self.h, ctypes.byref(c)))
@@ -374,6 +395,74 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().mek_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------- learned selector
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def features(A: CSR, t: float, tol: float = 1e-8, device: int = 0) -> np.ndarray:
+    """The 16 selector features of (A, t, tol), computed on the GPU."""
+    out = np.zeros(16, np.float32)
+    _check(lib().mek_features(A.h, float(t), float(tol), out.ctypes.data_as(_fp), int(device)))
+    return out
+
+
+class Mlp:
+    """16 -> 64 -> 64 -> 2 MLP trained and evaluated by the library's HIP kernels."""
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        h = _vp()
+        _check(lib().mek_mlp_create(int(seed), int(device), ctypes.byref(h)))
+        self.h = h
+        self.nparams = lib().mek_mlp_param_count()
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, np.float32)
+        _check(lib().mek_mlp_get_params(self.h, p.ctypes.data_as(_fp)))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.nparams
+        _check(lib().mek_mlp_set_params(self.h, p.ctypes.data_as(_fp)))
+
+    def forward(self, X) -> np.ndarray:
+        X = _f32(X)
+        Y = np.zeros((X.shape[0], 2), np.float32)
+        _check(lib().mek_mlp_forward(self.h, X.ctypes.data_as(_fp), X.shape[0], Y.ctypes.data_as(_fp)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X, T = _f32(X), _f32(T)
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().mek_mlp_loss_grad(self.h, X.ctypes.data_as(_fp), T.ctypes.data_as(_fp), X.shape[0],
+                                       ctypes.byref(loss), g.ctypes.data_as(_fp)))
+        return loss.value, g
+
+    def train(self, X, T, batch=256, epochs=100, lr=1e-3, seed=0) -> float:
+        X, T = _f32(X), _f32(T)
+        loss = ctypes.c_float()
+        _check(lib().mek_mlp_train(self.h, X.ctypes.data_as(_fp), T.ctypes.data_as(_fp), X.shape[0],
+                                   int(batch), int(epochs), float(lr), int(seed), ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mek_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

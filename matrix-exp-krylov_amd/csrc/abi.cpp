@@ -15,6 +15,7 @@
 #include "mek/arnoldi_runner.hpp"
 #include "mek/batch_runner.hpp"
 #include "mek/csr_host.hpp"
+#include "mek/mlp.hpp"
 
 struct mek_csr {
   CSRHost A;
@@ -26,6 +27,10 @@ struct mek_runner {
 
 struct mek_batch {
   std::unique_ptr<mek::BatchRunner> B;
+};
+
+struct mek_mlp {
+  std::unique_ptr<mek::Mlp> N;
 };
 
 struct mek_spmv_plan {
@@ -379,6 +384,73 @@ int mek_batch_cycles(const mek_batch* B, int* cycles) {
 }
 
 void mek_batch_destroy(mek_batch* B) { delete B; }
+
+// ------------------------------------------------------- learned selector
+
+int mek_features(const mek_csr* A, double t, double tol, float* out16, int device_id) {
+  return guarded([&] {
+    need(A && out16, "csr, out");
+    mek::features_csr(A->A, t, tol, out16, device_id);
+  });
+}
+
+int mek_batch_features(mek_batch* B, float* out) {
+  return guarded([&] {
+    need(B && out, "batch, out");
+    B->B->features(out);
+  });
+}
+
+int mek_mlp_param_count(void) { return mek::Mlp::param_count(); }
+
+int mek_mlp_create(unsigned seed, int device_id, mek_mlp** out) {
+  return guarded([&] {
+    need(out != nullptr, "out");
+    auto h = std::make_unique<mek_mlp>();
+    h->N = std::make_unique<mek::Mlp>(seed, device_id);
+    *out = h.release();
+  });
+}
+
+int mek_mlp_set_params(mek_mlp* N, const float* p) {
+  return guarded([&] {
+    need(N && p, "mlp, params");
+    N->N->set_params(p);
+  });
+}
+
+int mek_mlp_get_params(const mek_mlp* N, float* p) {
+  return guarded([&] {
+    need(N && p, "mlp, params");
+    N->N->get_params(p);
+  });
+}
+
+int mek_mlp_forward(mek_mlp* N, const float* X, int B, float* Y) {
+  return guarded([&] {
+    need(N && X && Y && B >= 1, "mlp, X, Y, B");
+    N->N->forward(X, B, Y);
+  });
+}
+
+int mek_mlp_loss_grad(mek_mlp* N, const float* X, const float* T, int B, float* loss, float* grads) {
+  return guarded([&] {
+    need(N && X && T && B >= 1, "mlp, X, T, B");
+    const float l = N->N->loss_grad(X, T, B, grads);
+    if (loss) *loss = l;
+  });
+}
+
+int mek_mlp_train(mek_mlp* N, const float* X, const float* T, int n, int batch, int epochs, float lr,
+                  unsigned seed, float* loss) {
+  return guarded([&] {
+    need(N && X && T && n >= 1 && batch >= 1 && epochs >= 0, "mlp training arguments");
+    const float l = N->N->train(X, T, n, batch, epochs, lr, seed);
+    if (loss) *loss = l;
+  });
+}
+
+void mek_mlp_destroy(mek_mlp* N) { delete N; }
 
 // ----------------------------------------------------------- kernel level
 

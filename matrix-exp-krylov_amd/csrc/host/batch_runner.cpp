@@ -8,6 +8,7 @@
 
 #include "../kernels/batch.hpp"
 #include "../kernels/launch.hpp"
+#include "../kernels/learn.hpp"
 #include "hip_check.hpp"
 #include "mek/device_context.hpp"
 
@@ -223,6 +224,36 @@ BatchProblemStats BatchRunner::stats(int p) const {
   r.t_done = s[kBTDone];
   r.last_residual = s[kBLastRes];
   return r;
+}
+
+void BatchRunner::features(float* out) {
+  MEK_HIP(hipSetDevice(device_));
+  std::vector<double> t(P_), tol(P_);
+  for (int p = 0; p < P_; ++p) {
+    t[p] = pscal_init_[p * kPScal + kBT];
+    tol[p] = pscal_init_[p * kPScal + kBTol];
+  }
+  int* d_p0 = upload(row0_);
+  double* d_t = upload(t);
+  double* d_tol = upload(tol);
+  double *d_col = nullptr, *d_stats = nullptr;
+  float* d_f = nullptr;
+  hipError_t e = hipMalloc(&d_col, sizeof(double) * n_tot_);
+  if (e == hipSuccess) e = hipMalloc(&d_stats, sizeof(double) * 8 * P_);
+  if (e == hipSuccess) e = hipMalloc(&d_f, sizeof(float) * kNumFeatures * P_);
+  if (e == hipSuccess) e = hipMemsetAsync(d_col, 0, sizeof(double) * n_tot_, stream_);
+  if (e == hipSuccess) {
+    const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
+    feature_launch(d_row_ptr_, d_col_idx_, d_vals_, b.vals_f32, d_p0, P_, d_col, d_stats, d_t, d_tol, d_f,
+                   stream_);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_f, sizeof(float) * kNumFeatures * P_, hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  for (void* p : {static_cast<void*>(d_p0), static_cast<void*>(d_t), static_cast<void*>(d_tol),
+                  static_cast<void*>(d_col), static_cast<void*>(d_stats), static_cast<void*>(d_f)})
+    if (p) (void)hipFree(p);
+  MEK_HIP(e);
 }
 
 void BatchRunner::destroy() {
