@@ -96,8 +96,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS))
     ap.add_argument("--step-control", default="expokit", choices=["expokit", "reference"])
-    ap.add_argument("--gs", default="always", choices=["always", "dgks", "never"])
-    ap.add_argument("--no-fuse", action="store_true", help="SpMV and first-pass dots as two kernels")
+    ap.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"],
+                    help="second GS pass: dgks = only when ||w'|| < 0.1 ||w|| (default), always = reference MGS2 semantics")
+    ap.add_argument("--fuse", action="store_true", help="first-pass dots inside the SpMV kernel")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -121,7 +122,7 @@ def main():
     A = mek.CSR.from_arrays(rp, ci, va)
     R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
                    gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=True,
-                   fuse_spmv_dot=not args.no_fuse)
+                   fuse_spmv_dot=args.fuse)
     A.close()
     d_v = mek.DeviceBuffer.from_array(v)
     d_y = mek.DeviceBuffer(8 * n)
@@ -195,7 +196,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "t": w["t"], "m": w["m"],
                    "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
-                   "fused_spmv_dot": not args.no_fuse, "seed": 0},
+                   "fused_spmv_dot": args.fuse, "seed": 0},
         "rel_err": rel,
         "solve": {"substeps": stats["accepted"], "expm_attempts": stats["attempts"],
                   "krylov_columns": stats["spmv_count"], "extra_spmv": stats["extra_spmv"],

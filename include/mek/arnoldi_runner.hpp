@@ -62,7 +62,7 @@ struct SolverOptions {
   bool values_f32 = false;  // store matrix values in FP32 on the device
   bool verbose = true;      // print the reference's per-column and [STEP] lines
   bool timing = false;      // HIP-event timing of every kernel launch
-  bool fuse_spmv_dot = true;  // first-pass dots inside the SpMV kernel
+  bool fuse_spmv_dot = false;  // first-pass dots inside the SpMV kernel
 };
 
 struct StepRecord {

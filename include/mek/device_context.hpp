@@ -41,10 +41,12 @@ struct DeviceContext {
   double* d_expm_ws = nullptr;    // global expm workspace for large m
   double* h_status = nullptr;     // pinned host copy of the scalar slots
   int grid = 0;                   // blocks of the streaming kernels
+  int spmv_grid = 1;              // blocks of the SpMV kernel
 
   int local_rows = 0;
   long long local_nnz = 0;
   int allocated_m_max = 0;
+  bool log = true;                // print the reference's allocation lines
 
   explicit DeviceContext(int dev_id) : device_id(dev_id) {}
 

@@ -270,7 +270,7 @@ class Runner:
 
     def __init__(self, A: CSR, m=30, t=1.0, tol=1e-8, max_restarts=10, reorthogonalize=True,
                  step_control="reference", gs_mode=-1, dgks_eta=0.1, values_f32=False,
-                 verbose=False, timing=False, fuse_spmv_dot=True, device=0):
+                 verbose=False, timing=False, fuse_spmv_dot=False, device=0):
         p = ArnoldiParams(int(m), float(t), float(tol), int(max_restarts), 1 if reorthogonalize else 0)
         o = SolverOptions()
         lib().mek_solver_options_default(ctypes.byref(o))

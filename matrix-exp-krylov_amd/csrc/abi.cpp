@@ -462,8 +462,7 @@ int mek_spmv_plan_create(int n, long long nnz, const int* d_row_ptr, const int* 
     need(h_row_ptr[n] == nnz, "h_row_ptr[n] == nnz");
     auto P = std::make_unique<mek_spmv_plan>();
     const std::vector<int> blocks = mek::build_row_blocks(h_row_ptr, n, mek::kSpmvNnzCap, mek::kSpmvRowCap);
-    int grid = static_cast<int>(std::min<long long>(1024, std::max<long long>(8, (n + 1023) / 1024)));
-    grid = (grid + 7) / 8 * 8;
+    const int grid = std::max(1, std::min(static_cast<int>(blocks.size()) - 1, mek::kMaxGrid));
     MEK_HIP(hipMalloc(&P->d_blocks, sizeof(int) * blocks.size()));
     MEK_HIP(hipMemcpy(P->d_blocks, blocks.data(), sizeof(int) * blocks.size(), hipMemcpyHostToDevice));
     MEK_HIP(hipMalloc(&P->d_partials, sizeof(double) * grid));
@@ -480,6 +479,7 @@ int mek_spmv_plan_create(int n, long long nnz, const int* d_row_ptr, const int* 
     b.iscal = P->d_iscal;
     b.partials = P->d_partials;
     b.grid = grid;
+    b.spmv_grid = grid;
     *out = P.release();
   });
 }

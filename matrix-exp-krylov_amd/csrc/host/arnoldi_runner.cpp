@@ -89,6 +89,7 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
   dc.device_id = device_ids[0];
   dc.init_handles();
   dc.create_streams();
+  dc.log = options.verbose;
   const int alloc_m = std::max(params.m, ts_params.m_max > 0 ? ts_params.m_max : params.m);
   dc.alloc_from(partitions[0], global_cols, alloc_m, options.values_f32);
   static const CSRHost::GhostMap kEmpty;
@@ -168,33 +169,34 @@ void ArnoldiRunner::run_columns(int m_cols) {
   const double bspmv = static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
                        4.0 * (n + 1) + 16.0 * n;
   const int gs_mode = (options.gs_mode >= 0) ? options.gs_mode : (params.reorthogonalize ? 1 : 0);
+  // quantity slot for the SpMV's ||w||^2 partial when the dots kernel
+  // computes the norm itself (partials then share one block stride)
+  const int spare_q = dc.allocated_m_max + 3;
   for (int j = 0; j < m_cols; ++j) {
     const int nq = j + 1;
-    if (nq <= mek::kMaxFusedDots) {
-      if (options.fuse_spmv_dot) {
-        begin_timed(mek::kKSpmv);
-        mek::launch_spmv_dot(b, j, nq, nq, s);
-        end_timed(mek::kKSpmv, bspmv + 8.0 * n * nq);
-      } else {
-        begin_timed(mek::kKSpmv);
-        mek::launch_spmv_dot(b, j, 0, nq, s);
-        end_timed(mek::kKSpmv, bspmv);
+    if (nq <= mek::kMaxFusedDots && options.fuse_spmv_dot) {
+      begin_timed(mek::kKSpmv);
+      mek::launch_spmv_dot(b, j, nq, nq, s);
+      end_timed(mek::kKSpmv, bspmv + 8.0 * n * nq);
+      mek::launch_fin_h1(b, nq, b.spmv_grid, s);
+    } else {
+      begin_timed(mek::kKSpmv);
+      mek::launch_spmv_dot(b, j, 0, spare_q, s);
+      end_timed(mek::kKSpmv, bspmv);
+      for (int c0 = 0; c0 < nq; c0 += mek::kMaxFusedDots) {
+        const int cnt = std::min(mek::kMaxFusedDots, nq - c0);
         begin_timed(mek::kKDots);
-        mek::launch_dots(b, dc.d_w, 0, nq, 0, 0, s);
-        end_timed(mek::kKDots, 8.0 * n * (nq + 1));
+        mek::launch_dots(b, dc.d_w, c0, cnt, (c0 + cnt == nq) ? 1 : 0, c0, s);
+        end_timed(mek::kKDots, 8.0 * n * (cnt + 1));
       }
-      mek::launch_fin_h1(b, nq, s);
+      mek::launch_fin_h1(b, nq, b.grid, s);
+    }
+    if (nq <= mek::kMaxFusedDots) {
       begin_timed(mek::kKUpdateDot);
       mek::launch_update_dot(b, j, s);
       end_timed(mek::kKUpdateDot, 8.0 * n * (nq + 2));
     } else {
-      // generic path for j+1 > 64: dots in chunks of 64 columns
-      begin_timed(mek::kKSpmv);
-      mek::launch_spmv_dot(b, j, mek::kMaxFusedDots, nq, s);
-      end_timed(mek::kKSpmv, bspmv + 8.0 * n * mek::kMaxFusedDots);
-      for (int c0 = mek::kMaxFusedDots; c0 < nq; c0 += mek::kMaxFusedDots)
-        mek::launch_dots(b, dc.d_w, c0, std::min(mek::kMaxFusedDots, nq - c0), 0, c0, s);
-      mek::launch_fin_h1(b, nq, s);
+      // generic path for j+1 > 64: update, then dots in chunks of 64 columns
       double* vn = dc.d_V_m + static_cast<long long>(j + 1) * dc.ld;
       mek::launch_update(b, dc.d_w, vn, b.h1, nq, s);
       for (int c0 = 0; c0 < nq; c0 += mek::kMaxFusedDots) {
@@ -202,11 +204,11 @@ void ArnoldiRunner::run_columns(int m_cols) {
         mek::launch_dots(b, vn, c0, cnt, (c0 + cnt == nq) ? 1 : 0, c0, s);
       }
     }
-    mek::launch_fin_h2(b, j, gs_mode, options.dgks_eta, s);
+    mek::launch_fin_h2(b, j, s);
     begin_timed(mek::kKUpdateNorm);
-    mek::launch_update_norm(b, j, s);
+    mek::launch_update_norm(b, j, gs_mode, options.dgks_eta, s);
     end_timed(mek::kKUpdateNorm, 8.0 * n * (nq + 2));
-    mek::launch_fin_col(b, j, s);
+    mek::launch_fin_col(b, j, gs_mode, options.dgks_eta, s);
     MEK_HIP_LAUNCHED();
   }
   stats.spmv_count += m_cols;  // launched; columns after a breakdown exit at once
@@ -242,7 +244,7 @@ void ArnoldiRunner::lift(int ncoef, double skip_below) {
   begin_timed(mek::kKLift);
   mek::launch_lift(b, ncoef, skip_below, dc.stream_compute);
   end_timed(mek::kKLift, 8.0 * dc.local_rows * (ncoef + 1));
-  mek::launch_fin_norm(b, mek::kYNorm, dc.stream_compute);
+  mek::launch_fin_norm(b, mek::kYNorm, b.grid, dc.stream_compute);
   MEK_HIP_LAUNCHED();
 }
 
@@ -371,7 +373,7 @@ void ArnoldiRunner::expokit_controller() {
                              dc.stream_compute);
       end_timed(mek::kKSpmv, static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
                                  4.0 * (dc.local_rows + 1) + 16.0 * dc.local_rows);
-      mek::launch_fin_norm(b, mek::kAvNorm, dc.stream_compute);
+      mek::launch_fin_norm(b, mek::kAvNorm, b.spmv_grid, dc.stream_compute);
       MEK_HIP_LAUNCHED();
       stats.extra_spmv++;
     }

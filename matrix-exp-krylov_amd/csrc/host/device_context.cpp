@@ -30,11 +30,12 @@ std::vector<int> build_row_blocks(const int* row_ptr, int n, int nnz_cap, int ro
   return starts;
 }
 
-// Blocks of the streaming kernels: enough to fill 256 CUs several times over,
-// at least 4 rows per thread, a multiple of the 8 XCDs.
+// Blocks of the streaming kernels: up to 4096 (16 per CU, measured to need
+// >= 2048 for full HBM bandwidth), at least 4 rows per thread, a multiple of
+// the 8 XCDs.
 static int streaming_grid(int n) {
   long long g = (static_cast<long long>(n) + kBlock * 4 - 1) / (kBlock * 4);
-  g = std::min<long long>(g, 1024);
+  g = std::min<long long>(g, kMaxGrid);
   g = std::max<long long>(g, 8);
   g = (g + 7) / 8 * 8;
   return static_cast<int>(g);
@@ -62,7 +63,7 @@ void DeviceContext::alloc_from(const CSRHost& part, int /*global_cols*/, int m_a
   MEK_HIP(hipSetDevice(device_id));
   local_rows = part.rows;
   local_nnz = part.nnz;
-  std::cout << "GPU " << device_id << " allocating: local_rows=" << local_rows
+  if (log) std::cout << "GPU " << device_id << " allocating: local_rows=" << local_rows
             << ", local_nnz=" << local_nnz << std::endl;
   hipStream_t s = stream_compute;
   values_f32 = store_f32;
@@ -90,7 +91,8 @@ void DeviceContext::alloc_from(const CSRHost& part, int /*global_cols*/, int m_a
   MEK_HIP(hipMalloc(&d_small, sizeof(double) * small_doubles()));
   MEK_HIP(hipMemsetAsync(d_small, 0, sizeof(double) * small_doubles(), s));
   grid = mek::streaming_grid(local_rows);
-  MEK_HIP(hipMalloc(&d_partials, sizeof(double) * (ncols + 2) * static_cast<size_t>(grid)));
+  // partials: m+4 quantities for either the streaming or the SpMV grid
+  MEK_HIP(hipMalloc(&d_partials, sizeof(double) * (ncols + 2) * static_cast<size_t>(mek::kMaxGrid)));
   const int kk = allocated_m_max + 2;
   if (kk > mek::kExpmMaxKLds)
     MEK_HIP(hipMalloc(&d_expm_ws, sizeof(double) * 6 * static_cast<size_t>(kk) * kk));
@@ -105,10 +107,11 @@ void DeviceContext::create_descriptors(const CSRHost& part, int /*global_cols*/,
   std::vector<int> blocks =
       mek::build_row_blocks(part.row_ptr.data(), part.rows, mek::kSpmvNnzCap, mek::kSpmvRowCap);
   n_row_blocks = static_cast<int>(blocks.size()) - 1;
+  spmv_grid = std::max(1, std::min(n_row_blocks, mek::kMaxGrid));
   const size_t bytes = sizeof(int) * blocks.size();
   MEK_HIP(hipMalloc(&d_row_blocks, bytes));
   MEK_HIP(hipMemcpy(d_row_blocks, blocks.data(), bytes, hipMemcpyHostToDevice));
-  std::cout << "  GPU " << device_id << " SpMV buffer size: " << bytes << " bytes" << std::endl;
+  if (log) std::cout << "  GPU " << device_id << " SpMV buffer size: " << bytes << " bytes" << std::endl;
 }
 
 mek::ArnoldiBufs DeviceContext::bufs() const {
@@ -136,6 +139,7 @@ mek::ArnoldiBufs DeviceContext::bufs() const {
   b.ldh = static_cast<int>(M + 2);
   b.partials = d_partials;
   b.grid = grid;
+  b.spmv_grid = spmv_grid;
   return b;
 }
 

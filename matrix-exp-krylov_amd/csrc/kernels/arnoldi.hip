@@ -1,34 +1,35 @@
 // Arnoldi column kernels for gfx950 (wave64).
 //
 // One Arnoldi column j (reference: main.cu:1722-1942, spmv_on_off +
-// orthogonalize_mgs + normalize_new_vector) is three streaming sweeps over
-// the basis plus three one-workgroup finalisers:
+// orthogonalize_mgs + normalize_new_vector) is a SpMV and three streaming
+// sweeps over the basis, each followed by a finaliser that reduces the
+// per-block partial sums (one workgroup per reduced quantity):
 //
-//   K1 spmv_dot     w = A q_j  (row-blocked CSR, LDS-staged products) and the
-//                   partial sums of h1 = V^T w and ||w||^2          (sweep 1)
+//   K1 spmv(+dot)   w = A q_j (row-blocked CSR, LDS-staged products); partial
+//                   sums of ||w||^2 and, when fused, of h1 = V^T w
+//   K1b dots        h1 = V^T w partial sums (when not fused)     (sweep 1)
 //   F1              h1, ||w||^2
 //   K2 update_dot   w' = w - V h1, stored unnormalised in V[:, j+1], and the
-//                   partial sums of h2 = V^T w' and ||w'||^2        (sweep 2)
-//   F2              h2, ||w'||^2, decision on the second pass
-//   K3 update_norm  w'' = w' - V h2 in place, partial ||w''||^2     (sweep 3)
+//                   partial sums of h2 = V^T w' and ||w'||^2      (sweep 2)
+//   F2              h2, ||w'||^2
+//   K3 update_norm  second pass w'' = w' - V h2 in place, ||w''||^2 (sweep 3)
 //   F3              H(:, j), breakdown test, colscale[j+1] = 1/||w''||
 //
 // This is classical Gram-Schmidt with a second pass (CGS2), which for the
 // reference's unconditional re-orthogonalisation (main.cu:1829-1866) gives
 // H(i,j) = h1_i + h2_i exactly as the reference's h_first + h_corr.  When the
-// second pass is conditional (DGKS test ||w'|| < eta ||w||) K3/F3 exit at once
-// on columns that do not need it.
+// second pass is conditional (DGKS test ||w'|| < eta ||w||), K3 and F3 decide
+// from ||w||, ||w'|| and skip the sweep on columns that do not need it.
 //
-// Every reduction is deterministic: per-thread partials, fixed-order
-// in-block reduction (DPP), per-block partials reduced by one workgroup in
-// block order.  After a breakdown every later kernel of the attempt returns
-// immediately, so the host can enqueue all m columns without reading back.
+// Every reduction is deterministic: per-thread partials over a fixed row set,
+// a fixed-order in-block reduction (DPP), and per-block partials summed in
+// block order.  After a breakdown every later kernel of the attempt returns at
+// once, so the host can enqueue all m columns without reading back.
 #include "common.hpp"
 
 namespace mek {
 
 constexpr int kNnzCap = 2048;  // nnz of one short-row SpMV block (LDS products)
-constexpr int kMaxQuantities = 512;  // reduced quantities per finaliser (m <= 500)
 
 template <typename VT>
 __device__ __forceinline__ double load_val(const void* vals, long long k) {
@@ -38,9 +39,8 @@ __device__ __forceinline__ double load_val(const void* vals, long long k) {
 // Block-reduce NQ per-thread values (uniform control flow) and write them as
 // partials[q*grid + blk].  `red` is LDS scratch of kWavesPerBlock*NQ doubles.
 template <int NQ>
-__device__ __forceinline__ void block_partials(const double (&v)[NQ], int nq,
-                                               double* red, double* partials,
-                                               int grid, int blk) {
+__device__ __forceinline__ void block_partials(const double (&v)[NQ], int nq, double* red,
+                                               double* partials, int grid, int blk) {
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
 #pragma unroll
@@ -56,6 +56,62 @@ __device__ __forceinline__ void block_partials(const double (&v)[NQ], int nq,
     for (int w = 1; w < kWavesPerBlock; ++w) s += red[w * NQ + q];
     partials[static_cast<long long>(q) * grid + blk] = s;
   }
+}
+
+// Sum of partials[q*grid + 0..grid) by one workgroup: thread t sums blocks
+// t, t+256, ... in order, then a fixed-order block reduction.  Uniform flow.
+__device__ double block_sum_quantity(const double* partials, int grid, int q, double* red) {
+  double s = 0.0;
+  const double* p = partials + static_cast<long long>(q) * grid;
+  for (int g = threadIdx.x; g < grid; g += kBlock) s += p[g];
+  s = wave_sum(s);
+  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = s;
+  __syncthreads();
+  const double tot = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return tot;
+}
+
+__device__ __forceinline__ long long gtid() {
+  return static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+}
+__device__ __forceinline__ long long gstride() { return static_cast<long long>(gridDim.x) * kBlock; }
+
+// Copy n coefficients into LDS (uniform per kernel) so the column loops read
+// them by broadcast instead of issuing per-iteration global loads.
+__device__ __forceinline__ void stage(double* dst, const double* src, int n) {
+  for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+}
+
+// The second-pass decision, from ||w||^2 and ||w'||^2 of the column.
+// gs_mode 0: never, 1: always (reference), 2: DGKS ||w'|| < eta ||w||.
+__device__ __forceinline__ int reorth_needed(const ArnoldiBufs& b, int gs_mode, double eta) {
+  const double wp2 = b.scal[kWpNorm2];
+  if (!(wp2 == wp2)) return 0;  // NaN: the column breaks down either way
+  if (gs_mode == 0) return 0;
+  if (gs_mode == 1) return 1;
+  return (sqrt(wp2) < eta * sqrt(b.scal[kWNorm2])) ? 1 : 0;
+}
+
+constexpr int kMaxStaged = 512;  // coefficients staged in LDS (m <= 500)
+constexpr int kChunk = 16;       // basis columns loaded back-to-back per row
+
+// sum_{i<nq} c_i * (s_i * V[i*ld + r]), loading kChunk columns at a time so
+// their loads are in flight together; c and s are LDS-staged.
+__device__ __forceinline__ double sweep_combination(const double* __restrict__ V, long long ld,
+                                                    long long r, int nq, const double* c,
+                                                    const double* s) {
+  double sum = 0.0;
+  for (int i0 = 0; i0 < nq; i0 += kChunk) {
+    double v[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      v[k] = (i0 + k < nq) ? V[static_cast<long long>(i0 + k) * ld + r] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      if (i0 + k < nq) sum += c[i0 + k] * (s[i0 + k] * v[k]);
+  }
+  return sum;
 }
 
 // ------------------------------------------------------------------- K1
@@ -75,8 +131,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
   __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
   const int lb = xcd_block_id();
   const int tid = threadIdx.x;
+  const int grid = gridDim.x;
   const double* __restrict__ x = x_in;
   const double xs = xs_ptr ? *xs_ptr : 1.0;
+  // fully unrolled column loops below: the coefficients stay in scalar registers
   const double* __restrict__ cs = b.colscale;
   const int* __restrict__ rp = b.row_ptr;
   const int* __restrict__ ci = b.col_idx;
@@ -87,8 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
   for (int q = 0; q <= MAXJ; ++q) acc[q] = 0.0;
 
   const long long nrb = b.n_row_blocks;
-  const int rb0 = static_cast<int>(nrb * lb / b.grid);
-  const int rb1 = static_cast<int>(nrb * (lb + 1) / b.grid);
+  const int rb0 = static_cast<int>(nrb * lb / grid);
+  const int rb1 = static_cast<int>(nrb * (lb + 1) / grid);
   for (int rb = rb0; rb < rb1; ++rb) {
     const int r0 = b.row_blocks[rb];
     const int r1 = b.row_blocks[rb + 1];
@@ -129,156 +187,257 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
       __syncthreads();
     }
   }
-  block_partials<MAXJ + 1>(acc, ndots, red, b.partials, b.grid, lb);
+  block_partials<MAXJ + 1>(acc, ndots, red, b.partials, grid, lb);
   __syncthreads();
   double nv[1] = {acc[MAXJ]};
-  block_partials<1>(nv, 1, red, b.partials + static_cast<long long>(normq) * b.grid, b.grid, lb);
+  block_partials<1>(nv, 1, red, b.partials + static_cast<long long>(normq) * grid, grid, lb);
 }
 
-// ------------------------------------------------------------------- K2
+// ------------------------------------------- K2 and K1b: column-split sweeps (nq <= 64)
+//
+// Wave w of the workgroup owns basis columns w, w+4, w+8, ... (up to MAXK of
+// them); a tile is 64 consecutive rows, one per lane.  Column indices past nq
+// are clamped to nq-1 with a zero coefficient, so every load of a tile is
+// issued unconditionally and back-to-back, and the coefficients are
+// wave-uniform values computed once per kernel.
 
-// w' = w - sum_{i<=j} h1_i q_i, written to V[:, j+1]; partials of q_i . w'
-// (i <= j) and ||w'||^2 (index j+1).  The q_i of a row are loaded once and
-// kept in registers for both uses.
-template <int MAXJ>
-__global__ __launch_bounds__(kBlock) void k_update_dot(ArnoldiBufs b, int j) {
+// w' = w - sum_i h1_i q_i into V[:, j+1] and partials of q_i . w' and ||w'||^2.
+// The four waves' partial combinations meet in double-buffered LDS (one
+// barrier per tile).  h2 partial of column i = cs_i * sum_r V[r,i] w'[r].
+template <int MAXK>
+__global__ __launch_bounds__(kBlock) void k_update_dot_cs(ArnoldiBufs b, int j) {
   if (b.iscal[kBreakdown]) return;
-  __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
-  const int lb = xcd_block_id();
+  __shared__ double part[2][kWavesPerBlock][kWave];
+  __shared__ double red[kWavesPerBlock * (4 * MAXK + 1)];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
   const int nq = j + 1;
-  const double* __restrict__ cs = b.colscale;
-  const double* __restrict__ h = b.h1;
+  long long col[MAXK];
+  double c[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const int i = wave + 4 * k;
+    const int ic = i < nq ? i : nq - 1;
+    col[k] = static_cast<long long>(ic) * b.ld;
+    c[k] = (i < nq) ? b.h1[ic] * b.colscale[ic] : 0.0;
+  }
+  const double* __restrict__ V = b.V;
   const double* __restrict__ w = b.w;
   double* __restrict__ out = b.V + static_cast<long long>(j + 1) * b.ld;
-  double acc[MAXJ + 1];
+  double acc[MAXK];
 #pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) acc[q] = 0.0;
-
-  const long long per = (static_cast<long long>(b.n) + b.grid - 1) / b.grid;
-  const long long r0 = per * lb;
-  const long long r1 = min(static_cast<long long>(b.n), r0 + per);
-  for (long long r = r0 + threadIdx.x; r < r1; r += kBlock) {
-    double q[MAXJ];
-    double a = w[r];
+  for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
+  double nrm = 0.0;
+  const long long ntiles = (static_cast<long long>(b.n) + kWave - 1) / kWave;
+  int buf = 0;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long r = t * kWave + lane;
+    const long long rc = r < b.n ? r : b.n - 1;
+    double v[MAXK];
 #pragma unroll
-    for (int i = 0; i < MAXJ; ++i) {
-      if (i < nq) {
-        q[i] = cs[i] * b.V[static_cast<long long>(i) * b.ld + r];
-        a -= h[i] * q[i];
+    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
+    const double wr = w[rc];
+    double p = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
+    part[buf][wave][lane] = p;
+    __syncthreads();
+    const double a = wr - ((part[buf][0][lane] + part[buf][1][lane]) +
+                           (part[buf][2][lane] + part[buf][3][lane]));
+    if (r < b.n) {
+      if (wave == 0) {
+        out[r] = a;
+        nrm += a * a;
       }
+#pragma unroll
+      for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
     }
-    out[r] = a;
-#pragma unroll
-    for (int i = 0; i < MAXJ; ++i)
-      if (i < nq) acc[i] += q[i] * a;
-    acc[MAXJ] += a * a;
+    buf ^= 1;
   }
-  double o[MAXJ + 1];
+  // per-column block partials: column wave + 4k from this wave
 #pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) o[q] = (q < nq) ? acc[q] : ((q == nq) ? acc[MAXJ] : 0.0);
-  block_partials<MAXJ + 1>(o, nq + 1, red, b.partials, b.grid, lb);
+  for (int k = 0; k < MAXK; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[wave + 4 * k] = s;
+  }
+  const double sn = wave_sum(nrm);
+  if (threadIdx.x == 0) red[4 * MAXK] = sn;
+  __syncthreads();
+  for (int q = threadIdx.x; q <= nq; q += kBlock) {
+    const double s = (q < nq) ? b.colscale[q] * red[q] : red[4 * MAXK];
+    b.partials[static_cast<long long>(q) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// partials of q_i . x for i in [i0, i0+nq) (quantity i - i0 + qoff) and, if
+// with_norm, ||x||^2 at quantity qoff + nq.  No LDS exchange is needed.
+template <int MAXK>
+__global__ __launch_bounds__(kBlock) void k_dots_cs(ArnoldiBufs b, const double* xv, int i0, int nq,
+                                                    int with_norm, int qoff) {
+  if (b.iscal[kBreakdown]) return;
+  __shared__ double red[kWavesPerBlock * (4 * MAXK + 1)];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  long long col[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const int i = wave + 4 * k;
+    col[k] = static_cast<long long>(i0 + (i < nq ? i : nq - 1)) * b.ld;
+  }
+  const double* __restrict__ V = b.V;
+  double acc[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
+  double nrm = 0.0;
+  const long long ntiles = (static_cast<long long>(b.n) + kWave - 1) / kWave;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long r = t * kWave + lane;
+    if (r >= b.n) continue;
+    double v[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + r];
+    const double a = xv[r];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
+    if (wave == 0) nrm += a * a;
+  }
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[wave + 4 * k] = s;
+  }
+  const double sn = wave_sum(nrm);
+  if (threadIdx.x == 0) red[4 * MAXK] = sn;
+  __syncthreads();
+  const int ntot = nq + (with_norm ? 1 : 0);
+  for (int q = threadIdx.x; q < ntot; q += kBlock) {
+    const double s = (q < nq) ? b.colscale[i0 + q] * red[q] : red[4 * MAXK];
+    b.partials[static_cast<long long>(qoff + q) * gridDim.x + blockIdx.x] = s;
+  }
 }
 
 // ------------------------------------------------------------------- K3
 
-// w'' = w' - sum_{i<=j} h2_i q_i in place in V[:, j+1]; partial ||w''||^2.
-__global__ __launch_bounds__(kBlock) void k_update_norm(ArnoldiBufs b, int j) {
-  if (b.iscal[kBreakdown] || !b.iscal[kReorth]) return;
+// Second pass w'' = w' - sum_{i<=j} h2_i q_i in place in V[:, j+1]; partial
+// ||w''||^2.  Exits at once when the column needs no second pass.
+__global__ __launch_bounds__(kBlock) void k_update_norm(ArnoldiBufs b, int j, int gs_mode, double eta) {
+  if (b.iscal[kBreakdown] || !reorth_needed(b, gs_mode, eta)) return;
   __shared__ double red[kWavesPerBlock];
-  const int lb = xcd_block_id();
+  __shared__ double cs[kMaxStaged];
+  __shared__ double h[kMaxStaged];
   const int nq = j + 1;
-  const double* __restrict__ cs = b.colscale;
-  const double* __restrict__ h = b.h2;
+  stage(cs, b.colscale, nq);
+  stage(h, b.h2, nq);
+  __syncthreads();
   double* __restrict__ io = b.V + static_cast<long long>(j + 1) * b.ld;
   double acc = 0.0;
-  const long long per = (static_cast<long long>(b.n) + b.grid - 1) / b.grid;
-  const long long r0 = per * lb;
-  const long long r1 = min(static_cast<long long>(b.n), r0 + per);
-  for (long long r = r0 + threadIdx.x; r < r1; r += kBlock) {
+  for (long long r = gtid(); r < b.n; r += gstride()) {
     double a = io[r];
-    for (int i = 0; i < nq; ++i) a -= h[i] * (cs[i] * b.V[static_cast<long long>(i) * b.ld + r]);
+    a -= sweep_combination(b.V, b.ld, r, nq, h, cs);
     io[r] = a;
     acc += a * a;
   }
   double o[1] = {acc};
-  block_partials<1>(o, 1, red, b.partials, b.grid, lb);
+  block_partials<1>(o, 1, red, b.partials, gridDim.x, blockIdx.x);
+}
+
+// Column-split form of K3 for nq <= 64 (same wave/column ownership and LDS
+// exchange as k_update_dot_cs).
+template <int MAXK>
+__global__ __launch_bounds__(kBlock) void k_update_norm_cs(ArnoldiBufs b, int j, int gs_mode,
+                                                           double eta) {
+  if (b.iscal[kBreakdown] || !reorth_needed(b, gs_mode, eta)) return;
+  __shared__ double part[2][kWavesPerBlock][kWave];
+  __shared__ double red[kWavesPerBlock];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const int nq = j + 1;
+  long long col[MAXK];
+  double c[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const int i = wave + 4 * k;
+    const int ic = i < nq ? i : nq - 1;
+    col[k] = static_cast<long long>(ic) * b.ld;
+    c[k] = (i < nq) ? b.h2[ic] * b.colscale[ic] : 0.0;
+  }
+  const double* __restrict__ V = b.V;
+  double* __restrict__ io = b.V + static_cast<long long>(j + 1) * b.ld;
+  double nrm = 0.0;
+  const long long ntiles = (static_cast<long long>(b.n) + kWave - 1) / kWave;
+  int buf = 0;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long r = t * kWave + lane;
+    const long long rc = r < b.n ? r : b.n - 1;
+    double v[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
+    const double wr = (wave == 0) ? io[rc] : 0.0;
+    double p = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
+    part[buf][wave][lane] = p;
+    __syncthreads();
+    if (wave == 0 && r < b.n) {
+      const double a = wr - ((part[buf][0][lane] + part[buf][1][lane]) +
+                             (part[buf][2][lane] + part[buf][3][lane]));
+      io[r] = a;
+      nrm += a * a;
+    }
+    buf ^= 1;
+  }
+  const double sn = wave_sum(nrm);
+  if (threadIdx.x == 0) red[0] = sn;
+  __syncthreads();
+  if (threadIdx.x == 0) b.partials[blockIdx.x] = red[0];
 }
 
 // --------------------------------------------- generic kernels (j+1 > 64)
 
-// y -= sum_{i<nq} h_i q_i  (in place on `io`, or io = src - ... if src != io)
-__global__ __launch_bounds__(kBlock) void k_update(ArnoldiBufs b, const double* src,
-                                                   double* dst, const double* h, int nq) {
+// dst = src - sum_{i<nq} h_i q_i
+__global__ __launch_bounds__(kBlock) void k_update(ArnoldiBufs b, const double* src, double* dst,
+                                                   const double* h, int nq) {
   if (b.iscal[kBreakdown]) return;
-  const double* __restrict__ cs = b.colscale;
-  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
-  for (long long r = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; r < b.n; r += stride) {
-    double a = src[r];
-    for (int i = 0; i < nq; ++i) a -= h[i] * (cs[i] * b.V[static_cast<long long>(i) * b.ld + r]);
-    dst[r] = a;
-  }
-}
-
-// partials of q_i . x for i in [i0, i0+nq) (quantity index i - i0 + qoff) and,
-// if with_norm, of ||x||^2 at quantity index qoff + nq.
-template <int MAXJ>
-__global__ __launch_bounds__(kBlock) void k_dots(ArnoldiBufs b, const double* xv, int i0,
-                                                 int nq, int with_norm, int qoff) {
-  if (b.iscal[kBreakdown]) return;
-  __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
-  const int lb = xcd_block_id();
-  const double* __restrict__ cs = b.colscale;
-  double acc[MAXJ + 1];
-#pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) acc[q] = 0.0;
-  const long long per = (static_cast<long long>(b.n) + b.grid - 1) / b.grid;
-  const long long r0 = per * lb;
-  const long long r1 = min(static_cast<long long>(b.n), r0 + per);
-  for (long long r = r0 + threadIdx.x; r < r1; r += kBlock) {
-    const double a = xv[r];
-#pragma unroll
-    for (int i = 0; i < MAXJ; ++i)
-      if (i < nq) acc[i] += (cs[i0 + i] * b.V[static_cast<long long>(i0 + i) * b.ld + r]) * a;
-    acc[MAXJ] += a * a;
-  }
-  const int ntot = nq + (with_norm ? 1 : 0);
-  double o[MAXJ + 1];
-#pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) o[q] = (q < nq) ? acc[q] : ((q == nq) ? acc[MAXJ] : 0.0);
-  block_partials<MAXJ + 1>(o, ntot, red, b.partials + static_cast<long long>(qoff) * b.grid,
-                           b.grid, lb);
+  __shared__ double cs[kMaxStaged];
+  __shared__ double hs[kMaxStaged];
+  stage(cs, b.colscale, nq);
+  stage(hs, h, nq);
+  __syncthreads();
+  for (long long r = gtid(); r < b.n; r += gstride()) dst[r] = src[r] - sweep_combination(b.V, b.ld, r, nq, hs, cs);
 }
 
 // --------------------------------------------------------------- finalise
 
-// Sum partials[q*grid + 0..grid) for q < nq into dst[q], in block order per
-// lane and a fixed-order wave sum.  Runs as one workgroup.
-__device__ void reduce_partials(const double* partials, int grid, int nq, double* dst) {
-  const int lane = threadIdx.x % kWave;
-  const int wave = threadIdx.x / kWave;
-  for (int q = wave; q < nq; q += kWavesPerBlock) {
-    double s = 0.0;
-    for (int g = lane; g < grid; g += kWave) s += partials[static_cast<long long>(q) * grid + g];
-    s = wave_sum(s);
-    if (lane == 0) dst[q] = s;
+// Finaliser of K1 (and the chunked dots): one workgroup per quantity; block q
+// (q < nq) writes h1[q], block nq writes ||w||^2.  pgrid: blocks of the
+// kernels that wrote the partials.
+__global__ __launch_bounds__(kBlock) void k_fin_h1(ArnoldiBufs b, int nq, int pgrid) {
+  if (b.iscal[kBreakdown]) return;
+  __shared__ double red[kWavesPerBlock];
+  const int q = blockIdx.x;
+  const double s = block_sum_quantity(b.partials, pgrid, q, red);
+  if (threadIdx.x == 0) {
+    if (q < nq) b.h1[q] = s;
+    else b.scal[kWNorm2] = s;
   }
 }
 
-// Finaliser of K1 (+ generic dots): h1[0..nq), scal[kWNorm2].
-__global__ __launch_bounds__(kBlock) void k_fin_h1(ArnoldiBufs b, int nq) {
+// Finaliser of K2: h2[q] (q < nq) and ||w'||^2.
+__global__ __launch_bounds__(kBlock) void k_fin_h2(ArnoldiBufs b, int nq, int pgrid) {
   if (b.iscal[kBreakdown]) return;
-  __shared__ double tmp[kMaxQuantities];
-  reduce_partials(b.partials, b.grid, nq + 1, tmp);
-  __syncthreads();
-  for (int i = threadIdx.x; i < nq; i += kBlock) b.h1[i] = tmp[i];
-  if (threadIdx.x == 0) b.scal[kWNorm2] = tmp[nq];
+  __shared__ double red[kWavesPerBlock];
+  const int q = blockIdx.x;
+  const double s = block_sum_quantity(b.partials, pgrid, q, red);
+  if (threadIdx.x == 0) {
+    if (q < nq) b.h2[q] = s;
+    else b.scal[kWpNorm2] = s;
+  }
 }
 
 // Writes column j of H (zeros below the subdiagonal), the breakdown flag, the
 // next column scale and k_used.  reference: normalize_new_vector
 // (main.cu:1878-1942), breakdown when the norm is < 1e-14 or not finite.
-__device__ void finish_column(ArnoldiBufs& b, int j, double norm2, int reorth) {
-  if (threadIdx.x != 0) return;
+__device__ void finish_column(const ArnoldiBufs& b, int j, double norm2, int reorth) {
   const int ldh = b.ldh;
   for (int i = 0; i <= j; ++i) b.H[static_cast<long long>(j) * ldh + i] = b.h1[i] + (reorth ? b.h2[i] : 0.0);
   for (int i = j + 2; i < ldh; ++i) b.H[static_cast<long long>(j) * ldh + i] = 0.0;
@@ -294,37 +453,18 @@ __device__ void finish_column(ArnoldiBufs& b, int j, double norm2, int reorth) {
   if (reorth) b.iscal[kReorthCount] += 1;
 }
 
-// Finaliser of K2: h2, ||w'||^2 and the second-pass decision.
-// gs_mode 0: never re-orthogonalise, 1: always (reference), 2: DGKS with eta.
-__global__ __launch_bounds__(kBlock) void k_fin_h2(ArnoldiBufs b, int j, int gs_mode, double eta) {
+// Finaliser of the column (one workgroup): the second-pass norm if K3 ran.
+__global__ __launch_bounds__(kBlock) void k_fin_col(ArnoldiBufs b, int j, int gs_mode, double eta, int pgrid) {
   if (b.iscal[kBreakdown]) return;
-  __shared__ double tmp[kMaxQuantities];
-  const int nq = j + 1;
-  reduce_partials(b.partials, b.grid, nq + 1, tmp);
-  __syncthreads();
-  for (int i = threadIdx.x; i < nq; i += kBlock) b.h2[i] = tmp[i];
-  __syncthreads();
+  __shared__ double red[kWavesPerBlock];
+  const int reorth = reorth_needed(b, gs_mode, eta);
+  double norm2 = b.scal[kWpNorm2];
+  if (reorth) norm2 = block_sum_quantity(b.partials, pgrid, 0, red);
   if (threadIdx.x == 0) {
-    const double wp2 = tmp[nq];
-    b.scal[kWpNorm2] = wp2;
-    int reorth = 1;
-    if (gs_mode == 0) reorth = 0;
-    else if (gs_mode == 2) reorth = (sqrt(wp2) < eta * sqrt(b.scal[kWNorm2])) ? 1 : 0;
-    if (!(wp2 == wp2)) reorth = 0;  // NaN: the column breaks down either way
+    if (reorth) b.scal[kWppNorm2] = norm2;
     b.iscal[kReorth] = reorth;
+    finish_column(b, j, norm2, reorth);
   }
-  __syncthreads();
-  if (!b.iscal[kReorth]) finish_column(b, j, b.scal[kWpNorm2], 0);
-}
-
-// Finaliser of K3.
-__global__ __launch_bounds__(kBlock) void k_fin_col(ArnoldiBufs b, int j) {
-  if (b.iscal[kBreakdown] || !b.iscal[kReorth]) return;
-  __shared__ double tmp[kWavesPerBlock];
-  reduce_partials(b.partials, b.grid, 1, tmp);
-  __syncthreads();
-  if (threadIdx.x == 0) b.scal[kWppNorm2] = tmp[0];
-  finish_column(b, j, tmp[0], 1);
 }
 
 // ---------------------------------------------------- start vector / lift
@@ -332,25 +472,19 @@ __global__ __launch_bounds__(kBlock) void k_fin_col(ArnoldiBufs b, int j) {
 // partials of ||x||^2 over x[0..n)
 __global__ __launch_bounds__(kBlock) void k_norm2(ArnoldiBufs b, const double* xv) {
   __shared__ double red[kWavesPerBlock];
-  const int lb = xcd_block_id();
-  const long long per = (static_cast<long long>(b.n) + b.grid - 1) / b.grid;
-  const long long r0 = per * lb;
-  const long long r1 = min(static_cast<long long>(b.n), r0 + per);
   double acc = 0.0;
-  for (long long r = r0 + threadIdx.x; r < r1; r += kBlock) acc += xv[r] * xv[r];
+  for (long long r = gtid(); r < b.n; r += gstride()) acc += xv[r] * xv[r];
   double o[1] = {acc};
-  block_partials<1>(o, 1, red, b.partials, b.grid, lb);
+  block_partials<1>(o, 1, red, b.partials, gridDim.x, blockIdx.x);
 }
 
-// Start of a solve or of a new sub-step: beta = ||V[:,0]|| and
-// colscale[0] = 1/beta, guarded as on the reference's CPU path
-// (main_cpu.cpp:473-484, 589-600: scale 1 when beta <= 1e-14).
-__global__ __launch_bounds__(kBlock) void k_fin_start(ArnoldiBufs b) {
-  __shared__ double tmp[kWavesPerBlock];
-  reduce_partials(b.partials, b.grid, 1, tmp);
-  __syncthreads();
+// Start of a solve: beta = ||V[:,0]|| and colscale[0] = 1/beta, guarded as on
+// the reference's CPU path (main_cpu.cpp:473-484: scale 1 when beta <= 1e-14).
+__global__ __launch_bounds__(kBlock) void k_fin_start(ArnoldiBufs b, int pgrid) {
+  __shared__ double red[kWavesPerBlock];
+  const double s = block_sum_quantity(b.partials, pgrid, 0, red);
   if (threadIdx.x == 0) {
-    const double beta = sqrt(tmp[0]);
+    const double beta = sqrt(s);
     b.scal[kVNorm] = beta;
     b.colscale[0] = (beta > 1e-14) ? (1.0 / beta) : 1.0;
     b.iscal[kKUsed] = 0;
@@ -363,41 +497,57 @@ __global__ __launch_bounds__(kBlock) void k_fin_start(ArnoldiBufs b) {
 // Partials of ||y||^2.
 __global__ __launch_bounds__(kBlock) void k_lift(ArnoldiBufs b, int ncoef, double skip_below) {
   __shared__ double red[kWavesPerBlock];
-  const int lb = xcd_block_id();
-  const double beta = b.scal[kVNorm];
-  const double* __restrict__ cs = b.colscale;
-  const double* __restrict__ wh = b.wH;
-  double* __restrict__ y = b.y;
-  const long long per = (static_cast<long long>(b.n) + b.grid - 1) / b.grid;
-  const long long r0 = per * lb;
-  const long long r1 = min(static_cast<long long>(b.n), r0 + per);
-  double acc = 0.0;
-  for (long long r = r0 + threadIdx.x; r < r1; r += kBlock) {
-    double a = 0.0;
+  __shared__ double cs[kMaxStaged];
+  __shared__ double coef[kMaxStaged];
+  __shared__ int used[kMaxStaged];
+  __shared__ int nused;
+  // compact the kept terms (|wH_i| > skip_below) in order: coef_i = beta*wH_i
+  if (threadIdx.x == 0) {
+    const double beta = b.scal[kVNorm];
+    int u = 0;
     for (int i = 0; i < ncoef; ++i) {
-      const double c = wh[i];
-      if (fabs(c) > skip_below) a += (beta * c) * (cs[i] * b.V[static_cast<long long>(i) * b.ld + r]);
+      const double c = b.wH[i];
+      if (fabs(c) > skip_below) {
+        used[u] = i;
+        coef[u] = beta * c;
+        cs[u] = b.colscale[i];
+        ++u;
+      }
+    }
+    nused = u;
+  }
+  __syncthreads();
+  const int nu = nused;
+  double* __restrict__ y = b.y;
+  double acc = 0.0;
+  for (long long r = gtid(); r < b.n; r += gstride()) {
+    double a = 0.0;
+    for (int u0 = 0; u0 < nu; u0 += kChunk) {
+      double v[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k)
+        v[k] = (u0 + k < nu) ? b.V[static_cast<long long>(used[u0 + k]) * b.ld + r] : 0.0;
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k)
+        if (u0 + k < nu) a += coef[u0 + k] * (cs[u0 + k] * v[k]);
     }
     y[r] = a;
     acc += a * a;
   }
   double o[1] = {acc};
-  block_partials<1>(o, 1, red, b.partials, b.grid, lb);
+  block_partials<1>(o, 1, red, b.partials, gridDim.x, blockIdx.x);
 }
 
 // scal[slot] = sqrt(sum of the quantity-0 partials)
-__global__ __launch_bounds__(kBlock) void k_fin_norm(ArnoldiBufs b, int slot) {
-  __shared__ double tmp[kWavesPerBlock];
-  reduce_partials(b.partials, b.grid, 1, tmp);
-  __syncthreads();
-  if (threadIdx.x == 0) b.scal[slot] = sqrt(tmp[0]);
+__global__ __launch_bounds__(kBlock) void k_fin_norm(ArnoldiBufs b, int slot, int pgrid) {
+  __shared__ double red[kWavesPerBlock];
+  const double s = block_sum_quantity(b.partials, pgrid, 0, red);
+  if (threadIdx.x == 0) b.scal[slot] = sqrt(s);
 }
 
 // Next sub-step starts from y: V[:,0] = y, beta = ||y||, colscale[0] guarded.
 __global__ __launch_bounds__(kBlock) void k_restart(ArnoldiBufs b) {
-  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
-  for (long long r = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; r < b.n; r += stride)
-    b.V[r] = b.y[r];
+  for (long long r = gtid(); r < b.n; r += gstride()) b.V[r] = b.y[r];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const double beta = b.scal[kYNorm];
     b.scal[kVNorm] = beta;
@@ -407,20 +557,21 @@ __global__ __launch_bounds__(kBlock) void k_restart(ArnoldiBufs b) {
   }
 }
 
-// Host-side launchers (C++ linkage; the C ABI wraps them).
+// ------------------------------------------------------------ launchers
+
 template <int MAXJ>
 static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const double* xs, double* w,
                               int ndots, int normq, int chk, hipStream_t s) {
   if (b.vals_f32)
-    hipLaunchKernelGGL((k_spmv_dot<float, MAXJ>), dim3(b.grid), dim3(kBlock), 0, s, b, x, xs, w,
+    hipLaunchKernelGGL((k_spmv_dot<float, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
                        ndots, normq, chk);
   else
-    hipLaunchKernelGGL((k_spmv_dot<double, MAXJ>), dim3(b.grid), dim3(kBlock), 0, s, b, x, xs, w,
+    hipLaunchKernelGGL((k_spmv_dot<double, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
                        ndots, normq, chk);
 }
 
 // Arnoldi column j: w = A q_j with dots against q_0..q_{ndots-1}; the norm
-// partial goes to quantity index normq.
+// partial goes to quantity index normq.  Partials use b.spmv_grid blocks.
 void launch_spmv_dot(const ArnoldiBufs& b, int j, int ndots, int normq, hipStream_t s) {
   const double* x = b.V + static_cast<long long>(j) * b.ld;
   const double* xs = b.colscale + j;
@@ -440,16 +591,30 @@ void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, 
 
 void launch_update_dot(const ArnoldiBufs& b, int j, hipStream_t s) {
   const int nq = j + 1;
-  if (nq <= 8) hipLaunchKernelGGL(k_update_dot<8>, dim3(b.grid), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 16) hipLaunchKernelGGL(k_update_dot<16>, dim3(b.grid), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 32) hipLaunchKernelGGL(k_update_dot<32>, dim3(b.grid), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 48) hipLaunchKernelGGL(k_update_dot<48>, dim3(b.grid), dim3(kBlock), 0, s, b, j);
-  else hipLaunchKernelGGL(k_update_dot<64>, dim3(b.grid), dim3(kBlock), 0, s, b, j);
+#define MEK_UD(K) hipLaunchKernelGGL(k_update_dot_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, j)
+  if (nq <= 8) MEK_UD(2);
+  else if (nq <= 16) MEK_UD(4);
+  else if (nq <= 24) MEK_UD(6);
+  else if (nq <= 32) MEK_UD(8);
+  else if (nq <= 40) MEK_UD(10);
+  else if (nq <= 48) MEK_UD(12);
+  else if (nq <= 56) MEK_UD(14);
+  else MEK_UD(16);
+#undef MEK_UD
 }
 
 void launch_dots(const ArnoldiBufs& b, const double* x, int i0, int nq, int with_norm, int qoff,
                  hipStream_t s) {
-  hipLaunchKernelGGL(k_dots<64>, dim3(b.grid), dim3(kBlock), 0, s, b, x, i0, nq, with_norm, qoff);
+#define MEK_DT(K) hipLaunchKernelGGL(k_dots_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, x, i0, nq, with_norm, qoff)
+  if (nq <= 8) MEK_DT(2);
+  else if (nq <= 16) MEK_DT(4);
+  else if (nq <= 24) MEK_DT(6);
+  else if (nq <= 32) MEK_DT(8);
+  else if (nq <= 40) MEK_DT(10);
+  else if (nq <= 48) MEK_DT(12);
+  else if (nq <= 56) MEK_DT(14);
+  else MEK_DT(16);
+#undef MEK_DT
 }
 
 void launch_update(const ArnoldiBufs& b, const double* src, double* dst, const double* h, int nq,
@@ -457,30 +622,41 @@ void launch_update(const ArnoldiBufs& b, const double* src, double* dst, const d
   hipLaunchKernelGGL(k_update, dim3(b.grid), dim3(kBlock), 0, s, b, src, dst, h, nq);
 }
 
-void launch_update_norm(const ArnoldiBufs& b, int j, hipStream_t s) {
-  hipLaunchKernelGGL(k_update_norm, dim3(b.grid), dim3(kBlock), 0, s, b, j);
+void launch_update_norm(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s) {
+  const int nq = j + 1;
+#define MEK_UN(K) hipLaunchKernelGGL(k_update_norm_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, j, gs_mode, eta)
+  if (nq <= 8) MEK_UN(2);
+  else if (nq <= 16) MEK_UN(4);
+  else if (nq <= 24) MEK_UN(6);
+  else if (nq <= 32) MEK_UN(8);
+  else if (nq <= 40) MEK_UN(10);
+  else if (nq <= 48) MEK_UN(12);
+  else if (nq <= 56) MEK_UN(14);
+  else if (nq <= 64) MEK_UN(16);
+  else hipLaunchKernelGGL(k_update_norm, dim3(b.grid), dim3(kBlock), 0, s, b, j, gs_mode, eta);
+#undef MEK_UN
 }
 
-void launch_fin_h1(const ArnoldiBufs& b, int nq, hipStream_t s) {
-  hipLaunchKernelGGL(k_fin_h1, dim3(1), dim3(kBlock), 0, s, b, nq);
+void launch_fin_h1(const ArnoldiBufs& b, int nq, int pgrid, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_h1, dim3(nq + 1), dim3(kBlock), 0, s, b, nq, pgrid);
 }
-void launch_fin_h2(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s) {
-  hipLaunchKernelGGL(k_fin_h2, dim3(1), dim3(kBlock), 0, s, b, j, gs_mode, eta);
+void launch_fin_h2(const ArnoldiBufs& b, int j, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_h2, dim3(j + 2), dim3(kBlock), 0, s, b, j + 1, b.grid);
 }
-void launch_fin_col(const ArnoldiBufs& b, int j, hipStream_t s) {
-  hipLaunchKernelGGL(k_fin_col, dim3(1), dim3(kBlock), 0, s, b, j);
+void launch_fin_col(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_col, dim3(1), dim3(kBlock), 0, s, b, j, gs_mode, eta, b.grid);
 }
 void launch_norm2(const ArnoldiBufs& b, const double* x, hipStream_t s) {
   hipLaunchKernelGGL(k_norm2, dim3(b.grid), dim3(kBlock), 0, s, b, x);
 }
 void launch_fin_start(const ArnoldiBufs& b, hipStream_t s) {
-  hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(kBlock), 0, s, b);
+  hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(kBlock), 0, s, b, b.grid);
 }
 void launch_lift(const ArnoldiBufs& b, int ncoef, double skip_below, hipStream_t s) {
   hipLaunchKernelGGL(k_lift, dim3(b.grid), dim3(kBlock), 0, s, b, ncoef, skip_below);
 }
-void launch_fin_norm(const ArnoldiBufs& b, int slot, hipStream_t s) {
-  hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(kBlock), 0, s, b, slot);
+void launch_fin_norm(const ArnoldiBufs& b, int slot, int pgrid, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(kBlock), 0, s, b, slot, pgrid);
 }
 void launch_restart(const ArnoldiBufs& b, hipStream_t s) {
   hipLaunchKernelGGL(k_restart, dim3(b.grid), dim3(kBlock), 0, s, b);

@@ -100,8 +100,9 @@ struct ArnoldiBufs {
   double* wH;            // m+2: exp(dt H) e1 (and Expokit extras)
   double* scal;          // scalar slots, see ScalarSlot
   int* iscal;            // int slots, see IntSlot
-  double* partials;      // npart x grid partial sums (row index = quantity)
+  double* partials;      // per-block partial sums: quantity q of block g at q*grid + g
   int grid;              // blocks of the streaming kernels
+  int spmv_grid;         // blocks of the SpMV kernel
 };
 
 enum ScalarSlot : int {

@@ -10,22 +10,24 @@ namespace mek {
 constexpr int kMaxFusedDots = 64;  // dots carried in registers by one sweep
 constexpr int kSpmvNnzCap = 2048;  // must equal kNnzCap in arnoldi.hip
 constexpr int kSpmvRowCap = 2048;
-constexpr int kMaxKrylov = 500;    // largest supported m (finaliser LDS)
+constexpr int kMaxKrylov = 500;    // largest supported m
+constexpr int kMaxGrid = 4096;     // blocks of the streaming and SpMV kernels
 
 void launch_spmv_dot(const ArnoldiBufs& b, int j, int ndots, int normq, hipStream_t s);
 void launch_update_dot(const ArnoldiBufs& b, int j, hipStream_t s);
-void launch_update_norm(const ArnoldiBufs& b, int j, hipStream_t s);
+void launch_update_norm(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s);
 void launch_dots(const ArnoldiBufs& b, const double* x, int i0, int nq, int with_norm, int qoff,
                  hipStream_t s);
 void launch_update(const ArnoldiBufs& b, const double* src, double* dst, const double* h, int nq,
                    hipStream_t s);
-void launch_fin_h1(const ArnoldiBufs& b, int nq, hipStream_t s);
-void launch_fin_h2(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s);
-void launch_fin_col(const ArnoldiBufs& b, int j, hipStream_t s);
+// finalisers read partials written by pgrid blocks
+void launch_fin_h1(const ArnoldiBufs& b, int nq, int pgrid, hipStream_t s);
+void launch_fin_h2(const ArnoldiBufs& b, int j, hipStream_t s);
+void launch_fin_col(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s);
 void launch_norm2(const ArnoldiBufs& b, const double* x, hipStream_t s);
 void launch_fin_start(const ArnoldiBufs& b, hipStream_t s);
 void launch_lift(const ArnoldiBufs& b, int ncoef, double skip_below, hipStream_t s);
-void launch_fin_norm(const ArnoldiBufs& b, int slot, hipStream_t s);
+void launch_fin_norm(const ArnoldiBufs& b, int slot, int pgrid, hipStream_t s);
 void launch_restart(const ArnoldiBufs& b, hipStream_t s);
 void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, double* y,
                        hipStream_t s);

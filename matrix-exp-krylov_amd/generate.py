@@ -96,8 +96,10 @@ def tridiag_1d(N: int, lower: float, diag: float, upper: float) -> np.ndarray:
     return T
 
 
-def _finish_random(n, r, c, v):
-    """Sum duplicates, add the Gershgorin-negative diagonal, build sorted CSR."""
+def _finish_random(n, r, c, v, f32_exact=False):
+    """Sum duplicates, add the Gershgorin-negative diagonal, build sorted CSR.
+    With f32_exact every value (diagonal included) is rounded to the nearest
+    FP32 number, so FP32 storage of the matrix is lossless."""
     key = r.astype(np.int64) * n + c
     order = np.argsort(key, kind="stable")
     key = key[order]
@@ -106,8 +108,12 @@ def _finish_random(n, r, c, v):
     vs = np.add.reduceat(v, start) if v.size else v
     rr = (uk // n).astype(np.int64)
     cc = (uk % n).astype(np.int64)
+    if f32_exact:
+        vs = vs.astype(np.float32).astype(np.float64)
     rowabs = np.bincount(rr, weights=np.abs(vs), minlength=n)
     diag = -1.0 - rowabs
+    if f32_exact:  # rounding moves a_ii by < 1e-6 |a_ii|: still Gershgorin-negative
+        diag = diag.astype(np.float32).astype(np.float64)
     # merge the diagonal into each row at its sorted position
     rr = np.concatenate([rr, np.arange(n)])
     cc = np.concatenate([cc, np.arange(n)])
@@ -131,20 +137,62 @@ def random_negative(n: int, density: float = 5e-3, seed: int = 42):
     return _finish_random(n, r[keep], c[keep], v[keep])
 
 
-def random_rows(n: int, nnz_lo: int = 5, nnz_hi: int = 30, seed: int = 0):
+def random_rows(n: int, nnz_lo: int = 5, nnz_hi: int = 30, seed: int = 0, f32_exact: bool = False):
     """Each row has U{nnz_lo..nnz_hi} nonzeros in distinct random off-diagonal
     columns (values U[-1,1]) plus a_ii = -1 - sum_{j != i} |a_ij|."""
     rng = np.random.default_rng(seed)
     k = rng.integers(nnz_lo, nnz_hi + 1, n)
     k = np.minimum(k, max(n - 1, 0))
-    r = np.repeat(np.arange(n), k)
+    r = np.repeat(np.arange(n, dtype=np.int64), k)
     c = rng.integers(0, max(n - 1, 1), r.size)
     c = c + (c >= r)  # skip the diagonal
     v = rng.uniform(-1.0, 1.0, r.size)
-    # distinct columns per row: drop repeated (r, c)
-    key = r.astype(np.int64) * n + c
-    _, first = np.unique(key, return_index=True)
-    return _finish_random(n, r[first], c[first], v[first])
+    # one sort by (row, col); repeated columns of a row keep their first draw
+    key = r * n + c
+    order = np.argsort(key, kind="stable")
+    key, v = key[order], v[order]
+    keep = np.ones(key.size, bool)
+    keep[1:] = key[1:] != key[:-1]
+    key, v = key[keep], v[keep]
+    rows, cols = key // n, key % n
+    if f32_exact:
+        v = v.astype(np.float32).astype(np.float64)
+    cnt = np.bincount(rows, minlength=n)
+    diag = -1.0 - np.bincount(rows, weights=np.abs(v), minlength=n)
+    if f32_exact:  # rounding moves a_ii by < 1e-6 |a_ii|: still Gershgorin-negative
+        diag = diag.astype(np.float32).astype(np.float64)
+    # merged rows hold one extra (diagonal) entry, placed after the columns < i
+    row_ptr = np.zeros(n + 1, np.int64)
+    np.cumsum(cnt + 1, out=row_ptr[1:])
+    off_start = np.zeros(n + 1, np.int64)
+    np.cumsum(cnt, out=off_start[1:])
+    idx_in_row = np.arange(key.size, dtype=np.int64) - off_start[rows]
+    pos = row_ptr[rows] + idx_in_row + (cols > rows)
+    n_lt = np.bincount(rows[cols < rows], minlength=n)
+    dpos = row_ptr[:-1] + n_lt
+    nnz = int(row_ptr[-1])
+    col_idx = np.empty(nnz, np.int32)
+    vals = np.empty(nnz, np.float64)
+    col_idx[pos] = cols
+    vals[pos] = v
+    col_idx[dpos] = np.arange(n)
+    vals[dpos] = diag
+    return row_ptr.astype(np.int32), col_idx, vals
+
+
+def batch_problems(P: int = 256, n_lo: float = 1e3, n_hi: float = 2e5, seed: int = 0,
+                   f32_exact: bool = True):
+    """P independent problems: n log-uniform in [n_lo, n_hi], random rows with
+    5-30 nonzeros, Gershgorin-negative diagonal, t ~ U[0.1, 5], m ~ U{20..60},
+    v ~ N(0,1) normalised.  Returns a list of dicts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for p in range(P):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        rp, ci, va = random_rows(n, 5, 30, seed=seed * 7919 + p, f32_exact=f32_exact)
+        out.append(dict(rp=rp, ci=ci, va=va, t=float(rng.uniform(0.1, 5.0)),
+                        m=int(rng.integers(20, 61)), v=normal_vector(n, seed=seed * 7919 + p)))
+    return out
 
 
 def normal_vector(n: int, seed: int = 0, normalise: bool = True) -> np.ndarray:
