@@ -66,9 +66,15 @@ def cpu_baseline(rp, ci, va, v, columns_per_solve, m, n, seconds_budget):
     import pyoracle
     pyoracle.build()
     # pick a column count that fits the time budget (first call is a probe)
-    probe = pyoracle.arnoldi_columns(rp, ci, va, v, 2)
-    per_col0 = probe.mean()
-    ncols = int(max(3, min(12, seconds_budget * 1000.0 / max(per_col0 * 3.0, 1e-3))))
+    probe = pyoracle.arnoldi_columns(rp, ci, va, v, 3)
+    slope = max(float(probe[2] - probe[1]), 0.0)
+    ncols, total = 0, 0.0
+    while ncols < m:  # largest column count whose predicted time fits the budget
+        nxt = float(probe[0]) + slope * ncols
+        if ncols >= 3 and total + nxt > seconds_budget * 1000.0:
+            break
+        total += nxt
+        ncols += 1
     t0 = time.time()
     col_ms = pyoracle.arnoldi_columns(rp, ci, va, v, ncols)
     wall = time.time() - t0
@@ -99,7 +105,7 @@ def main():
     ap.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"],
                     help="second GS pass: dgks = only when ||w'|| < 0.1 ||w|| (default), always = reference MGS2 semantics")
     ap.add_argument("--fuse", action="store_true", help="first-pass dots inside the SpMV kernel")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0, help="CPU baseline budget (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this solver runs on one GPU; multi-GPU runs are not part of this build "

@@ -207,7 +207,8 @@ void ArnoldiRunner::run_columns(int m_cols) {
     mek::launch_fin_h2(b, j, s);
     begin_timed(mek::kKUpdateNorm);
     mek::launch_update_norm(b, j, gs_mode, options.dgks_eta, s);
-    end_timed(mek::kKUpdateNorm, 8.0 * n * (nq + 2));
+    // a DGKS column that needs no second pass exits at once: count no bytes
+    end_timed(mek::kKUpdateNorm, gs_mode == 2 ? 0.0 : 8.0 * n * (nq + 2));
     mek::launch_fin_col(b, j, gs_mode, options.dgks_eta, s);
     MEK_HIP_LAUNCHED();
   }

@@ -1,0 +1,236 @@
+"""Profiling workloads and summaries for the Arnoldi solver kernels.
+
+Two modes:
+
+``python matrix-exp-krylov_amd/profile.py workload [--workload W] [--gs G]``
+    The program to run under ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc
+    WRITE_SIZE``: first a calibration elementwise kernel (``torch.neg`` of a
+    1 GiB FP64 buffer into another, 1 GiB read + 1 GiB written per launch,
+    three launches), then one warm-up and one timed solve of the workload.
+
+``python matrix-exp-krylov_amd/profile.py summarize --stats DIR --fetch DIR
+--write DIR --workload W --out-md FILE --out-json FILE [--command TEXT ...]``
+    Reads the rocprofv3 CSV outputs (kernel-trace statistics, and the two
+    counter runs), scales each counter by what it reads for the calibration
+    kernel of known size, and writes a markdown summary plus a JSON map
+    ``{kernel kind: HBM bytes per launch}`` that bench.py attaches to its
+    roofline object as ``traffic``.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import importlib.util
+import json
+import os
+import sys
+from collections import defaultdict
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CAL_BYTES = 1 << 30  # calibration buffer size (bytes read = bytes written)
+
+# kernel symbol prefix -> kind reported by the runner's event timers
+KINDS = [
+    ("k_spmv_dot", "spmv"),
+    ("k_update_dot", "update_dot"),
+    ("k_update_norm", "update_norm"),
+    ("k_dots", "dots"),
+    ("k_update", "update"),
+    ("k_lift", "lift"),
+    ("k_expm", "expm"),
+    ("k_fin_", "finalise"),
+    ("k_norm2", "norm"),
+    ("k_restart", "restart"),
+]
+
+
+def kind_of(name: str) -> str:
+    base = name.split("(")[0]
+    base = base.replace("void ", "").replace("mek::", "").strip()
+    for prefix, kind in KINDS:
+        if base.startswith(prefix):
+            return kind
+    return "other"
+
+
+def _load(name, fn):
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(HERE, fn))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+WORKLOADS = {
+    "laplacian4096": dict(kind="laplacian", N=4096, t=10.0, m=40, tol=1e-8, f32=False),
+    "convdiff256_f32": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=True),
+    "convdiff256_f64": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=False),
+}
+
+
+def workload(args):
+    import torch
+    mek = _load("mek", "__init__.py")
+    gen = _load("mek_generate", "generate.py")
+    src = torch.ones(CAL_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+    w = WORKLOADS[args.workload]
+    if w["kind"] == "laplacian":
+        rp, ci, va = gen.laplacian_2d(w["N"])
+    else:
+        rp, ci, va = gen.convdiff_3d(w["N"])
+    v = gen.normal_vector(rp.size - 1, seed=0)
+    gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+    A = mek.CSR.from_arrays(rp, ci, va)
+    R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control="expokit",
+                   gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=False)
+    A.close()
+    d_v = mek.DeviceBuffer.from_array(v)
+    d_y = mek.DeviceBuffer(8 * v.size)
+    for _ in range(2):
+        R.expmv_device(d_v, d_y)
+    mek.synchronize()
+    print(f"[profile] {args.workload} solves done", file=sys.stderr, flush=True)
+    R.close()
+
+
+def _find(d, pattern):
+    hits = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    if not hits:
+        raise SystemExit(f"no {pattern} under {d}")
+    return hits
+
+
+def read_stats(d):
+    rows = []
+    for fn in _find(d, "*kernel_stats.csv"):
+        with open(fn) as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def read_counter(d, counter):
+    """Per-dispatch counter values: list of (kernel name, value)."""
+    per = defaultdict(float)
+    names = {}
+    for fn in _find(d, "*counter_collection.csv"):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                key = (fn, r["Dispatch_Id"])
+                per[key] += float(r["Counter_Value"])
+                names[key] = r["Kernel_Name"]
+    return [(names[k], v) for k, v in per.items()]
+
+
+def summarize(args):
+    stats = read_stats(args.stats)
+    fetch = read_counter(args.fetch, "FETCH_SIZE")
+    write = read_counter(args.write, "WRITE_SIZE")
+
+    def is_cal(name):
+        return "neg" in name.lower()
+
+    cal_f = [v for n, v in fetch if is_cal(n)]
+    cal_w = [v for n, v in write if is_cal(n)]
+    if not cal_f or not cal_w:
+        raise SystemExit("calibration kernel not found in the counter runs")
+    # counters read KiB; scale so the calibration kernel reads CAL_BYTES
+    f_scale = CAL_BYTES / (sum(cal_f) / len(cal_f) * 1024.0)
+    w_scale = CAL_BYTES / (sum(cal_w) / len(cal_w) * 1024.0)
+
+    def per_kind(vals, scale):
+        acc = defaultdict(lambda: [0.0, 0])
+        for n, v in vals:
+            if is_cal(n):
+                continue
+            k = kind_of(n)
+            acc[k][0] += v * 1024.0 * scale
+            acc[k][1] += 1
+        return acc
+
+    fk = per_kind(fetch, f_scale)
+    wk = per_kind(write, w_scale)
+    traffic = {}
+    for k in sorted(set(fk) | set(wk)):
+        nf = fk[k][1] if k in fk else 0
+        nw = wk[k][1] if k in wk else 0
+        if nf == 0 or nw == 0:
+            continue
+        traffic[k] = fk[k][0] / nf + wk[k][0] / nw
+    out = {"workload": args.workload, "calibration": {
+        "kernel": "torch.neg, 1 GiB FP64 read + 1 GiB written",
+        "fetch_scale": f_scale, "write_scale": w_scale},
+        "bytes_per_launch": traffic}
+    for k, v in traffic.items():
+        out[k] = v
+    with open(args.out_json, "w") as f:
+        json.dump(out, f, indent=1)
+
+    by_kind = defaultdict(lambda: [0, 0.0])
+    lines = []
+    for r in stats:
+        name = r.get("Name") or r.get("KERNEL_NAME") or ""
+        calls = int(r.get("Calls") or r.get("CALLS") or 0)
+        tot = float(r.get("TotalDurationNs") or r.get("TOTAL_DURATION") or 0)
+        k = kind_of(name)
+        by_kind[k][0] += calls
+        by_kind[k][1] += tot
+        lines.append((tot, name, calls, tot / max(calls, 1)))
+    lines.sort(reverse=True)
+    with open(args.out_md, "w") as f:
+        f.write(f"# Kernel profile: {args.workload}\n\n")
+        f.write("Commands (run from the repository root on one MI355X):\n\n```\n")
+        for c in args.command or []:
+            f.write(c + "\n")
+        f.write("```\n\n## rocprofv3 --kernel-trace --stats, by kernel kind\n\n")
+        f.write("| kind | calls | total ms | avg us |\n|---|---|---|---|\n")
+        for k, (c, t) in sorted(by_kind.items(), key=lambda kv: -kv[1][1]):
+            f.write(f"| {k} | {c} | {t / 1e6:.3f} | {t / max(c, 1) / 1e3:.1f} |\n")
+        f.write("\n## Top kernels (rocprofv3 kernel_stats.csv)\n\n")
+        f.write("| kernel | calls | total ms | avg us |\n|---|---|---|---|\n")
+        for tot, name, calls, avg in lines[:25]:
+            short = name.split("(")[0].replace("void ", "")
+            f.write(f"| `{short}` | {calls} | {tot / 1e6:.3f} | {avg / 1e3:.1f} |\n")
+        f.write("\n## HBM traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated)\n\n")
+        f.write(f"Calibration: torch.neg on 1 GiB FP64 (1 GiB read, 1 GiB written); "
+                f"FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} "
+                f"(bytes = KiB x 1024 x scale).\n\n")
+        f.write("| kind | launches | MB per launch (read + write) |\n|---|---|---|\n")
+        for k, v in sorted(traffic.items(), key=lambda kv: -kv[1]):
+            f.write(f"| {k} | {fk[k][1]} | {v / 1e6:.1f} |\n")
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    a = sub.add_parser("workload")
+    a.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS))
+    a.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"])
+    s = sub.add_parser("summarize")
+    s.add_argument("--stats", required=True)
+    s.add_argument("--fetch", required=True)
+    s.add_argument("--write", required=True)
+    s.add_argument("--workload", required=True)
+    s.add_argument("--out-md", required=True)
+    s.add_argument("--out-json", required=True)
+    s.add_argument("--command", action="append")
+    args = ap.parse_args()
+    if args.mode == "workload":
+        workload(args)
+    else:
+        summarize(args)
+
+
+if __name__ == "__main__":
+    main()
