@@ -41,6 +41,7 @@ def log(msg):
 
 WORKLOADS = {
     "laplacian4096": dict(kind="laplacian", N=4096, t=10.0, m=40, tol=1e-8, f32=False),
+    "batch256": dict(kind="batch", P=256, tol=1e-8),
     "convdiff256_f32": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=True),
     "convdiff256_f64": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=False),
 }
@@ -95,6 +96,127 @@ def cpu_baseline(rp, ci, va, v, columns_per_solve, m, n, seconds_budget):
     }
 
 
+def run_batch(args, mek, gen):
+    """256 independent problems (n log-uniform in [1e3, 2e5], random rows with
+    5-30 nonzeros and a Gershgorin-negative diagonal, t ~ U[0.1, 5],
+    m ~ U{20..60}).  Plain form: one ArnoldiRunner solve per problem in turn
+    (FP64 values, Expokit step control).  Batched forms: every problem in one
+    lockstep solve replayed from a hipGraph with Expokit step control on the
+    device, FP64 then FP32 values, then with (m, first sub-step) from the
+    learned selector and FP32 values.  Every form uses the DGKS second pass.
+    Errors are against a tight-tolerance (1e-13, m = 60) FP64 solve of each
+    problem."""
+    sel = _load("mek_selector", os.path.join(ROOT, "matrix-exp-krylov_amd", "selector.py"))
+    w = WORKLOADS["batch256"]
+    tol = w["tol"]
+    log("generating 256 problems")
+    probs = gen.batch_problems(P=w["P"], seed=0, f32_exact=True)
+    P = len(probs)
+    mats = [(q["rp"], q["ci"], q["va"]) for q in probs]
+    ts = [q["t"] for q in probs]
+    ms = [min(q["m"], q["rp"].size - 2) for q in probs]
+    vs = [q["v"] for q in probs]
+    rows = [q["rp"].size - 1 for q in probs]
+    nnz_tot = int(sum(int(q["rp"][-1]) for q in probs))
+    csrs = [mek.CSR.from_arrays(*A) for A in mats]
+    gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+
+    log("reference solves (tol 1e-13)")
+    refs = []
+    for c, t, v, n in zip(csrs, ts, vs, rows):
+        R = mek.Runner(c, m=min(60, n - 1), t=t, tol=1e-13, max_restarts=0, step_control="expokit",
+                       gs_mode=1)
+        refs.append(R.expmv(v))
+        R.close()
+
+    def max_err(ys):
+        return max(float(np.linalg.norm(y - r) / max(np.linalg.norm(r), 1e-300)) for y, r in zip(ys, refs))
+
+    # plain FP64: one runner per problem, solves in turn, v and y resident
+    log("plain FP64, one problem at a time")
+    runners = [mek.Runner(c, m=m, t=t, tol=tol, max_restarts=0, step_control="expokit", gs_mode=gs_mode)
+               for c, m, t in zip(csrs, ms, ts)]
+    dv = [mek.DeviceBuffer.from_array(v) for v in vs]
+    dy = [mek.DeviceBuffer(8 * n) for n in rows]
+    for R, a, b in zip(runners, dv, dy):  # warm-up
+        R.expmv_device(a, b)
+    mek.synchronize()
+    reps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for R, a, b in zip(runners, dv, dy):
+            R.expmv_device(a, b)
+    mek.synchronize()
+    plain_s = (time.perf_counter() - t0) / reps
+    plain_cols = sum(R.stats()["spmv_count"] for R in runners)
+    plain_err = max_err([b.to_array() for b in dy])
+    for R in runners:
+        R.close()
+    del dv, dy
+
+    heur_s = [0] * P  # Expokit's own first step
+
+    def batched(name, m_list, s_list, f32, select_s=0.0):
+        log(f"batched: {name}")
+        B = mek.Batch(csrs, ts, m_list, tol, substeps=s_list, values_f32=f32, gs_mode=gs_mode,
+                      step_control="expokit")
+        B.upload(vs)
+        B.solve()  # warm-up: captures the cycle graph
+        mek.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            B.solve()
+        mek.synchronize()
+        el = (time.perf_counter() - t0) / reps
+        ys = B.download()
+        st = [B.stats(p) for p in range(P)]
+        out = {"problems_per_s": P / (el + select_s), "solve_s": el, "select_s": select_s,
+               "cycles": B.cycles(), "krylov_columns": int(sum(x["spmv_count"] for x in st)),
+               "converged": int(sum(x["converged"] for x in st)), "max_rel_err": max_err(ys)}
+        B.close()
+        return out
+
+    res = {"plain_fp64": {"problems_per_s": P / plain_s, "solve_s": plain_s, "krylov_columns": plain_cols,
+                          "max_rel_err": plain_err}}
+    res["batched_fp64"] = batched("FP64, Expokit step control", ms, heur_s, False)
+    res["batched_mixed"] = batched("FP32 values, Expokit step control", ms, heur_s, True)
+    learned = None
+    if os.path.exists(sel.WEIGHTS):
+        S = sel.Selector.load()
+        FB = mek.Batch(csrs, ts, ms, tol, values_f32=True, gs_mode=gs_mode)
+        t0 = time.perf_counter()
+        F = FB.features()
+        choice = S.predict_features(F)
+        select_s = time.perf_counter() - t0
+        FB.close()
+        lm = [min(m, n - 1) for (m, s), n in zip(choice, rows)]
+        ls = [s for m, s in choice]
+        learned = batched("FP32 values, learned (m, sub-steps)", lm, ls, True, select_s)
+        res["batched_mixed_learned"] = learned
+    head = learned if learned is not None else res["batched_mixed"]
+    form = "batched_mixed_learned" if learned is not None else "batched_mixed"
+    out = {
+        "metric": "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64",
+        "value": round(head["problems_per_s"], 3),
+        "unit": "problems/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": 1,
+        "ms_per_step": round(1000.0 * (head["solve_s"] + head.get("select_s", 0.0)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-values/f64",
+        "data": "synthetic",
+        "config": {"workload": "batch256", "problems": P, "rows": int(sum(rows)), "nnz": nnz_tot,
+                   "tol": tol, "gs": args.gs, "headline_form": form, "seed": 0},
+        "rel_err": head["max_rel_err"],
+        "speedup_vs_plain_fp64": round(head["problems_per_s"] / res["plain_fp64"]["problems_per_s"], 3),
+        "forms": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +236,9 @@ def main():
 
     mek = _load("mek", os.path.join(ROOT, "matrix-exp-krylov_amd", "__init__.py"))
     gen = _load("mek_generate", os.path.join(ROOT, "matrix-exp-krylov_amd", "generate.py"))
+    if args.workload == "batch256":
+        run_batch(args, mek, gen)
+        return
     w = WORKLOADS[args.workload]
     log(f"building {args.workload}")
     if w["kind"] == "laplacian":

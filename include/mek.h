@@ -139,15 +139,18 @@ void mek_runner_destroy(mek_runner* R);
 
 /* Many independent problems y_p = exp(t_p A_p) v_p of uneven size, solved in
  * lockstep on one GPU (one block-diagonal CSR, per-problem reductions, the
- * reference controller of main.cu:1425-1518 on the device, one hipGraph per
- * cycle).  No reference counterpart: the reference solves one problem per
- * process.  m[p] in [1, 62]; substeps[p] > 0 sets the first sub-step to
- * t/substeps with no step cap (learned selector), 0 keeps max_restarts. */
+ * step controller on the device, one hipGraph per cycle).  No reference
+ * counterpart: the reference solves one problem per process.  m[p] in
+ * [1, 62].  step_control 0: the reference controller of main.cu:1425-1518
+ * (substeps[p] > 0 sets the first sub-step to t/substeps with no step cap;
+ * 0 keeps max_restarts).  step_control 1: Expokit step control (first step
+ * from Expokit's formula, or t/substeps[p] when substeps[p] > 0, e.g. from
+ * the learned selector). */
 typedef struct mek_batch mek_batch;
 int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
                      const double* tol, const int* max_restarts, const int* substeps,
-                     int values_f32, int gs_mode, double dgks_eta, int use_graph, int device_id,
-                     mek_batch** out);
+                     int step_control, int values_f32, int gs_mode, double dgks_eta, int use_graph,
+                     int device_id, mek_batch** out);
 /* v[p]: host arrays of each problem's rows, copied to the device. */
 int mek_batch_upload_v(mek_batch* B, const double* const* v);
 /* Solve every problem from the uploaded v; y stays in device memory. */

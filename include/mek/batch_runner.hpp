@@ -1,8 +1,9 @@
 // BatchRunner: many independent y_p = exp(t_p A_p) v_p solves of uneven size
 // on one GPU, advanced in lockstep (additive; the reference solves one
 // problem per process).  Each problem follows the reference controller
-// (main.cu:1425-1518) with its own t, m, tol and step cap; the controller runs
-// on the device and one cycle of the solve is replayed from a hipGraph.
+// (main.cu:1425-1518) or Expokit step control with its own t, m, tol and step
+// cap; the controller runs on the device and one cycle of the solve is
+// replayed from a hipGraph.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,10 +21,18 @@ struct BatchProblem {
   double tol;
   int max_restarts;    // as ArnoldiParams::max_restarts
   int substeps;        // >0: first sub-step t/substeps and no step cap (learned selector)
+  int control = 0;     // 0 reference controller, 1 Expokit step control
+};
+
+// Launch sizes of one cycle, per Arnoldi column j: the problems are stored in
+// order of decreasing m, so column j involves the first np[j] problems, their
+// nsb[j] streaming blocks and nrb[j] SpMV row blocks.
+struct BatchPlan {
+  std::vector<int> np, nsb, nrb, spmv_grid;
 };
 
 struct BatchProblemStats {
-  int converged, accepted, attempts, columns, aborted, reorth_columns;
+  int converged, accepted, attempts, rejected, columns, aborted, reorth_columns;
   double t_done, last_residual;
 };
 
@@ -38,8 +47,9 @@ class BatchRunner {
 
   int num_problems() const { return P_; }
   long long total_rows() const { return n_tot_; }
-  int row_offset(int p) const { return row0_[p]; }
-  int rows(int p) const { return row0_[p + 1] - row0_[p]; }
+  // rows of problem p (the caller's index) in the batch's global row space
+  int row_offset(int p) const { return row0_[ext2int_[p]]; }
+  int rows(int p) const { return row0_[ext2int_[p] + 1] - row0_[ext2int_[p]]; }
 
   // Start vectors (host, one array per problem) into the device copy of v.
   void upload_v(const std::vector<const double*>& v);
@@ -64,7 +74,8 @@ class BatchRunner {
   bool use_graph_ = true;
   int gs_mode_ = 1;
   double eta_ = 0.1;
-  int spmv_grid_ = 0;
+  std::vector<int> ext2int_;   // caller's problem index -> storage order
+  BatchPlan plan_;
   hipStream_t stream_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   hipGraph_t graph_ = nullptr;

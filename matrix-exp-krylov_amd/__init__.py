@@ -94,8 +94,8 @@ SIGNATURES = {
     "mek_runner_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "mek_runner_destroy": (None, [_vp]),
     "mek_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _dp, _ip, _dp, _ip, _ip,
-                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                        ctypes.c_int, ctypes.POINTER(_vp)]),
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "mek_batch_upload_v": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
     "mek_batch_solve": (ctypes.c_int, [_vp]),
     "mek_batch_download_y": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
@@ -346,7 +346,7 @@ class Batch:
     """Many independent exp(t_p A_p) v_p solves in lockstep on one GPU."""
 
     def __init__(self, mats, t, m, tol, max_restarts=None, substeps=None, values_f32=False,
-                 gs_mode=1, dgks_eta=0.1, use_graph=True, device=0):
+                 gs_mode=1, dgks_eta=0.1, use_graph=True, device=0, step_control="reference"):
         P = len(mats)
         self.csrs = [A if isinstance(A, CSR) else CSR.from_arrays(*A) for A in mats]
         self.rows = [c.info()["rows"] for c in self.csrs]
@@ -357,7 +357,8 @@ class Batch:
         mr = _i32(np.broadcast_to(10 if max_restarts is None else max_restarts, P))
         ss = _i32(np.broadcast_to(0 if substeps is None else substeps, P))
         h = _vp()
-        _check(lib().mek_batch_create(P, handles, _pd(t), _pi(m), _pd(tol), _pi(mr), _pi(ss),
+        ctrl = {"reference": 0, "expokit": 1}[step_control]
+        _check(lib().mek_batch_create(P, handles, _pd(t), _pi(m), _pd(tol), _pi(mr), _pi(ss), ctrl,
                                       1 if values_f32 else 0, int(gs_mode), float(dgks_eta),
                                       1 if use_graph else 0, int(device), ctypes.byref(h)))
         self.h = h

@@ -316,16 +316,18 @@ void mek_runner_destroy(mek_runner* R) {
 // ------------------------------------------------------------------ batch
 
 int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
-                     const double* tol, const int* max_restarts, const int* substeps, int values_f32,
-                     int gs_mode, double dgks_eta, int use_graph, int device_id, mek_batch** out) {
+                     const double* tol, const int* max_restarts, const int* substeps, int step_control,
+                     int values_f32, int gs_mode, double dgks_eta, int use_graph, int device_id,
+                     mek_batch** out) {
   return guarded([&] {
     need(nprob >= 1 && A && t && m && tol && out, "batch arguments");
     need(gs_mode >= 0 && gs_mode <= 2, "gs_mode");
+    need(step_control == 0 || step_control == 1, "step_control");
     std::vector<mek::BatchProblem> probs(nprob);
     for (int p = 0; p < nprob; ++p) {
       need(A[p] != nullptr, "matrix");
       probs[p] = {&A[p]->A, t[p], m[p], tol[p], max_restarts ? max_restarts[p] : 10,
-                  substeps ? substeps[p] : 0};
+                  substeps ? substeps[p] : 0, step_control};
     }
     auto h = std::make_unique<mek_batch>();
     h->B = std::make_unique<mek::BatchRunner>(device_id, probs, values_f32 != 0, gs_mode, dgks_eta,
@@ -366,7 +368,7 @@ int mek_batch_stats(const mek_batch* B, int p, mek_solve_stats* o) {
     o->restarts_done = s.accepted;
     o->accepted = s.accepted;
     o->attempts = s.attempts;
-    o->rejected = s.attempts - s.accepted;
+    o->rejected = s.rejected;
     o->spmv_count = s.columns;
     o->reorth_columns = s.reorth_columns;
     o->aborted = s.aborted;

@@ -31,12 +31,22 @@ T* upload(const std::vector<T>& h) {
 
 }  // namespace
 
-BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs, bool values_f32,
+BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_in, bool values_f32,
                          int gs_mode, double dgks_eta, bool use_graph)
-    : P_(static_cast<int>(probs.size())), device_(device_id), use_graph_(use_graph),
+    : P_(static_cast<int>(probs_in.size())), device_(device_id), use_graph_(use_graph),
       gs_mode_(gs_mode), eta_(dgks_eta) {
   if (P_ < 1) throw Error("ERROR: a batch needs at least one problem");
   MEK_HIP(hipSetDevice(device_));
+  // internal order: decreasing m (stable), so Arnoldi column j concerns a
+  // prefix of the problems; callers keep their own indices
+  std::vector<int> order(P_);
+  for (int p = 0; p < P_; ++p) order[p] = p;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return probs_in[a].m > probs_in[c].m; });
+  ext2int_.assign(P_, 0);
+  for (int i = 0; i < P_; ++i) ext2int_[order[i]] = i;
+  std::vector<BatchProblem> probs(P_);
+  for (int i = 0; i < P_; ++i) probs[i] = probs_in[order[i]];
   // ---- host assembly: block-diagonal CSR, row blocks, streaming blocks
   row0_.assign(P_ + 1, 0);
   long long nnz_tot = 0;
@@ -54,7 +64,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs, 
   if (n_tot_ > 0x7fffffffLL || nnz_tot > 0x7fffffffLL) throw Error("ERROR: batch exceeds int32 rows/nnz");
   std::vector<int> row_ptr(static_cast<size_t>(n_tot_) + 1), col_idx(static_cast<size_t>(nnz_tot));
   std::vector<double> vals(static_cast<size_t>(nnz_tot));
-  std::vector<int> rb_start, rb_prob, sb_row0, sb_prob, p_sb0(P_ + 1);
+  std::vector<int> rb_start, rb_prob, sb_row0, sb_prob, p_sb0(P_ + 1), p_rb0(P_ + 1);
   long long k = 0;
   for (int p = 0; p < P_; ++p) {
     const CSRHost& A = *probs[p].A;
@@ -66,6 +76,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs, 
     }
     k += A.nnz;
     const std::vector<int> rb = build_row_blocks(A.row_ptr.data(), A.rows, kBatchNnzCap, kSpmvRowCap);
+    p_rb0[p] = static_cast<int>(rb_prob.size());
     for (size_t i = 0; i + 1 < rb.size(); ++i) {
       rb_start.push_back(r0 + rb[i]);
       rb_prob.push_back(p);
@@ -80,17 +91,37 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs, 
   rb_start.push_back(static_cast<int>(n_tot_));
   sb_row0.push_back(static_cast<int>(n_tot_));
   p_sb0[P_] = static_cast<int>(sb_prob.size());
+  p_rb0[P_] = static_cast<int>(rb_prob.size());
   const int nsb = static_cast<int>(sb_prob.size());
   const int nrb = static_cast<int>(rb_prob.size());
+  // per-column launch plan over the active prefix; SpMV grids are multiples
+  // of 8 (one share per XCD)
+  plan_.np.assign(M_, 0);
+  plan_.nsb.assign(M_, 0);
+  plan_.nrb.assign(M_, 0);
+  plan_.spmv_grid.assign(M_, 8);
+  for (int j = 0; j < M_; ++j) {
+    int np = 0;
+    while (np < P_ && probs[np].m > j) ++np;
+    plan_.np[j] = np;
+    plan_.nsb[j] = p_sb0[np];
+    plan_.nrb[j] = p_rb0[np];
+    const int g = std::min(plan_.nrb[j], 2048);
+    plan_.spmv_grid[j] = std::max(8, (g + 7) / 8 * 8);
+  }
 
   pint_init_.assign(static_cast<size_t>(P_) * kPInt, 0);
+  int any_expokit = 0;
   pscal_init_.assign(static_cast<size_t>(P_) * kPScal, 0.0);
   for (int p = 0; p < P_; ++p) {
     pint_init_[p * kPInt + kBM] = probs[p].m;
     pint_init_[p * kPInt + kBMaxRestarts] = probs[p].max_restarts;
     pint_init_[p * kPInt + kBSubsteps] = probs[p].substeps;
+    pint_init_[p * kPInt + kBCtrl] = probs[p].control;
     pscal_init_[p * kPScal + kBT] = probs[p].t;
     pscal_init_[p * kPScal + kBTol] = probs[p].tol;
+    pscal_init_[p * kPScal + kBAnorm] = probs[p].A->norm_inf();
+    if (probs[p].control == 1) any_expokit = 1;
   }
 
   // ---- device
@@ -120,7 +151,8 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs, 
   MEK_HIP(hipMalloc(&d_pscal_, sizeof(double) * kPScal * P_));
   MEK_HIP(hipMalloc(&d_pint_, sizeof(int) * kPInt * P_));
   MEK_HIP(hipMalloc(&d_partials_, sizeof(double) * (ncols + 1) * nsb));
-  if (M_ > kExpmMaxKLds) MEK_HIP(hipMalloc(&d_expm_ws_, sizeof(double) * 6 * M_ * M_ * P_));
+  const int ord = any_expokit ? M_ + 2 : M_;  // largest exponential order
+  if (ord > kExpmMaxKLds) MEK_HIP(hipMalloc(&d_expm_ws_, sizeof(double) * 6 * ord * ord * P_));
   MEK_HIP(hipMalloc(&d_counters_, sizeof(int) * 4));
   MEK_HIP(hipMemset(d_counters_, 0, sizeof(int) * 4));
   MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), sizeof(int) * 4));
@@ -153,8 +185,8 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs, 
   b->partials = d_partials_;
   b->expm_ws = d_expm_ws_;
   b->counters = d_counters_;
+  b->any_expokit = any_expokit;
   bufs_ = b;
-  spmv_grid_ = static_cast<int>(std::min<long long>(std::max(nrb, 1), 2048));
 }
 
 BatchRunner::~BatchRunner() { destroy(); }
@@ -162,12 +194,13 @@ BatchRunner::~BatchRunner() { destroy(); }
 void BatchRunner::upload_v(const std::vector<const double*>& v) {
   if (static_cast<int>(v.size()) != P_) throw Error("ERROR: upload_v needs one vector per problem");
   for (int p = 0; p < P_; ++p)
-    MEK_HIP(hipMemcpyAsync(d_v0_ + row0_[p], v[p], sizeof(double) * rows(p), hipMemcpyHostToDevice, stream_));
+    MEK_HIP(hipMemcpyAsync(d_v0_ + row_offset(p), v[p], sizeof(double) * rows(p), hipMemcpyHostToDevice,
+                           stream_));
   MEK_HIP(hipStreamSynchronize(stream_));
 }
 
 void BatchRunner::run_cycle_eager() {
-  batch_launch_cycle(*static_cast<BatchBufs*>(bufs_), gs_mode_, eta_, spmv_grid_, stream_);
+  batch_launch_cycle(*static_cast<BatchBufs*>(bufs_), plan_, gs_mode_, eta_, stream_);
   MEK_HIP_LAUNCHED();
 }
 
@@ -184,7 +217,7 @@ void BatchRunner::solve_device() {
   MEK_HIP_LAUNCHED();
   if (use_graph_ && !graph_exec_) {
     MEK_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    batch_launch_cycle(b, gs_mode_, eta_, spmv_grid_, stream_);
+    batch_launch_cycle(b, plan_, gs_mode_, eta_, stream_);
     MEK_HIP(hipStreamEndCapture(stream_, &graph_));
     MEK_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
   }
@@ -205,11 +238,13 @@ void BatchRunner::solve_device() {
 void BatchRunner::download_y(const std::vector<double*>& y) const {
   if (static_cast<int>(y.size()) != P_) throw Error("ERROR: download_y needs one vector per problem");
   for (int p = 0; p < P_; ++p)
-    MEK_HIP(hipMemcpyAsync(y[p], d_y_ + row0_[p], sizeof(double) * rows(p), hipMemcpyDeviceToHost, stream_));
+    MEK_HIP(hipMemcpyAsync(y[p], d_y_ + row_offset(p), sizeof(double) * rows(p), hipMemcpyDeviceToHost,
+                           stream_));
   MEK_HIP(hipStreamSynchronize(stream_));
 }
 
-BatchProblemStats BatchRunner::stats(int p) const {
+BatchProblemStats BatchRunner::stats(int p_ext) const {
+  const int p = ext2int_[p_ext];
   int it[kPInt];
   double s[kPScal];
   MEK_HIP(hipMemcpy(it, d_pint_ + static_cast<size_t>(p) * kPInt, sizeof(it), hipMemcpyDeviceToHost));
@@ -218,6 +253,7 @@ BatchProblemStats BatchRunner::stats(int p) const {
   r.converged = it[kBConverged];
   r.accepted = it[kBAccepted];
   r.attempts = it[kBAttempts];
+  r.rejected = it[kBRejTotal];
   r.columns = it[kBColumns];
   r.aborted = it[kBAborted];
   r.reorth_columns = it[kBReorthCount];
@@ -248,8 +284,14 @@ void BatchRunner::features(float* out) {
                    stream_);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(out, d_f, sizeof(float) * kNumFeatures * P_, hipMemcpyDeviceToHost, stream_);
+  std::vector<float> f_int(static_cast<size_t>(kNumFeatures) * P_);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(f_int.data(), d_f, sizeof(float) * f_int.size(), hipMemcpyDeviceToHost, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (e == hipSuccess)
+    for (int p = 0; p < P_; ++p)
+      std::memcpy(out + static_cast<size_t>(p) * kNumFeatures,
+                  f_int.data() + static_cast<size_t>(ext2int_[p]) * kNumFeatures, sizeof(float) * kNumFeatures);
   for (void* p : {static_cast<void*>(d_p0), static_cast<void*>(d_t), static_cast<void*>(d_tol),
                   static_cast<void*>(d_col), static_cast<void*>(d_stats), static_cast<void*>(d_f)})
     if (p) (void)hipFree(p);

@@ -26,6 +26,7 @@
 // block order.  After a breakdown every later kernel of the attempt returns at
 // once, so the host can enqueue all m columns without reading back.
 #include "common.hpp"
+#include "sweeps.hpp"
 
 namespace mek {
 
@@ -194,126 +195,25 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
 }
 
 // ------------------------------------------- K2 and K1b: column-split sweeps (nq <= 64)
-//
-// Wave w of the workgroup owns basis columns w, w+4, w+8, ... (up to MAXK of
-// them); a tile is 64 consecutive rows, one per lane.  Column indices past nq
-// are clamped to nq-1 with a zero coefficient, so every load of a tile is
-// issued unconditionally and back-to-back, and the coefficients are
-// wave-uniform values computed once per kernel.
 
-// w' = w - sum_i h1_i q_i into V[:, j+1] and partials of q_i . w' and ||w'||^2.
-// The four waves' partial combinations meet in double-buffered LDS (one
-// barrier per tile).  h2 partial of column i = cs_i * sum_r V[r,i] w'[r].
+// w' = w - sum_i h1_i q_i into V[:, j+1]; partials of q_i . w' (quantity i)
+// and ||w'||^2 (quantity j+1).
 template <int MAXK>
 __global__ __launch_bounds__(kBlock) void k_update_dot_cs(ArnoldiBufs b, int j) {
   if (b.iscal[kBreakdown]) return;
-  __shared__ double part[2][kWavesPerBlock][kWave];
-  __shared__ double red[kWavesPerBlock * (4 * MAXK + 1)];
-  const int lane = threadIdx.x % kWave;
-  const int wave = threadIdx.x / kWave;
-  const int nq = j + 1;
-  long long col[MAXK];
-  double c[MAXK];
-#pragma unroll
-  for (int k = 0; k < MAXK; ++k) {
-    const int i = wave + 4 * k;
-    const int ic = i < nq ? i : nq - 1;
-    col[k] = static_cast<long long>(ic) * b.ld;
-    c[k] = (i < nq) ? b.h1[ic] * b.colscale[ic] : 0.0;
-  }
-  const double* __restrict__ V = b.V;
-  const double* __restrict__ w = b.w;
-  double* __restrict__ out = b.V + static_cast<long long>(j + 1) * b.ld;
-  double acc[MAXK];
-#pragma unroll
-  for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
-  double nrm = 0.0;
-  const long long ntiles = (static_cast<long long>(b.n) + kWave - 1) / kWave;
-  int buf = 0;
-  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long r = t * kWave + lane;
-    const long long rc = r < b.n ? r : b.n - 1;
-    double v[MAXK];
-#pragma unroll
-    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
-    const double wr = w[rc];
-    double p = 0.0;
-#pragma unroll
-    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
-    part[buf][wave][lane] = p;
-    __syncthreads();
-    const double a = wr - ((part[buf][0][lane] + part[buf][1][lane]) +
-                           (part[buf][2][lane] + part[buf][3][lane]));
-    if (r < b.n) {
-      if (wave == 0) {
-        out[r] = a;
-        nrm += a * a;
-      }
-#pragma unroll
-      for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
-    }
-    buf ^= 1;
-  }
-  // per-column block partials: column wave + 4k from this wave
-#pragma unroll
-  for (int k = 0; k < MAXK; ++k) {
-    const double s = wave_sum(acc[k]);
-    if (lane == 0) red[wave + 4 * k] = s;
-  }
-  const double sn = wave_sum(nrm);
-  if (threadIdx.x == 0) red[4 * MAXK] = sn;
-  __syncthreads();
-  for (int q = threadIdx.x; q <= nq; q += kBlock) {
-    const double s = (q < nq) ? b.colscale[q] * red[q] : red[4 * MAXK];
-    b.partials[static_cast<long long>(q) * gridDim.x + blockIdx.x] = s;
-  }
+  sweep_update_dot<MAXK>(b.V, b.ld, j + 1, b.h1, b.colscale, b.w,
+                         b.V + static_cast<long long>(j + 1) * b.ld, grid_tiles(b.n), b.partials,
+                         gridDim.x, blockIdx.x);
 }
 
 // partials of q_i . x for i in [i0, i0+nq) (quantity i - i0 + qoff) and, if
-// with_norm, ||x||^2 at quantity qoff + nq.  No LDS exchange is needed.
+// with_norm, ||x||^2 at quantity qoff + nq.
 template <int MAXK>
 __global__ __launch_bounds__(kBlock) void k_dots_cs(ArnoldiBufs b, const double* xv, int i0, int nq,
                                                     int with_norm, int qoff) {
   if (b.iscal[kBreakdown]) return;
-  __shared__ double red[kWavesPerBlock * (4 * MAXK + 1)];
-  const int lane = threadIdx.x % kWave;
-  const int wave = threadIdx.x / kWave;
-  long long col[MAXK];
-#pragma unroll
-  for (int k = 0; k < MAXK; ++k) {
-    const int i = wave + 4 * k;
-    col[k] = static_cast<long long>(i0 + (i < nq ? i : nq - 1)) * b.ld;
-  }
-  const double* __restrict__ V = b.V;
-  double acc[MAXK];
-#pragma unroll
-  for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
-  double nrm = 0.0;
-  const long long ntiles = (static_cast<long long>(b.n) + kWave - 1) / kWave;
-  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long r = t * kWave + lane;
-    if (r >= b.n) continue;
-    double v[MAXK];
-#pragma unroll
-    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + r];
-    const double a = xv[r];
-#pragma unroll
-    for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
-    if (wave == 0) nrm += a * a;
-  }
-#pragma unroll
-  for (int k = 0; k < MAXK; ++k) {
-    const double s = wave_sum(acc[k]);
-    if (lane == 0) red[wave + 4 * k] = s;
-  }
-  const double sn = wave_sum(nrm);
-  if (threadIdx.x == 0) red[4 * MAXK] = sn;
-  __syncthreads();
-  const int ntot = nq + (with_norm ? 1 : 0);
-  for (int q = threadIdx.x; q < ntot; q += kBlock) {
-    const double s = (q < nq) ? b.colscale[i0 + q] * red[q] : red[4 * MAXK];
-    b.partials[static_cast<long long>(qoff + q) * gridDim.x + blockIdx.x] = s;
-  }
+  sweep_dots<MAXK>(b.V, b.ld, i0, nq, b.colscale + i0, xv, grid_tiles(b.n), with_norm != 0,
+                   b.partials + static_cast<long long>(qoff) * gridDim.x, gridDim.x, blockIdx.x);
 }
 
 // ------------------------------------------------------------------- K3
@@ -341,55 +241,14 @@ __global__ __launch_bounds__(kBlock) void k_update_norm(ArnoldiBufs b, int j, in
   block_partials<1>(o, 1, red, b.partials, gridDim.x, blockIdx.x);
 }
 
-// Column-split form of K3 for nq <= 64 (same wave/column ownership and LDS
-// exchange as k_update_dot_cs).
+// Column-split form of K3 for nq <= 64.
 template <int MAXK>
 __global__ __launch_bounds__(kBlock) void k_update_norm_cs(ArnoldiBufs b, int j, int gs_mode,
                                                            double eta) {
   if (b.iscal[kBreakdown] || !reorth_needed(b, gs_mode, eta)) return;
-  __shared__ double part[2][kWavesPerBlock][kWave];
-  __shared__ double red[kWavesPerBlock];
-  const int lane = threadIdx.x % kWave;
-  const int wave = threadIdx.x / kWave;
-  const int nq = j + 1;
-  long long col[MAXK];
-  double c[MAXK];
-#pragma unroll
-  for (int k = 0; k < MAXK; ++k) {
-    const int i = wave + 4 * k;
-    const int ic = i < nq ? i : nq - 1;
-    col[k] = static_cast<long long>(ic) * b.ld;
-    c[k] = (i < nq) ? b.h2[ic] * b.colscale[ic] : 0.0;
-  }
-  const double* __restrict__ V = b.V;
-  double* __restrict__ io = b.V + static_cast<long long>(j + 1) * b.ld;
-  double nrm = 0.0;
-  const long long ntiles = (static_cast<long long>(b.n) + kWave - 1) / kWave;
-  int buf = 0;
-  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long r = t * kWave + lane;
-    const long long rc = r < b.n ? r : b.n - 1;
-    double v[MAXK];
-#pragma unroll
-    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
-    const double wr = (wave == 0) ? io[rc] : 0.0;
-    double p = 0.0;
-#pragma unroll
-    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
-    part[buf][wave][lane] = p;
-    __syncthreads();
-    if (wave == 0 && r < b.n) {
-      const double a = wr - ((part[buf][0][lane] + part[buf][1][lane]) +
-                             (part[buf][2][lane] + part[buf][3][lane]));
-      io[r] = a;
-      nrm += a * a;
-    }
-    buf ^= 1;
-  }
-  const double sn = wave_sum(nrm);
-  if (threadIdx.x == 0) red[0] = sn;
-  __syncthreads();
-  if (threadIdx.x == 0) b.partials[blockIdx.x] = red[0];
+  sweep_update_norm<MAXK>(b.V, b.ld, j + 1, b.h2, b.colscale,
+                          b.V + static_cast<long long>(j + 1) * b.ld, grid_tiles(b.n), b.partials,
+                          blockIdx.x);
 }
 
 // --------------------------------------------- generic kernels (j+1 > 64)
@@ -495,13 +354,11 @@ __global__ __launch_bounds__(kBlock) void k_fin_start(ArnoldiBufs b, int pgrid) 
 // y = sum_{i<ncoef, used} coef_i q_i where coef_i = beta * wH[i]; terms with
 // |wH[i]| <= skip_below are left out (reference: 1e-15, main.cu:1992-1994).
 // Partials of ||y||^2.
-__global__ __launch_bounds__(kBlock) void k_lift(ArnoldiBufs b, int ncoef, double skip_below) {
-  __shared__ double red[kWavesPerBlock];
-  __shared__ double cs[kMaxStaged];
-  __shared__ double coef[kMaxStaged];
-  __shared__ int used[kMaxStaged];
-  __shared__ int nused;
-  // compact the kept terms (|wH_i| > skip_below) in order: coef_i = beta*wH_i
+// Kept terms of the lift in order (|wH_i| > skip_below): column used[u] with
+// coefficient beta * wH_i (coef) and its column scale (cs); returns the count.
+// One thread compacts; the result is in LDS after the barrier.
+__device__ int compact_lift_terms(const ArnoldiBufs& b, int ncoef, double skip_below, int* used,
+                                  double* coef, double* cs, int* nused) {
   if (threadIdx.x == 0) {
     const double beta = b.scal[kVNorm];
     int u = 0;
@@ -514,10 +371,39 @@ __global__ __launch_bounds__(kBlock) void k_lift(ArnoldiBufs b, int ncoef, doubl
         ++u;
       }
     }
-    nused = u;
+    *nused = u;
   }
   __syncthreads();
-  const int nu = nused;
+  return *nused;
+}
+
+// Column-split lift for at most 64 terms: y = sum_u (beta wH_u) (cs_u V[:, u]).
+template <int MAXK>
+__global__ __launch_bounds__(kBlock) void k_lift_cs(ArnoldiBufs b, int ncoef, double skip_below) {
+  __shared__ double cs[kSweepMaxCols];
+  __shared__ double coef[kSweepMaxCols];
+  __shared__ int used[kSweepMaxCols];
+  __shared__ int nused;
+  const int nu = compact_lift_terms(b, ncoef, skip_below, used, coef, cs, &nused);
+  if (nu == 0) {  // no kept term: y = 0
+    for (long long r = gtid(); r < b.n; r += gstride()) b.y[r] = 0.0;
+    if (threadIdx.x == 0) b.partials[blockIdx.x] = 0.0;
+    return;
+  }
+  // one combined coefficient per kept column, in kept order
+  for (int u = threadIdx.x; u < nu; u += kBlock) coef[u] = coef[u] * cs[u];
+  __syncthreads();
+  sweep_lift<MAXK>(b.V, b.ld, nu, used, coef, b.y, grid_tiles(b.n), b.partials, blockIdx.x);
+}
+
+// Generic lift for more than 64 terms (chunked loads, LDS coefficients).
+__global__ __launch_bounds__(kBlock) void k_lift(ArnoldiBufs b, int ncoef, double skip_below) {
+  __shared__ double red[kWavesPerBlock];
+  __shared__ double cs[kMaxStaged];
+  __shared__ double coef[kMaxStaged];
+  __shared__ int used[kMaxStaged];
+  __shared__ int nused;
+  const int nu = compact_lift_terms(b, ncoef, skip_below, used, coef, cs, &nused);
   double* __restrict__ y = b.y;
   double acc = 0.0;
   for (long long r = gtid(); r < b.n; r += gstride()) {
@@ -590,31 +476,17 @@ void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, 
 }
 
 void launch_update_dot(const ArnoldiBufs& b, int j, hipStream_t s) {
-  const int nq = j + 1;
-#define MEK_UD(K) hipLaunchKernelGGL(k_update_dot_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, j)
-  if (nq <= 8) MEK_UD(2);
-  else if (nq <= 16) MEK_UD(4);
-  else if (nq <= 24) MEK_UD(6);
-  else if (nq <= 32) MEK_UD(8);
-  else if (nq <= 40) MEK_UD(10);
-  else if (nq <= 48) MEK_UD(12);
-  else if (nq <= 56) MEK_UD(14);
-  else MEK_UD(16);
-#undef MEK_UD
+#define MEK_L(K) hipLaunchKernelGGL(k_update_dot_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, j)
+  MEK_SWEEP_BUCKETS(j + 1, MEK_L);
+#undef MEK_L
 }
 
 void launch_dots(const ArnoldiBufs& b, const double* x, int i0, int nq, int with_norm, int qoff,
                  hipStream_t s) {
-#define MEK_DT(K) hipLaunchKernelGGL(k_dots_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, x, i0, nq, with_norm, qoff)
-  if (nq <= 8) MEK_DT(2);
-  else if (nq <= 16) MEK_DT(4);
-  else if (nq <= 24) MEK_DT(6);
-  else if (nq <= 32) MEK_DT(8);
-  else if (nq <= 40) MEK_DT(10);
-  else if (nq <= 48) MEK_DT(12);
-  else if (nq <= 56) MEK_DT(14);
-  else MEK_DT(16);
-#undef MEK_DT
+#define MEK_L(K) \
+  hipLaunchKernelGGL(k_dots_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, x, i0, nq, with_norm, qoff)
+  MEK_SWEEP_BUCKETS(nq, MEK_L);
+#undef MEK_L
 }
 
 void launch_update(const ArnoldiBufs& b, const double* src, double* dst, const double* h, int nq,
@@ -623,20 +495,15 @@ void launch_update(const ArnoldiBufs& b, const double* src, double* dst, const d
 }
 
 void launch_update_norm(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s) {
-  const int nq = j + 1;
-#define MEK_UN(K) hipLaunchKernelGGL(k_update_norm_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, j, gs_mode, eta)
-  if (nq <= 8) MEK_UN(2);
-  else if (nq <= 16) MEK_UN(4);
-  else if (nq <= 24) MEK_UN(6);
-  else if (nq <= 32) MEK_UN(8);
-  else if (nq <= 40) MEK_UN(10);
-  else if (nq <= 48) MEK_UN(12);
-  else if (nq <= 56) MEK_UN(14);
-  else if (nq <= 64) MEK_UN(16);
-  else hipLaunchKernelGGL(k_update_norm, dim3(b.grid), dim3(kBlock), 0, s, b, j, gs_mode, eta);
-#undef MEK_UN
+  if (j + 1 > kSweepMaxCols) {
+    hipLaunchKernelGGL(k_update_norm, dim3(b.grid), dim3(kBlock), 0, s, b, j, gs_mode, eta);
+    return;
+  }
+#define MEK_L(K) \
+  hipLaunchKernelGGL(k_update_norm_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, j, gs_mode, eta)
+  MEK_SWEEP_BUCKETS(j + 1, MEK_L);
+#undef MEK_L
 }
-
 void launch_fin_h1(const ArnoldiBufs& b, int nq, int pgrid, hipStream_t s) {
   hipLaunchKernelGGL(k_fin_h1, dim3(nq + 1), dim3(kBlock), 0, s, b, nq, pgrid);
 }
@@ -653,7 +520,13 @@ void launch_fin_start(const ArnoldiBufs& b, hipStream_t s) {
   hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(kBlock), 0, s, b, b.grid);
 }
 void launch_lift(const ArnoldiBufs& b, int ncoef, double skip_below, hipStream_t s) {
-  hipLaunchKernelGGL(k_lift, dim3(b.grid), dim3(kBlock), 0, s, b, ncoef, skip_below);
+  if (ncoef > kSweepMaxCols) {
+    hipLaunchKernelGGL(k_lift, dim3(b.grid), dim3(kBlock), 0, s, b, ncoef, skip_below);
+    return;
+  }
+#define MEK_L(K) hipLaunchKernelGGL(k_lift_cs<K>, dim3(b.grid), dim3(kBlock), 0, s, b, ncoef, skip_below)
+  MEK_SWEEP_BUCKETS(ncoef, MEK_L);
+#undef MEK_L
 }
 void launch_fin_norm(const ArnoldiBufs& b, int slot, int pgrid, hipStream_t s) {
   hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(kBlock), 0, s, b, slot, pgrid);

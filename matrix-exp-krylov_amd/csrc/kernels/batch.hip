@@ -15,6 +15,7 @@
 // the exponential and the lift, like the single-problem runner.
 #include "batch.hpp"
 #include "expm_core.hpp"
+#include "sweeps.hpp"
 
 namespace mek {
 
@@ -63,6 +64,25 @@ __device__ void reduce_problem(const BatchBufs& b, int p, int nq, double* dst) {
   }
 }
 
+// Expokit step rounding: up to two significant digits (Sidje 1998).
+__device__ __forceinline__ double round_step(double x) {
+  const double sc = pow(10.0, floor(log10(x)) - 1.0);
+  return ceil(x / sc) * sc;
+}
+
+// Expokit constants (Sidje 1998, expv): safety factors and limits.
+constexpr double kExpokitGamma = 0.9;
+constexpr double kExpokitDelta = 1.2;
+constexpr double kExpokitBtol = 1.0e-7;
+constexpr int kExpokitMxrej = 10;
+constexpr int kExpokitMaxSteps = 100000;
+
+// Problem p takes part in the Expokit ||A v_{m+1}|| product of this cycle.
+__device__ __forceinline__ bool av_active(const BatchBufs& b, int p) {
+  const int* s = pi(b, p);
+  return s[kBActive] && s[kBNeedBasis] && s[kBCtrl] == 1 && !s[kBBreakdown];
+}
+
 template <typename VT>
 __device__ __forceinline__ double ldv(const void* vals, long long k) {
   return static_cast<double>(static_cast<const VT*>(vals)[k]);
@@ -99,14 +119,33 @@ __global__ __launch_bounds__(kBlock) void kb_init(BatchBufs b) {
   const double t = s[kBT];
   const int R = it[kBMaxRestarts];
   const double dt_max = t, dt_min = fmax(1e-12, t * 1e-6);
-  const double dt_init = (it[kBSubsteps] > 0) ? t / it[kBSubsteps] : t / (R > 0 ? R : 1);
   s[kBDtMax] = dt_max;
   s[kBDtMin] = dt_min;
-  s[kBDt] = fmin(fmin(fmax(dt_init, dt_min), dt_max), t);
+  if (it[kBCtrl] == 1) {
+    // Expokit's first step from ||A||_inf, beta and tol (expv), or t/substeps
+    // when the caller predicted the sub-step count
+    const int m = it[kBM];
+    const double anorm = s[kBAnorm] > 0.0 ? s[kBAnorm] : 1.0;
+    double t_new;
+    if (it[kBSubsteps] > 0) {
+      t_new = t / it[kBSubsteps];
+    } else {
+      const double fact = pow((m + 1) / exp(1.0), m + 1) * sqrt(2.0 * M_PI * (m + 1));
+      const double bet = beta > 0.0 ? beta : 1.0;
+      t_new = round_step((1.0 / anorm) * pow((fact * s[kBTol]) / (4.0 * bet * anorm), 1.0 / m));
+    }
+    s[kBTNew] = t_new;
+    s[kBDt] = fmin(t_new, t);
+    s[kBSErr] = 0.0;
+  } else {
+    const double dt_init = (it[kBSubsteps] > 0) ? t / it[kBSubsteps] : t / (R > 0 ? R : 1);
+    s[kBDt] = fmin(fmin(fmax(dt_init, dt_min), dt_max), t);
+  }
   s[kBTDone] = 0.0;
   s[kBErr] = 0.0;
   s[kBLastRes] = 0.0;
-  it[kBMaxSteps] = (it[kBSubsteps] > 0) ? 1000000 : (R > 0 ? R : 1000);
+  if (it[kBCtrl] == 1) it[kBMaxSteps] = (R > 0) ? max(R, kExpokitMaxSteps) : kExpokitMaxSteps;
+  else it[kBMaxSteps] = (it[kBSubsteps] > 0) ? 1000000 : (R > 0 ? R : 1000);
   it[kBActive] = (t > 0.0) ? 1 : 0;
   it[kBNeedBasis] = 1;
   it[kBRestart] = 0;
@@ -119,6 +158,10 @@ __global__ __launch_bounds__(kBlock) void kb_init(BatchBufs b) {
   it[kBAttempts] = 0;
   it[kBColumns] = 0;
   it[kBReorthCount] = 0;
+  it[kBIReject] = 0;
+  it[kBLift] = 0;
+  it[kBKLift] = 0;
+  it[kBRejTotal] = 0;
 }
 
 // ----------------------------------------------------------- cycle start
@@ -137,6 +180,7 @@ __global__ __launch_bounds__(kBlock) void kb_cycle_begin(BatchBufs b) {
   for (int p = blockIdx.x * kBlock + threadIdx.x; p < b.P; p += gridDim.x * kBlock) {
     double* s = ps(b, p);
     int* it = pi(b, p);
+    it[kBLift] = 0;
     if (!it[kBActive]) continue;
     if (it[kBRestart]) {
       const double beta = s[kBYNorm];
@@ -147,7 +191,9 @@ __global__ __launch_bounds__(kBlock) void kb_cycle_begin(BatchBufs b) {
       it[kBBreakdown] = 0;
       it[kBKUsed] = 0;
       it[kBRejects] = 0;
-      s[kBDt] = fmin(s[kBDt], s[kBT] - s[kBTDone]);
+      it[kBIReject] = 0;
+      if (it[kBCtrl] == 1) s[kBDt] = fmin(s[kBTNew], s[kBT] - s[kBTDone]);
+      else s[kBDt] = fmin(s[kBDt], s[kBT] - s[kBTDone]);
     }
   }
 }
@@ -155,17 +201,29 @@ __global__ __launch_bounds__(kBlock) void kb_cycle_begin(BatchBufs b) {
 // ------------------------------------------------------------- columns
 
 // w = A q_j for every row block of a problem in column j (block-diagonal CSR).
+// With av = 1 (Expokit), w = A q_{m_p} for the problems in av_active.  Row
+// blocks [0, nrb_lim) are split into one contiguous range per XCD, and the
+// workgroups of an XCD stride through its range together, so the rows in
+// flight on an XCD (and the x entries they gather) stay close and in its L2.
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j) {
+__global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, int nrb_lim) {
   __shared__ double prod[kBatchNnzCap];
-  const long long nrb = b.nrb;
-  const int rb0 = static_cast<int>(nrb * blockIdx.x / gridDim.x);
-  const int rb1 = static_cast<int>(nrb * (blockIdx.x + 1) / gridDim.x);
-  const double* __restrict__ x = b.V + static_cast<long long>(j) * b.ld;
-  for (int rb = rb0; rb < rb1; ++rb) {
+  int xcd = 0, li = blockIdx.x, gx = gridDim.x, nx = 1;
+  if (gridDim.x % kNumXcd == 0) {
+    xcd = blockIdx.x % kNumXcd;
+    li = blockIdx.x / kNumXcd;
+    gx = gridDim.x / kNumXcd;
+    nx = kNumXcd;
+  }
+  const long long nrb = nrb_lim;
+  const int rb0 = static_cast<int>(nrb * xcd / nx);
+  const int rb1 = static_cast<int>(nrb * (xcd + 1) / nx);
+  for (int rb = rb0 + li; rb < rb1; rb += gx) {
     const int p = b.rb_prob[rb];
-    if (!col_active(b, p, j)) continue;  // uniform over the block
-    const double xs = pcs(b, p)[j];
+    if (av ? !av_active(b, p) : !col_active(b, p, j)) continue;  // uniform over the block
+    const int jc = av ? pi(b, p)[kBM] : j;
+    const double* __restrict__ x = b.V + static_cast<long long>(jc) * b.ld;
+    const double xs = pcs(b, p)[jc];
     const int r0 = b.rb_start[rb], r1 = b.rb_start[rb + 1];
     const int k0 = b.row_ptr[r0], k1 = b.row_ptr[r1];
     if (k1 - k0 <= kBatchNnzCap) {
@@ -190,86 +248,59 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j) {
   }
 }
 
+// Streaming blocks run the column-split sweeps over their own row range.
+__device__ __forceinline__ TileRange sb_tiles(const BatchBufs& b, int sb) {
+  return block_tiles(b.sb_row0[sb], b.sb_row0[sb + 1]);
+}
+
 // partials of q_i . w (i <= j) and ||w||^2 (quantity j+1) per streaming block
-template <int MAXJ>
+template <int MAXK>
 __global__ __launch_bounds__(kBlock) void kb_dots1(BatchBufs b, int j) {
-  __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
   const int sb = blockIdx.x;
   const int p = b.sb_prob[sb];
   if (!col_active(b, p, j)) return;
-  const int nq = j + 1;
-  const double* __restrict__ cs = pcs(b, p);
-  double acc[MAXJ + 1];
-#pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) acc[q] = 0.0;
-  for (int r = b.sb_row0[sb] + threadIdx.x; r < b.sb_row0[sb + 1]; r += kBlock) {
-    const double a = b.w[r];
-#pragma unroll
-    for (int i = 0; i < MAXJ; ++i)
-      if (i < nq) acc[i] += (cs[i] * b.V[static_cast<long long>(i) * b.ld + r]) * a;
-    acc[MAXJ] += a * a;
-  }
-  double o[MAXJ + 1];
-#pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) o[q] = (q < nq) ? acc[q] : ((q == nq) ? acc[MAXJ] : 0.0);
-  block_partials_b<MAXJ + 1>(o, nq + 1, red, b.partials, b.nsb, sb);
+  sweep_dots<MAXK>(b.V, b.ld, 0, j + 1, pcs(b, p), b.w, sb_tiles(b, sb), true, b.partials, b.nsb, sb);
 }
 
 // w' = w - sum h1_i q_i into V[:, j+1]; partials of q_i . w' and ||w'||^2
-template <int MAXJ>
+template <int MAXK>
 __global__ __launch_bounds__(kBlock) void kb_update_dot(BatchBufs b, int j) {
-  __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
   const int sb = blockIdx.x;
   const int p = b.sb_prob[sb];
   if (!col_active(b, p, j)) return;
-  const int nq = j + 1;
-  const double* __restrict__ cs = pcs(b, p);
-  const double* __restrict__ h = b.h1 + static_cast<long long>(p) * (b.M + 2);
-  double* __restrict__ out = b.V + static_cast<long long>(j + 1) * b.ld;
-  double acc[MAXJ + 1];
-#pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) acc[q] = 0.0;
-  for (int r = b.sb_row0[sb] + threadIdx.x; r < b.sb_row0[sb + 1]; r += kBlock) {
-    double q[MAXJ];
-    double a = b.w[r];
-#pragma unroll
-    for (int i = 0; i < MAXJ; ++i) {
-      if (i < nq) {
-        q[i] = cs[i] * b.V[static_cast<long long>(i) * b.ld + r];
-        a -= h[i] * q[i];
-      }
-    }
-    out[r] = a;
-#pragma unroll
-    for (int i = 0; i < MAXJ; ++i)
-      if (i < nq) acc[i] += q[i] * a;
-    acc[MAXJ] += a * a;
-  }
-  double o[MAXJ + 1];
-#pragma unroll
-  for (int q = 0; q <= MAXJ; ++q) o[q] = (q < nq) ? acc[q] : ((q == nq) ? acc[MAXJ] : 0.0);
-  block_partials_b<MAXJ + 1>(o, nq + 1, red, b.partials, b.nsb, sb);
+  sweep_update_dot<MAXK>(b.V, b.ld, j + 1, b.h1 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
+                         b.w, b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb), b.partials,
+                         b.nsb, sb);
 }
 
 // w'' = w' - sum h2_i q_i in place; partial ||w''||^2 (only if re-orthogonalising)
+template <int MAXK>
 __global__ __launch_bounds__(kBlock) void kb_update_norm(BatchBufs b, int j) {
-  __shared__ double red[kWavesPerBlock];
   const int sb = blockIdx.x;
   const int p = b.sb_prob[sb];
   if (!col_active(b, p, j) || !pi(b, p)[kBReorth]) return;
-  const int nq = j + 1;
-  const double* __restrict__ cs = pcs(b, p);
-  const double* __restrict__ h = b.h2 + static_cast<long long>(p) * (b.M + 2);
-  double* __restrict__ io = b.V + static_cast<long long>(j + 1) * b.ld;
+  sweep_update_norm<MAXK>(b.V, b.ld, j + 1, b.h2 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
+                          b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb), b.partials, sb);
+}
+
+// partials of ||w||^2 per streaming block of the av_active problems
+__global__ __launch_bounds__(kBlock) void kb_av_norm(BatchBufs b) {
+  __shared__ double red[kWavesPerBlock];
+  const int sb = blockIdx.x;
+  if (!av_active(b, b.sb_prob[sb])) return;
   double acc = 0.0;
-  for (int r = b.sb_row0[sb] + threadIdx.x; r < b.sb_row0[sb + 1]; r += kBlock) {
-    double a = io[r];
-    for (int i = 0; i < nq; ++i) a -= h[i] * (cs[i] * b.V[static_cast<long long>(i) * b.ld + r]);
-    io[r] = a;
-    acc += a * a;
-  }
+  for (int r = b.sb_row0[sb] + threadIdx.x; r < b.sb_row0[sb + 1]; r += kBlock) acc += b.w[r] * b.w[r];
   double o[1] = {acc};
   block_partials_b<1>(o, 1, red, b.partials, b.nsb, sb);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_fin_av(BatchBufs b) {
+  __shared__ double tmp[kWavesPerBlock];
+  const int p = blockIdx.x;
+  if (!av_active(b, p)) return;
+  reduce_problem(b, p, 1, tmp);
+  __syncthreads();
+  if (threadIdx.x == 0) ps(b, p)[kBAvNorm] = sqrt(tmp[0]);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_fin_h1(BatchBufs b, int j) {
@@ -359,102 +390,200 @@ __global__ __launch_bounds__(kBlock) void kb_expm(BatchBufs b) {
   if (!it[kBActive]) return;
   const int k = it[kBKUsed];
   const int brk = it[kBBreakdown];
+  const int expokit = it[kBCtrl] == 1;
   const int ldh = b.M + 2;
   const double* H = b.H + static_cast<long long>(p) * ldh * ldh;
   double* wH = b.wH + static_cast<long long>(p) * (b.M + 4);
   double* s = ps(b, p);
-  const double dt = s[kBDt];
+  // Expokit after a breakdown: the step covers the rest of [0, t]
+  const double dt = (expokit && brk) ? s[kBT] - s[kBTDone] : s[kBDt];
   if (k <= 0) {
     for (int i = threadIdx.x; i < b.M + 4; i += kBlock) wH[i] = 0.0;
     if (threadIdx.x == 0) s[kBErr] = 0.0;
     return;
   }
-  double* base = BIG ? b.expm_ws + static_cast<long long>(p) * 6 * b.M * b.M : smem;
+  // Expokit without breakdown: augmented (k+2) matrix with H(k+1, k) = 1
+  const int kk = (expokit && !brk) ? k + 2 : k;
+  const int ord = batch_expm_order(b);
+  double* base = BIG ? b.expm_ws + static_cast<long long>(p) * 6 * ord * ord : smem;
   double* S[6];
-  for (int i = 0; i < 6; ++i) S[i] = base + static_cast<long long>(i) * k * k;
-  for (int e = threadIdx.x; e < k * k; e += kBlock) {
-    const int r = e % k, c = e / k;
-    S[0][e] = dt * H[static_cast<long long>(c) * ldh + r];
+  for (int i = 0; i < 6; ++i) S[i] = base + static_cast<long long>(i) * kk * kk;
+  for (int e = threadIdx.x; e < kk * kk; e += kBlock) {
+    const int r = e % kk, c = e / kk;
+    double v = 0.0;
+    if (c < k && (r < k || (kk > k && r == k))) v = H[static_cast<long long>(c) * ldh + r];
+    if (kk > k && r == k + 1 && c == k) v = 1.0;
+    S[0][e] = dt * v;
   }
   __syncthreads();
-  double* E = expm_slots<BIG>(S, k, s_b, s_int, s_red);
-  for (int i = threadIdx.x; i < b.M + 4; i += kBlock) wH[i] = (i < k) ? E[i] : 0.0;
+  double* E = expm_slots<BIG>(S, kk, s_b, s_int, s_red);
+  for (int i = threadIdx.x; i < b.M + 4; i += kBlock) wH[i] = (i < kk) ? E[i] : 0.0;
+  if (threadIdx.x == 0 && expokit && brk) s[kBDt] = dt;
   if (threadIdx.x == 0) {
     const double hk = H[static_cast<long long>(k - 1) * ldh + k];
     s[kBErr] = brk ? 0.0 : fabs(hk * E[k - 1]) * s[kBVNorm];
   }
 }
 
-// y = sum_{i<=k, |wH_i| > 1e-15} beta wH_i q_i (main.cu:1992-1999); partials ||y||^2
+// y = sum_{i < KLift, |wH_i| > 1e-15} beta wH_i q_i (main.cu:1992-1999) for
+// the problems the controller marked; partials of ||y||^2.
+template <int MAXK>
 __global__ __launch_bounds__(kBlock) void kb_lift(BatchBufs b) {
-  __shared__ double red[kWavesPerBlock];
+  __shared__ double coef[kSweepMaxCols];
+  __shared__ int used[kSweepMaxCols];
+  __shared__ int nused;
   const int sb = blockIdx.x;
   const int p = b.sb_prob[sb];
   const int* it = pi(b, p);
-  if (!it[kBActive]) return;
-  const int k = it[kBKUsed];
-  const double beta = ps(b, p)[kBVNorm];
-  const double* __restrict__ cs = pcs(b, p);
-  const double* __restrict__ wh = b.wH + static_cast<long long>(p) * (b.M + 4);
-  double acc = 0.0;
-  for (int r = b.sb_row0[sb] + threadIdx.x; r < b.sb_row0[sb + 1]; r += kBlock) {
-    double a = 0.0;
-    for (int i = 0; i <= k; ++i) {
+  if (!it[kBLift]) return;
+  if (threadIdx.x == 0) {
+    const int kl = it[kBKLift];
+    const double beta = ps(b, p)[kBVNorm];
+    const double* cs = pcs(b, p);
+    const double* wh = b.wH + static_cast<long long>(p) * (b.M + 4);
+    int u = 0;
+    for (int i = 0; i < kl && i < kSweepMaxCols; ++i) {
       const double c = wh[i];
-      if (fabs(c) > 1e-15) a += (beta * c) * (cs[i] * b.V[static_cast<long long>(i) * b.ld + r]);
+      if (fabs(c) > 1e-15) {
+        used[u] = i;
+        coef[u] = (beta * c) * cs[i];
+        ++u;
+      }
     }
-    b.y[r] = a;
-    acc += a * a;
+    nused = u;
   }
-  double o[1] = {acc};
-  block_partials_b<1>(o, 1, red, b.partials, b.nsb, sb);
+  __syncthreads();
+  const int nu = nused;
+  if (nu == 0) {
+    for (int r = b.sb_row0[sb] + threadIdx.x; r < b.sb_row0[sb + 1]; r += kBlock) b.y[r] = 0.0;
+    if (threadIdx.x == 0) b.partials[sb] = 0.0;
+    return;
+  }
+  sweep_lift<MAXK>(b.V, b.ld, nu, used, coef, b.y, sb_tiles(b, sb), b.partials, sb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_fin_lift(BatchBufs b) {
   __shared__ double tmp[kWavesPerBlock];
   const int p = blockIdx.x;
-  if (!pi(b, p)[kBActive]) return;
+  if (!pi(b, p)[kBLift]) return;
   reduce_problem(b, p, 1, tmp);
   __syncthreads();
   if (threadIdx.x == 0) ps(b, p)[kBYNorm] = sqrt(tmp[0]);
 }
 
-// The reference controller's accept / reject step (main.cu:1476-1516), one
-// thread per problem; writes the number of still-active problems.
+// The reference controller's accept / reject step (main.cu:1476-1516).  The
+// reference lifts y on every attempt, so every attempted problem is lifted.
+__device__ void control_reference(const BatchBufs& b, int p) {
+  double* s = ps(b, p);
+  int* it = pi(b, p);
+  const double err = s[kBErr];
+  const double tol = s[kBTol];
+  it[kBAttempts] += 1;
+  it[kBLift] = 1;
+  it[kBKLift] = it[kBKUsed] + 1;
+  if (err <= tol || it[kBBreakdown]) {
+    s[kBTDone] += s[kBDt];
+    it[kBAccepted] += 1;
+    s[kBLastRes] = err;
+    if (s[kBTDone] >= s[kBT]) {
+      it[kBConverged] = 1;
+      it[kBActive] = 0;
+    } else if (it[kBAccepted] >= it[kBMaxSteps]) {
+      it[kBActive] = 0;  // step cap reached with t_done < t
+    } else {
+      it[kBRestart] = 1;
+      if (err < 0.1 * tol) s[kBDt] = fmin(s[kBDt] * 1.5 * 0.8, s[kBDtMax]);
+    }
+  } else {
+    it[kBRejects] += 1;
+    it[kBRejTotal] += 1;
+    if (it[kBRejects] >= 20) {
+      it[kBAborted] = 1;
+      it[kBActive] = 0;
+      s[kBLastRes] = err;
+      s[kBTDone] = s[kBT];
+    } else {
+      s[kBDt] = fmax(s[kBDt] * 0.5 * 0.8, s[kBDtMin]);
+    }
+  }
+}
+
+// Expokit step control (Sidje 1998, expv): local error from the augmented
+// exponential, err_loc <= delta * t_step * tol to accept, the next step from
+// gamma * t_step * (t_step * tol / err_loc)^(1/m) rounded to two digits; a
+// rejected step keeps its basis and only redoes the exponential.  The lift
+// (m + 1 basis vectors without breakdown) runs only for accepted steps.
+__device__ void control_expokit(const BatchBufs& b, int p) {
+  double* s = ps(b, p);
+  int* it = pi(b, p);
+  const int m = it[kBM];
+  const int brk = it[kBBreakdown];
+  const double tol = s[kBTol];
+  const double beta = s[kBVNorm];
+  const double* F = b.wH + static_cast<long long>(p) * (b.M + 4);
+  const double t_step = s[kBDt];
+  it[kBAttempts] += 1;
+  double err_loc, xm = 1.0 / m;
+  bool accept;
+  if (brk) {
+    err_loc = kExpokitBtol;
+    accept = true;
+  } else {
+    const double phi1 = fabs(beta * F[m]);
+    const double phi2 = fabs(beta * F[m + 1] * s[kBAvNorm]);
+    if (phi1 > 10.0 * phi2) {
+      err_loc = phi2;
+    } else if (phi1 > phi2) {
+      err_loc = (phi1 * phi2) / (phi1 - phi2);
+    } else {
+      err_loc = phi1;
+      xm = 1.0 / (m - 1);
+    }
+    accept = err_loc <= kExpokitDelta * t_step * tol;
+  }
+  if (!accept) {
+    it[kBRejTotal] += 1;
+    if (it[kBIReject] == kExpokitMxrej) {
+      // the requested tolerance is too high: stop with this attempt's y
+      it[kBAborted] = 1;
+      it[kBActive] = 0;
+      it[kBLift] = 1;
+      it[kBKLift] = m + 1;
+      s[kBLastRes] = err_loc;
+      return;
+    }
+    s[kBDt] = round_step(kExpokitGamma * t_step * pow(t_step * tol / err_loc, xm));
+    it[kBIReject] += 1;
+    return;
+  }
+  it[kBLift] = 1;
+  it[kBKLift] = brk ? it[kBKUsed] : m + 1;
+  s[kBTDone] += t_step;
+  it[kBAccepted] += 1;
+  s[kBTNew] = round_step(kExpokitGamma * t_step * pow(t_step * tol / err_loc, xm));
+  const double rndoff = s[kBAnorm] * 2.220446049250313e-16;
+  err_loc = fmax(err_loc, rndoff);
+  s[kBSErr] += err_loc;
+  s[kBLastRes] = err_loc;
+  if (s[kBTDone] >= s[kBT]) {
+    it[kBConverged] = 1;
+    it[kBActive] = 0;
+  } else if (it[kBAccepted] >= it[kBMaxSteps]) {
+    it[kBActive] = 0;
+  } else {
+    it[kBRestart] = 1;
+  }
+}
+
+// One thread per problem; writes the number of still-active problems.
 __global__ __launch_bounds__(1024) void kb_control(BatchBufs b) {
   __shared__ int cnt[1024 / kWave];
   int active = 0;
   for (int p = threadIdx.x; p < b.P; p += blockDim.x) {
-    double* s = ps(b, p);
     int* it = pi(b, p);
     if (!it[kBActive]) continue;
-    const double err = s[kBErr];
-    const double tol = s[kBTol];
-    it[kBAttempts] += 1;
-    if (err <= tol || it[kBBreakdown]) {
-      s[kBTDone] += s[kBDt];
-      it[kBAccepted] += 1;
-      s[kBLastRes] = err;
-      if (s[kBTDone] >= s[kBT]) {
-        it[kBConverged] = 1;
-        it[kBActive] = 0;
-      } else if (it[kBAccepted] >= it[kBMaxSteps]) {
-        it[kBActive] = 0;  // step cap reached with t_done < t
-      } else {
-        it[kBRestart] = 1;
-        if (err < 0.1 * tol) s[kBDt] = fmin(s[kBDt] * 1.5 * 0.8, s[kBDtMax]);
-      }
-    } else {
-      it[kBRejects] += 1;
-      if (it[kBRejects] >= 20) {
-        it[kBAborted] = 1;
-        it[kBActive] = 0;
-        s[kBLastRes] = err;
-        s[kBTDone] = s[kBT];
-      } else {
-        s[kBDt] = fmax(s[kBDt] * 0.5 * 0.8, s[kBDtMin]);
-      }
-    }
+    if (it[kBCtrl] == 1) control_expokit(b, p);
+    else control_reference(b, p);
     active += it[kBActive];
   }
   // count of active problems (fixed-order reduction, no atomics)
@@ -483,48 +612,52 @@ void batch_launch_init(const BatchBufs& b, hipStream_t s) {
   hipLaunchKernelGGL(kb_init, dim3(b.P), dim3(kBlock), 0, s, b);
 }
 
-static void launch_dots1(const BatchBufs& b, int j, hipStream_t s) {
-  const int nq = j + 1;
-  if (nq <= 8) hipLaunchKernelGGL(kb_dots1<8>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 16) hipLaunchKernelGGL(kb_dots1<16>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 32) hipLaunchKernelGGL(kb_dots1<32>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 48) hipLaunchKernelGGL(kb_dots1<48>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else hipLaunchKernelGGL(kb_dots1<64>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-}
-
-static void launch_update_dot_b(const BatchBufs& b, int j, hipStream_t s) {
-  const int nq = j + 1;
-  if (nq <= 8) hipLaunchKernelGGL(kb_update_dot<8>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 16) hipLaunchKernelGGL(kb_update_dot<16>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 32) hipLaunchKernelGGL(kb_update_dot<32>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else if (nq <= 48) hipLaunchKernelGGL(kb_update_dot<48>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-  else hipLaunchKernelGGL(kb_update_dot<64>, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-}
-
-void batch_launch_cycle(const BatchBufs& b, int gs_mode, double eta, int spmv_grid, hipStream_t s) {
+void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int gs_mode, double eta,
+                        hipStream_t s) {
   const int pg = (b.P + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(kb_restart_copy, dim3(b.nsb), dim3(kBlock), 0, s, b);
   hipLaunchKernelGGL(kb_cycle_begin, dim3(pg), dim3(kBlock), 0, s, b);
   for (int j = 0; j < b.M; ++j) {
-    if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(spmv_grid), dim3(kBlock), 0, s, b, j);
-    else hipLaunchKernelGGL(kb_spmv<double>, dim3(spmv_grid), dim3(kBlock), 0, s, b, j);
-    launch_dots1(b, j, s);
-    hipLaunchKernelGGL(kb_fin_h1, dim3(b.P), dim3(kBlock), 0, s, b, j);
-    launch_update_dot_b(b, j, s);
-    hipLaunchKernelGGL(kb_fin_h2, dim3(b.P), dim3(kBlock), 0, s, b, j, gs_mode, eta);
-    hipLaunchKernelGGL(kb_update_norm, dim3(b.nsb), dim3(kBlock), 0, s, b, j);
-    hipLaunchKernelGGL(kb_fin_col, dim3(b.P), dim3(kBlock), 0, s, b, j);
+    // problems are ordered by decreasing m: column j concerns a prefix of them
+    const int np = plan.np[j], nsb = plan.nsb[j], nrb = plan.nrb[j];
+    const int sg = plan.spmv_grid[j];
+    if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(sg), dim3(kBlock), 0, s, b, j, 0, nrb);
+    else hipLaunchKernelGGL(kb_spmv<double>, dim3(sg), dim3(kBlock), 0, s, b, j, 0, nrb);
+#define MEK_L(K) hipLaunchKernelGGL(kb_dots1<K>, dim3(nsb), dim3(kBlock), 0, s, b, j)
+    MEK_SWEEP_BUCKETS(j + 1, MEK_L);
+#undef MEK_L
+    hipLaunchKernelGGL(kb_fin_h1, dim3(np), dim3(kBlock), 0, s, b, j);
+#define MEK_L(K) hipLaunchKernelGGL(kb_update_dot<K>, dim3(nsb), dim3(kBlock), 0, s, b, j)
+    MEK_SWEEP_BUCKETS(j + 1, MEK_L);
+#undef MEK_L
+    hipLaunchKernelGGL(kb_fin_h2, dim3(np), dim3(kBlock), 0, s, b, j, gs_mode, eta);
+    if (gs_mode != 0) {
+#define MEK_L(K) hipLaunchKernelGGL(kb_update_norm<K>, dim3(nsb), dim3(kBlock), 0, s, b, j)
+      MEK_SWEEP_BUCKETS(j + 1, MEK_L);
+#undef MEK_L
+      hipLaunchKernelGGL(kb_fin_col, dim3(np), dim3(kBlock), 0, s, b, j);
+    }
+  }
+  if (b.any_expokit) {
+    // ||A v_{m+1}|| of every Expokit problem that built a full basis
+    const int sg = plan.spmv_grid[0];
+    if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(sg), dim3(kBlock), 0, s, b, 0, 1, b.nrb);
+    else hipLaunchKernelGGL(kb_spmv<double>, dim3(sg), dim3(kBlock), 0, s, b, 0, 1, b.nrb);
+    hipLaunchKernelGGL(kb_av_norm, dim3(b.nsb), dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(kb_fin_av, dim3(b.P), dim3(kBlock), 0, s, b);
   }
   hipLaunchKernelGGL(kb_after_basis, dim3(pg), dim3(kBlock), 0, s, b);
-  const int kk = b.M;
+  const int kk = batch_expm_order(b);
   if (kk <= kMaxKLds) {
     hipLaunchKernelGGL(kb_expm<false>, dim3(b.P), dim3(kBlock), 5 * kk * kk * sizeof(double), s, b);
   } else {
     hipLaunchKernelGGL(kb_expm<true>, dim3(b.P), dim3(kBlock), 0, s, b);
   }
-  hipLaunchKernelGGL(kb_lift, dim3(b.nsb), dim3(kBlock), 0, s, b);
-  hipLaunchKernelGGL(kb_fin_lift, dim3(b.P), dim3(kBlock), 0, s, b);
   hipLaunchKernelGGL(kb_control, dim3(1), dim3(1024), 0, s, b);
+#define MEK_L(K) hipLaunchKernelGGL(kb_lift<K>, dim3(b.nsb), dim3(kBlock), 0, s, b)
+  MEK_SWEEP_BUCKETS(b.M + 2, MEK_L);
+#undef MEK_L
+  hipLaunchKernelGGL(kb_fin_lift, dim3(b.P), dim3(kBlock), 0, s, b);
 }
 
 }  // namespace mek

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "mek/batch_runner.hpp"
 
 namespace mek {
 
@@ -13,13 +14,22 @@ constexpr int kPInt = 24;           // per-problem int slots
 
 enum BatchScalar : int {
   kBVNorm = 0, kBWNorm2, kBWpNorm2, kBErr, kBYNorm, kBDt, kBTDone, kBLastRes,
-  kBT, kBTol, kBDtMin, kBDtMax
+  kBT, kBTol, kBDtMin, kBDtMax,
+  kBAvNorm,   // Expokit: ||A v_{m+1}||
+  kBTNew,     // Expokit: proposed next step
+  kBSErr,     // Expokit: sum of local error estimates
+  kBAnorm     // ||A||_inf (Expokit initial step and round-off floor)
 };
 
 enum BatchInt : int {
   kBActive = 0, kBNeedBasis, kBRestart, kBBreakdown, kBReorth, kBKUsed, kBM,
   kBAccepted, kBRejects, kBConverged, kBAborted, kBAttempts, kBColumns,
-  kBReorthCount, kBMaxRestarts, kBMaxSteps, kBSubsteps
+  kBReorthCount, kBMaxRestarts, kBMaxSteps, kBSubsteps,
+  kBCtrl,      // 0 reference controller, 1 Expokit step control
+  kBIReject,   // Expokit: rejections of the current step
+  kBLift,      // 1: lift y in this cycle
+  kBKLift,     // basis vectors in the lift
+  kBRejTotal   // all rejected attempts
 };
 
 struct BatchBufs {
@@ -50,10 +60,14 @@ struct BatchBufs {
   double* partials;         // (M+2) x nsb
   double* expm_ws;          // P x 6 M^2 when M > the LDS limit
   int* counters;            // [0] active problems, [1] cycles
+  int any_expokit;          // 1 if some problem uses Expokit step control
 };
 
 void batch_prepare();
 void batch_launch_init(const BatchBufs& b, hipStream_t s);
-void batch_launch_cycle(const BatchBufs& b, int gs_mode, double eta, int spmv_grid, hipStream_t s);
+void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int gs_mode, double eta,
+                        hipStream_t s);
+// Largest exponential order of a cycle (M, or M + 2 with Expokit problems).
+__host__ __device__ inline int batch_expm_order(const BatchBufs& b) { return b.any_expokit ? b.M + 2 : b.M; }
 
 }  // namespace mek

@@ -1,0 +1,264 @@
+// Column-split streaming sweeps over a column-major basis, shared by the
+// single-problem kernels (arnoldi.hip) and the batched ones (batch.hip).
+//
+// Wave w of a 256-thread workgroup owns basis columns w, w+4, w+8, ... (at
+// most MAXK of them, so up to 4*MAXK columns); a tile is 64 consecutive rows,
+// one per lane.  Column indices past the count are clamped to the last column
+// with a zero coefficient, so every load of a tile is issued back-to-back with
+// no branch, and the per-column coefficients are wave-uniform registers
+// computed once.  Sweeps that produce a combination (update, lift) meet the
+// four waves' partial sums in double-buffered LDS, one barrier per tile.
+//
+// Per-workgroup results go to partials[q * pstride + pidx] for quantity q, in
+// a fixed order (DPP wave sums, then the owning wave's lane 0), so the
+// reductions stay deterministic.
+#pragma once
+
+#include "common.hpp"
+
+namespace mek {
+
+// Rows base + t*64 + lane for t = t0, t0+ts, ... < ntiles; rows >= rend are
+// outside the range (their loads are clamped to rend-1 and their results
+// dropped).  Requires rend > base.
+struct TileRange {
+  long long base;
+  long long t0;
+  long long ntiles;
+  long long ts;
+  long long rend;
+};
+
+__device__ __forceinline__ TileRange grid_tiles(long long n) {
+  return TileRange{0, static_cast<long long>(blockIdx.x), (n + kWave - 1) / kWave,
+                   static_cast<long long>(gridDim.x), n};
+}
+
+__device__ __forceinline__ TileRange block_tiles(long long row0, long long row1) {
+  return TileRange{row0, 0, (row1 - row0 + kWave - 1) / kWave, 1, row1};
+}
+
+// Wave-owned columns: col[k] = first row of column (i0 + wave + 4k), clamped.
+template <int MAXK>
+__device__ __forceinline__ void own_columns(long long (&col)[MAXK], long long ld, int i0, int nq) {
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const int i = wave + 4 * k;
+    col[k] = static_cast<long long>(i0 + (i < nq ? i : nq - 1)) * ld;
+  }
+}
+
+// Write the per-wave column sums acc[k] (column wave + 4k, scaled by cs) and
+// an optional norm to the partials.  `red` holds 4*MAXK + 1 doubles.
+template <int MAXK>
+__device__ __forceinline__ void write_column_partials(const double (&acc)[MAXK], double nrm, int nq,
+                                                      bool with_norm, const double* cs, double* red,
+                                                      double* partials, long long pstride,
+                                                      long long pidx) {
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[wave + 4 * k] = s;
+  }
+  const double sn = wave_sum(nrm);
+  if (threadIdx.x == 0) red[4 * MAXK] = sn;
+  __syncthreads();
+  const int ntot = nq + (with_norm ? 1 : 0);
+  for (int q = threadIdx.x; q < ntot; q += kBlock) {
+    const double s = (q < nq) ? cs[q] * red[q] : red[4 * MAXK];
+    partials[static_cast<long long>(q) * pstride + pidx] = s;
+  }
+}
+
+// Partials of q_i . x = cs_i * sum_r V[r, i0+i] x[r] for i < nq (quantity i)
+// and, with_norm, of ||x||^2 (quantity nq).  cs points at column i0's scale.
+template <int MAXK>
+__device__ __forceinline__ void sweep_dots(const double* __restrict__ V, long long ld, int i0, int nq,
+                                           const double* cs, const double* __restrict__ x,
+                                           TileRange tr, bool with_norm, double* partials,
+                                           long long pstride, long long pidx) {
+  __shared__ double red[4 * MAXK + 1];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  long long col[MAXK];
+  own_columns<MAXK>(col, ld, i0, nq);
+  double acc[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
+  double nrm = 0.0;
+  for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
+    const long long r = tr.base + t * kWave + lane;
+    if (r >= tr.rend) continue;
+    double v[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + r];
+    const double a = x[r];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
+    if (wave == 0) nrm += a * a;
+  }
+  write_column_partials<MAXK>(acc, nrm, nq, with_norm, cs, red, partials, pstride, pidx);
+}
+
+// out = x - sum_{i<nq} c_i V[:, i] with c_i = h_i cs_i; partials of
+// q_i . out (quantity i) and ||out||^2 (quantity nq).  out is not a column
+// in [0, nq).
+template <int MAXK>
+__device__ __forceinline__ void sweep_update_dot(const double* __restrict__ V, long long ld, int nq,
+                                                 const double* h, const double* cs,
+                                                 const double* __restrict__ x, double* __restrict__ out,
+                                                 TileRange tr, double* partials, long long pstride,
+                                                 long long pidx) {
+  __shared__ double part[2][kWavesPerBlock][kWave];
+  __shared__ double red[4 * MAXK + 1];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  long long col[MAXK];
+  own_columns<MAXK>(col, ld, 0, nq);
+  double c[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const int i = wave + 4 * k;
+    c[k] = (i < nq) ? h[i] * cs[i] : 0.0;
+  }
+  double acc[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
+  double nrm = 0.0;
+  int buf = 0;
+  for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
+    const long long r = tr.base + t * kWave + lane;
+    const long long rc = r < tr.rend ? r : tr.rend - 1;
+    double v[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
+    const double xr = x[rc];
+    double p = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
+    part[buf][wave][lane] = p;
+    __syncthreads();
+    const double a = xr - ((part[buf][0][lane] + part[buf][1][lane]) +
+                           (part[buf][2][lane] + part[buf][3][lane]));
+    if (r < tr.rend) {
+      if (wave == 0) {
+        out[r] = a;
+        nrm += a * a;
+      }
+#pragma unroll
+      for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
+    }
+    buf ^= 1;
+  }
+  write_column_partials<MAXK>(acc, nrm, nq, true, cs, red, partials, pstride, pidx);
+}
+
+// io = io - sum_{i<nq} h_i cs_i V[:, i] in place; partial ||io||^2 (quantity 0).
+template <int MAXK>
+__device__ __forceinline__ void sweep_update_norm(const double* __restrict__ V, long long ld, int nq,
+                                                  const double* h, const double* cs,
+                                                  double* __restrict__ io, TileRange tr,
+                                                  double* partials, long long pidx) {
+  __shared__ double part[2][kWavesPerBlock][kWave];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  long long col[MAXK];
+  own_columns<MAXK>(col, ld, 0, nq);
+  double c[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const int i = wave + 4 * k;
+    c[k] = (i < nq) ? h[i] * cs[i] : 0.0;
+  }
+  double nrm = 0.0;
+  int buf = 0;
+  for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
+    const long long r = tr.base + t * kWave + lane;
+    const long long rc = r < tr.rend ? r : tr.rend - 1;
+    double v[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
+    const double xr = (wave == 0) ? io[rc] : 0.0;
+    double p = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
+    part[buf][wave][lane] = p;
+    __syncthreads();
+    if (wave == 0 && r < tr.rend) {
+      const double a = xr - ((part[buf][0][lane] + part[buf][1][lane]) +
+                             (part[buf][2][lane] + part[buf][3][lane]));
+      io[r] = a;
+      nrm += a * a;
+    }
+    buf ^= 1;
+  }
+  // only wave 0 holds a norm; lane 0 of wave 0 writes it after the wave sum
+  const double sn = wave_sum(nrm);
+  if (threadIdx.x == 0) partials[pidx] = sn;
+}
+
+// y = sum_{u<nu} coef_u V[:, used_u] (coefficients already include the
+// column scales); partial ||y||^2 (quantity 0).  used and coef live in LDS.
+template <int MAXK>
+__device__ __forceinline__ void sweep_lift(const double* __restrict__ V, long long ld, int nu,
+                                           const int* used, const double* coef,
+                                           double* __restrict__ y, TileRange tr, double* partials,
+                                           long long pidx) {
+  __shared__ double part[2][kWavesPerBlock][kWave];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  long long col[MAXK];
+  double c[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    const int u = wave + 4 * k;
+    const int uc = u < nu ? u : nu - 1;
+    col[k] = static_cast<long long>(used[uc]) * ld;
+    c[k] = (u < nu) ? coef[uc] : 0.0;
+  }
+  double nrm = 0.0;
+  int buf = 0;
+  for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
+    const long long r = tr.base + t * kWave + lane;
+    const long long rc = r < tr.rend ? r : tr.rend - 1;
+    double v[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
+    double p = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
+    part[buf][wave][lane] = p;
+    __syncthreads();
+    if (wave == 0 && r < tr.rend) {
+      const double a = (part[buf][0][lane] + part[buf][1][lane]) +
+                       (part[buf][2][lane] + part[buf][3][lane]);
+      y[r] = a;
+      nrm += a * a;
+    }
+    buf ^= 1;
+  }
+  const double sn = wave_sum(nrm);
+  if (threadIdx.x == 0) partials[pidx] = sn;
+}
+
+// Dispatch a sweep on the column count: MAXK = ceil(count / 4) rounded up to
+// an even bucket (2, 4, ..., 16), so up to 64 columns.
+#define MEK_SWEEP_BUCKETS(count, LAUNCH) \
+  do {                                   \
+    const int mek_c_ = (count);          \
+    if (mek_c_ <= 8) LAUNCH(2);          \
+    else if (mek_c_ <= 16) LAUNCH(4);    \
+    else if (mek_c_ <= 24) LAUNCH(6);    \
+    else if (mek_c_ <= 32) LAUNCH(8);    \
+    else if (mek_c_ <= 40) LAUNCH(10);   \
+    else if (mek_c_ <= 48) LAUNCH(12);   \
+    else if (mek_c_ <= 56) LAUNCH(14);   \
+    else LAUNCH(16);                     \
+  } while (0)
+
+constexpr int kSweepMaxCols = 64;
+
+}  // namespace mek

@@ -32,6 +32,14 @@ CAL_BYTES = 1 << 30  # calibration buffer size (bytes read = bytes written)
 
 # kernel symbol prefix -> kind reported by the runner's event timers
 KINDS = [
+    ("kb_spmv", "spmv"),
+    ("kb_dots1", "dots"),
+    ("kb_update_dot", "update_dot"),
+    ("kb_update_norm", "update_norm"),
+    ("kb_lift", "lift"),
+    ("kb_expm", "expm"),
+    ("kb_fin", "finalise"),
+    ("kb_", "batch_other"),
     ("k_spmv_dot", "spmv"),
     ("k_update_dot", "update_dot"),
     ("k_update_norm", "update_norm"),
@@ -71,6 +79,23 @@ WORKLOADS = {
 }
 
 
+def batch_workload(args, mek, gen):
+    """256 problems of the batch benchmark in one Expokit-controlled lockstep
+    solve with FP32 values (one warm-up solve, one measured)."""
+    probs = gen.batch_problems(P=256, seed=0, f32_exact=True)
+    mats = [(q["rp"], q["ci"], q["va"]) for q in probs]
+    ms = [min(q["m"], q["rp"].size - 2) for q in probs]
+    gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+    B = mek.Batch(mats, [q["t"] for q in probs], ms, 1e-8, values_f32=True, gs_mode=gs_mode,
+                  step_control="expokit")
+    B.upload([q["v"] for q in probs])
+    for _ in range(2):
+        B.solve()
+    mek.synchronize()
+    print(f"[profile] batch256 solves done, {B.cycles()} cycles", file=sys.stderr, flush=True)
+    B.close()
+
+
 def workload(args):
     import torch
     mek = _load("mek", "__init__.py")
@@ -82,6 +107,9 @@ def workload(args):
     torch.cuda.synchronize()
     del src, dst
     torch.cuda.empty_cache()
+    if args.workload == "batch256":
+        batch_workload(args, mek, gen)
+        return
     w = WORKLOADS[args.workload]
     if w["kind"] == "laplacian":
         rp, ci, va = gen.laplacian_2d(w["N"])
@@ -215,7 +243,7 @@ def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
     a = sub.add_parser("workload")
-    a.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS))
+    a.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS) + ["batch256"])
     a.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"])
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)

@@ -71,3 +71,34 @@ def test_batch_f32_stencils_and_substeps(gpu, oracle):
         import scipy.sparse.linalg as spla
         A = sp.csr_matrix((va, ci, rp))
         assert rel_err(y, spla.expm_multiply(t * A, v)) <= 1e-8
+
+
+@pytest.mark.parametrize("values_f32", [False, True])
+def test_batch_expokit_matches_single_runner(gpu, values_f32):
+    """Batched Expokit step control takes the single runner's steps (same
+    first step from ||A||_inf, same accept/reject rule) and gives its y."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    mats, ts, ms, vs = _problems(P=10, seed=5)
+    if values_f32:  # FP32 storage is lossless for values exactly representable in FP32
+        mats = [(rp, ci, va.astype(np.float32).astype(np.float64)) for rp, ci, va in mats]
+    tol = 1e-9
+    B = mek.Batch(mats, ts, ms, tol, step_control="expokit", gs_mode=1, values_f32=values_f32)
+    B.upload(vs)
+    B.solve()
+    ys = B.download()
+    print(f"[batch-expokit] cycles={B.cycles()} f32={values_f32}", flush=True)
+    for p, ((rp, ci, va), t, m, v, y) in enumerate(zip(mats, ts, ms, vs, ys)):
+        R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=m, t=t, tol=tol, max_restarts=0,
+                       step_control="expokit", gs_mode=1, values_f32=values_f32)
+        y1 = R.expmv(v)
+        st = R.stats()
+        R.close()
+        s = B.stats(p)
+        assert s["converged"] == 1, p
+        assert s["accepted"] == st["accepted"], (p, s, st)
+        assert s["attempts"] == st["attempts"], (p, s, st)
+        assert rel_err(y, y1) <= 1e-11, (p, rel_err(y, y1))
+        A = sp.csr_matrix((va, ci, rp))
+        assert rel_err(y, spla.expm_multiply(t * A, v)) <= 1e-8, p
+    B.close()
