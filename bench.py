@@ -103,7 +103,9 @@ def run_batch(args, mek, gen):
     (FP64 values, Expokit step control).  Batched forms: every problem in one
     lockstep solve replayed from a hipGraph with Expokit step control on the
     device, FP64 then FP32 values, then with (m, first sub-step) from the
-    learned selector and FP32 values.  Every form uses the DGKS second pass.
+    learned selector and FP32 values.  Every form uses the DGKS second pass
+    and Expokit's per-unit-time tolerance tol/t, relative to the norm of each
+    step's start vector.
     Errors are against a tight-tolerance (1e-13, m = 60) FP64 solve of each
     problem."""
     sel = _load("mek_selector", os.path.join(ROOT, "matrix-exp-krylov_amd", "selector.py"))
@@ -124,7 +126,7 @@ def run_batch(args, mek, gen):
     log("reference solves (tol 1e-13)")
     refs = []
     for c, t, v, n in zip(csrs, ts, vs, rows):
-        R = mek.Runner(c, m=min(60, n - 1), t=t, tol=1e-13, max_restarts=0, step_control="expokit",
+        R = mek.Runner(c, m=min(60, n - 1), t=t, tol=1e-13, max_restarts=0, step_control="expokit_rel",
                        gs_mode=1)
         refs.append(R.expmv(v))
         R.close()
@@ -134,8 +136,12 @@ def run_batch(args, mek, gen):
 
     # plain FP64: one runner per problem, solves in turn, v and y resident
     log("plain FP64, one problem at a time")
-    runners = [mek.Runner(c, m=m, t=t, tol=tol, max_restarts=0, step_control="expokit", gs_mode=gs_mode)
-               for c, m, t in zip(csrs, ms, ts)]
+    # Expokit's tolerance is per unit time: tol / t bounds the summed local
+    # errors of each problem by about tol, measured relative to the norm of
+    # each step's start vector (relative mode)
+    tols = [tol / t for t in ts]
+    runners = [mek.Runner(c, m=m, t=t, tol=tl, max_restarts=0, step_control="expokit_rel", gs_mode=gs_mode)
+               for c, m, t, tl in zip(csrs, ms, ts, tols)]
     dv = [mek.DeviceBuffer.from_array(v) for v in vs]
     dy = [mek.DeviceBuffer(8 * n) for n in rows]
     for R, a, b in zip(runners, dv, dy):  # warm-up
@@ -158,8 +164,8 @@ def run_batch(args, mek, gen):
 
     def batched(name, m_list, s_list, f32, select_s=0.0):
         log(f"batched: {name}")
-        B = mek.Batch(csrs, ts, m_list, tol, substeps=s_list, values_f32=f32, gs_mode=gs_mode,
-                      step_control="expokit")
+        B = mek.Batch(csrs, ts, m_list, tols, substeps=s_list, values_f32=f32, gs_mode=gs_mode,
+                      step_control="expokit_rel")
         B.upload(vs)
         B.solve()  # warm-up: captures the cycle graph
         mek.synchronize()

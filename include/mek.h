@@ -79,7 +79,8 @@ typedef struct {
 /* Additive options (all-zero except gs_mode = -1, verbose = 1 reproduces the
  * reference; see mek_solver_options_default). */
 typedef struct {
-    int step_control;   /* 0 reference controller, 1 Expokit heuristic       */
+    int step_control;   /* 0 reference controller, 1 Expokit heuristic,
+                           2 Expokit with the local error relative to ||w|| */
     int gs_mode;        /* -1 follow reorthogonalize, 0 never, 1 always,
                            2 conditional (DGKS, ||w'|| < dgks_eta ||w||)      */
     double dgks_eta;
@@ -145,7 +146,8 @@ void mek_runner_destroy(mek_runner* R);
  * (substeps[p] > 0 sets the first sub-step to t/substeps with no step cap;
  * 0 keeps max_restarts).  step_control 1: Expokit step control (first step
  * from Expokit's formula, or t/substeps[p] when substeps[p] > 0, e.g. from
- * the learned selector). */
+ * the learned selector); 2: the same with the local error measured relative
+ * to the norm of each step's start vector. */
 typedef struct mek_batch mek_batch;
 int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
                      const double* tol, const int* max_restarts, const int* substeps,

@@ -50,6 +50,9 @@ namespace mek {
 enum class StepControl {
   Reference = 0,  // the reference controller (main.cu:1425-1518)
   Expokit = 1,    // Sidje 1998 (ACM TOMS 24(1)) step-size heuristic
+  // Expokit with the local error measured relative to the norm of the
+  // step's start vector: err_loc <= 1.2 dt tol ||w||
+  ExpokitRelative = 2,
 };
 
 // Additive solver options (defaults reproduce the reference exactly).

@@ -21,7 +21,7 @@ struct BatchProblem {
   double tol;
   int max_restarts;    // as ArnoldiParams::max_restarts
   int substeps;        // >0: first sub-step t/substeps and no step cap (learned selector)
-  int control = 0;     // 0 reference controller, 1 Expokit step control
+  int control = 0;     // 0 reference controller, 1 Expokit, 2 Expokit relative
 };
 
 // Launch sizes of one cycle, per Arnoldi column j: the problems are stored in
@@ -65,7 +65,8 @@ class BatchRunner {
   void destroy();
 
  private:
-  void run_cycle_eager();
+  void run_cycle_eager(int mcols);
+  hipGraphExec_t cycle_graph(int mcols);
   int P_ = 0;
   int M_ = 0;
   long long n_tot_ = 0;
@@ -77,8 +78,10 @@ class BatchRunner {
   std::vector<int> ext2int_;   // caller's problem index -> storage order
   BatchPlan plan_;
   hipStream_t stream_ = nullptr;
-  hipGraphExec_t graph_exec_ = nullptr;
-  hipGraph_t graph_ = nullptr;
+  // one captured cycle per basis size in use: a cycle builds only as many
+  // columns as the largest m among the still-active problems
+  std::vector<hipGraphExec_t> graph_exec_;
+  std::vector<hipGraph_t> graph_;
   int cycles_ = 0;
   double solve_ms_ = 0.0;
   // device buffers

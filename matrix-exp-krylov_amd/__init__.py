@@ -25,6 +25,9 @@ KERNEL_KINDS = {"spmv": 0, "update_dot": 1, "update_norm": 2, "dots": 3, "final"
                 "expm": 5, "lift": 6}
 
 
+STEP_CONTROL = {"reference": 0, "expokit": 1, "expokit_rel": 2}
+
+
 class MekError(RuntimeError):
     pass
 
@@ -274,7 +277,7 @@ class Runner:
         p = ArnoldiParams(int(m), float(t), float(tol), int(max_restarts), 1 if reorthogonalize else 0)
         o = SolverOptions()
         lib().mek_solver_options_default(ctypes.byref(o))
-        o.step_control = {"reference": 0, "expokit": 1}[step_control]
+        o.step_control = STEP_CONTROL[step_control]
         o.gs_mode = int(gs_mode)
         o.dgks_eta = float(dgks_eta)
         o.values_f32 = 1 if values_f32 else 0
@@ -357,7 +360,7 @@ class Batch:
         mr = _i32(np.broadcast_to(10 if max_restarts is None else max_restarts, P))
         ss = _i32(np.broadcast_to(0 if substeps is None else substeps, P))
         h = _vp()
-        ctrl = {"reference": 0, "expokit": 1}[step_control]
+        ctrl = STEP_CONTROL[step_control]
         _check(lib().mek_batch_create(P, handles, _pd(t), _pi(m), _pd(tol), _pi(mr), _pi(ss), ctrl,
                                       1 if values_f32 else 0, int(gs_mode), float(dgks_eta),
                                       1 if use_graph else 0, int(device), ctypes.byref(h)))

@@ -188,7 +188,7 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
     h->R = std::make_unique<ArnoldiRunner>(1, params, A->A.rows, A->A.cols, A->A.nnz);
     if (o) {
       mek::SolverOptions& so = h->R->options;
-      need(o->step_control == 0 || o->step_control == 1, "step_control");
+      need(o->step_control >= 0 && o->step_control <= 2, "step_control");
       need(o->gs_mode >= -1 && o->gs_mode <= 2, "gs_mode");
       so.control = static_cast<mek::StepControl>(o->step_control);
       so.gs_mode = o->gs_mode;
@@ -322,7 +322,7 @@ int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const 
   return guarded([&] {
     need(nprob >= 1 && A && t && m && tol && out, "batch arguments");
     need(gs_mode >= 0 && gs_mode <= 2, "gs_mode");
-    need(step_control == 0 || step_control == 1, "step_control");
+    need(step_control >= 0 && step_control <= 2, "step_control");
     std::vector<mek::BatchProblem> probs(nprob);
     for (int p = 0; p < nprob; ++p) {
       need(A[p] != nullptr, "matrix");

@@ -72,7 +72,9 @@ int main(int argc, char* argv[]) {
     if (a == "--f32-values") { opts.values_f32 = true; continue; }
     if (a == "--step-control" && has_next) {
       const std::string v = args[++i];
-      opts.control = (v == "expokit") ? mek::StepControl::Expokit : mek::StepControl::Reference;
+      opts.control = (v == "expokit")       ? mek::StepControl::Expokit
+                     : (v == "expokit-rel") ? mek::StepControl::ExpokitRelative
+                                            : mek::StepControl::Reference;
       continue;
     }
     if (a == "--gs" && has_next) {

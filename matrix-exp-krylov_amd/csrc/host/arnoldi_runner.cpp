@@ -349,7 +349,8 @@ void ArnoldiRunner::expokit_controller() {
   read_status();
   double beta = dc.h_status[mek::kVNorm];
   const double fact = std::pow((m + 1) / std::exp(1.0), m + 1) * std::sqrt(2.0 * M_PI * (m + 1));
-  double t_new = (1.0 / anorm) * std::pow((fact * tol) / (4.0 * beta * anorm), 1.0 / m);
+  const double tol0 = (options.control == mek::StepControl::ExpokitRelative) ? tol * beta : tol;
+  double t_new = (1.0 / anorm) * std::pow((fact * tol0) / (4.0 * beta * anorm), 1.0 / m);
   t_new = round_step(t_new);
   double t_now = 0.0, s_error = 0.0, err_loc = 0.0;
   int nstep = 0;
@@ -380,10 +381,19 @@ void ArnoldiRunner::expokit_controller() {
     }
     int ireject = 0;
     double xm = 1.0 / m;
+    const bool relative = options.control == mek::StepControl::ExpokitRelative;
+    double tol_eff = tol;
     for (;;) {
       launch_expm(t_step, 1);
       read_status();
       const double* F = dc.h_status + 20;  // first column of exp(t_step * H_aug)
+      if (relative) {
+        // local error against the smaller of ||w|| at the step's start and
+        // its end, ||w_end|| = beta ||F(0:mb)|| (orthonormal basis)
+        double f2 = 0.0;
+        for (int i = 0; i < mb + std::max(0, k1 - 1); ++i) f2 += F[i] * F[i];
+        tol_eff = tol * beta * std::min(1.0, std::sqrt(f2));
+      }
       if (!breakdown) avnorm = dc.h_status[mek::kAvNorm];
       stats.attempts++;
       if (k1 == 0) {
@@ -402,9 +412,9 @@ void ArnoldiRunner::expokit_controller() {
         err_loc = phi1;
         xm = 1.0 / (m - 1);
       }
-      if (err_loc <= delta * t_step * tol) break;
+      if (err_loc <= delta * t_step * tol_eff) break;
       if (ireject == mxrej) throw mek::Error("ERROR: The requested tolerance is too high.");
-      t_step = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+      t_step = round_step(gamma * t_step * std::pow(t_step * tol_eff / err_loc, xm));
       ++ireject;
       stats.rejected++;
     }
@@ -419,7 +429,7 @@ void ArnoldiRunner::expokit_controller() {
     }
     stats.steps.push_back({t_now, t_step, err_loc, mb, nstep - 1, ireject});
     t_now += t_step;
-    t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+    t_new = round_step(gamma * t_step * std::pow(t_step * tol_eff / err_loc, xm));
     err_loc = std::max(err_loc, rndoff);
     s_error += err_loc;
   }
@@ -444,7 +454,7 @@ void ArnoldiRunner::solve_on_device(const double* d_v, const double* h_v) {
     return;
   }
   init_q1_device(d_v, h_v);
-  if (options.control == mek::StepControl::Expokit) expokit_controller();
+  if (options.control != mek::StepControl::Reference) expokit_controller();
   else reference_controller();
   MEK_HIP(hipStreamSynchronize(dc.stream_compute));
   flush_timing();

@@ -121,7 +121,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     pscal_init_[p * kPScal + kBT] = probs[p].t;
     pscal_init_[p * kPScal + kBTol] = probs[p].tol;
     pscal_init_[p * kPScal + kBAnorm] = probs[p].A->norm_inf();
-    if (probs[p].control == 1) any_expokit = 1;
+    if (probs[p].control >= 1) any_expokit = 1;
   }
 
   // ---- device
@@ -199,9 +199,23 @@ void BatchRunner::upload_v(const std::vector<const double*>& v) {
   MEK_HIP(hipStreamSynchronize(stream_));
 }
 
-void BatchRunner::run_cycle_eager() {
-  batch_launch_cycle(*static_cast<BatchBufs*>(bufs_), plan_, gs_mode_, eta_, stream_);
+void BatchRunner::run_cycle_eager(int mcols) {
+  batch_launch_cycle(*static_cast<BatchBufs*>(bufs_), plan_, mcols, gs_mode_, eta_, stream_);
   MEK_HIP_LAUNCHED();
+}
+
+hipGraphExec_t BatchRunner::cycle_graph(int mcols) {
+  if (graph_exec_.empty()) {
+    graph_exec_.assign(M_ + 1, nullptr);
+    graph_.assign(M_ + 1, nullptr);
+  }
+  if (!graph_exec_[mcols]) {
+    MEK_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    batch_launch_cycle(*static_cast<BatchBufs*>(bufs_), plan_, mcols, gs_mode_, eta_, stream_);
+    MEK_HIP(hipStreamEndCapture(stream_, &graph_[mcols]));
+    MEK_HIP(hipGraphInstantiate(&graph_exec_[mcols], graph_[mcols], nullptr, nullptr, 0));
+  }
+  return graph_exec_[mcols];
 }
 
 void BatchRunner::solve_device() {
@@ -215,22 +229,18 @@ void BatchRunner::solve_device() {
   batch_prepare();
   batch_launch_init(b, stream_);
   MEK_HIP_LAUNCHED();
-  if (use_graph_ && !graph_exec_) {
-    MEK_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    batch_launch_cycle(b, plan_, gs_mode_, eta_, stream_);
-    MEK_HIP(hipStreamEndCapture(stream_, &graph_));
-    MEK_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-  }
   cycles_ = 0;
+  int mcols = M_;
   const int max_cycles = 1000000;
   for (;;) {
-    if (use_graph_) MEK_HIP(hipGraphLaunch(graph_exec_, stream_));
-    else run_cycle_eager();
-    MEK_HIP(hipMemcpyAsync(h_counters_, d_counters_, sizeof(int) * 2, hipMemcpyDeviceToHost, stream_));
+    if (use_graph_) MEK_HIP(hipGraphLaunch(cycle_graph(mcols), stream_));
+    else run_cycle_eager(mcols);
+    MEK_HIP(hipMemcpyAsync(h_counters_, d_counters_, sizeof(int) * 3, hipMemcpyDeviceToHost, stream_));
     MEK_HIP(hipStreamSynchronize(stream_));
     ++cycles_;
     if (h_counters_[0] == 0) break;
     if (cycles_ >= max_cycles) throw Error("ERROR: batch did not finish within the cycle limit");
+    mcols = std::min(std::max(h_counters_[2], 1), M_);
   }
   solve_ms_ = now_ms() - t0;
 }
@@ -301,10 +311,12 @@ void BatchRunner::features(float* out) {
 void BatchRunner::destroy() {
   if (!bufs_) return;
   (void)hipSetDevice(device_);
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
-  graph_exec_ = nullptr;
-  graph_ = nullptr;
+  for (hipGraphExec_t g : graph_exec_)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (hipGraph_t g : graph_)
+    if (g) (void)hipGraphDestroy(g);
+  graph_exec_.clear();
+  graph_.clear();
   void* ptrs[] = {d_row_ptr_, d_col_idx_, d_rb_start_, d_rb_prob_, d_sb_prob_, d_sb_row0_, d_p_sb0_,
                   d_pint_, d_counters_, d_vals_, d_V_, d_w_, d_y_, d_v0_, d_small_, d_partials_,
                   d_expm_ws_, d_pscal_};

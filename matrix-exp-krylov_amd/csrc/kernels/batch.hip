@@ -80,7 +80,7 @@ constexpr int kExpokitMaxSteps = 100000;
 // Problem p takes part in the Expokit ||A v_{m+1}|| product of this cycle.
 __device__ __forceinline__ bool av_active(const BatchBufs& b, int p) {
   const int* s = pi(b, p);
-  return s[kBActive] && s[kBNeedBasis] && s[kBCtrl] == 1 && !s[kBBreakdown];
+  return s[kBActive] && s[kBNeedBasis] && s[kBCtrl] >= 1 && !s[kBBreakdown];
 }
 
 template <typename VT>
@@ -121,19 +121,17 @@ __global__ __launch_bounds__(kBlock) void kb_init(BatchBufs b) {
   const double dt_max = t, dt_min = fmax(1e-12, t * 1e-6);
   s[kBDtMax] = dt_max;
   s[kBDtMin] = dt_min;
-  if (it[kBCtrl] == 1) {
+  if (it[kBCtrl] >= 1) {
     // Expokit's first step from ||A||_inf, beta and tol (expv), or t/substeps
     // when the caller predicted the sub-step count
     const int m = it[kBM];
     const double anorm = s[kBAnorm] > 0.0 ? s[kBAnorm] : 1.0;
-    double t_new;
-    if (it[kBSubsteps] > 0) {
-      t_new = t / it[kBSubsteps];
-    } else {
-      const double fact = pow((m + 1) / exp(1.0), m + 1) * sqrt(2.0 * M_PI * (m + 1));
-      const double bet = beta > 0.0 ? beta : 1.0;
-      t_new = round_step((1.0 / anorm) * pow((fact * s[kBTol]) / (4.0 * bet * anorm), 1.0 / m));
-    }
+    const double fact = pow((m + 1) / exp(1.0), m + 1) * sqrt(2.0 * M_PI * (m + 1));
+    const double bet = beta > 0.0 ? beta : 1.0;
+    const double tol0 = (it[kBCtrl] == 2) ? s[kBTol] * bet : s[kBTol];
+    const double t_heur = round_step((1.0 / anorm) * pow((fact * tol0) / (4.0 * bet * anorm), 1.0 / m));
+    const double t_new = (it[kBSubsteps] > 0) ? t / it[kBSubsteps] : t_heur;
+    s[kBDtMin] = t_heur;  // Expokit's own first step: the fallback after a rejection
     s[kBTNew] = t_new;
     s[kBDt] = fmin(t_new, t);
     s[kBSErr] = 0.0;
@@ -144,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void kb_init(BatchBufs b) {
   s[kBTDone] = 0.0;
   s[kBErr] = 0.0;
   s[kBLastRes] = 0.0;
-  if (it[kBCtrl] == 1) it[kBMaxSteps] = (R > 0) ? max(R, kExpokitMaxSteps) : kExpokitMaxSteps;
+  if (it[kBCtrl] >= 1) it[kBMaxSteps] = (R > 0) ? max(R, kExpokitMaxSteps) : kExpokitMaxSteps;
   else it[kBMaxSteps] = (it[kBSubsteps] > 0) ? 1000000 : (R > 0 ? R : 1000);
   it[kBActive] = (t > 0.0) ? 1 : 0;
   it[kBNeedBasis] = 1;
@@ -192,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void kb_cycle_begin(BatchBufs b) {
       it[kBKUsed] = 0;
       it[kBRejects] = 0;
       it[kBIReject] = 0;
-      if (it[kBCtrl] == 1) s[kBDt] = fmin(s[kBTNew], s[kBT] - s[kBTDone]);
+      if (it[kBCtrl] >= 1) s[kBDt] = fmin(s[kBTNew], s[kBT] - s[kBTDone]);
       else s[kBDt] = fmin(s[kBDt], s[kBT] - s[kBTDone]);
     }
   }
@@ -390,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void kb_expm(BatchBufs b) {
   if (!it[kBActive]) return;
   const int k = it[kBKUsed];
   const int brk = it[kBBreakdown];
-  const int expokit = it[kBCtrl] == 1;
+  const int expokit = it[kBCtrl] >= 1;
   const int ldh = b.M + 2;
   const double* H = b.H + static_cast<long long>(p) * ldh * ldh;
   double* wH = b.wH + static_cast<long long>(p) * (b.M + 4);
@@ -518,9 +516,17 @@ __device__ void control_expokit(const BatchBufs& b, int p) {
   int* it = pi(b, p);
   const int m = it[kBM];
   const int brk = it[kBBreakdown];
-  const double tol = s[kBTol];
   const double beta = s[kBVNorm];
   const double* F = b.wH + static_cast<long long>(p) * (b.M + 4);
+  // relative mode (2): local error against the smaller of ||w|| at the
+  // step's start and at its end, ||w_end|| = beta ||F(0:m)|| (orthonormal basis)
+  double tol = s[kBTol];
+  if (it[kBCtrl] == 2) {
+    double f2 = 0.0;
+    const int nf = brk ? it[kBKUsed] : m + 1;
+    for (int i = 0; i < nf; ++i) f2 += F[i] * F[i];
+    tol *= beta * fmin(1.0, sqrt(f2));
+  }
   const double t_step = s[kBDt];
   it[kBAttempts] += 1;
   double err_loc, xm = 1.0 / m;
@@ -552,7 +558,10 @@ __device__ void control_expokit(const BatchBufs& b, int p) {
       s[kBLastRes] = err_loc;
       return;
     }
-    s[kBDt] = round_step(kExpokitGamma * t_step * pow(t_step * tol / err_loc, xm));
+    double t_next = round_step(kExpokitGamma * t_step * pow(t_step * tol / err_loc, xm));
+    // a predicted step that fails falls back to Expokit's own (safe) first step
+    if (it[kBSubsteps] > 0 && !(t_next <= s[kBDtMin])) t_next = s[kBDtMin];
+    s[kBDt] = t_next;
     it[kBIReject] += 1;
     return;
   }
@@ -578,14 +587,18 @@ __device__ void control_expokit(const BatchBufs& b, int p) {
 // One thread per problem; writes the number of still-active problems.
 __global__ __launch_bounds__(1024) void kb_control(BatchBufs b) {
   __shared__ int cnt[1024 / kWave];
-  int active = 0;
+  __shared__ int mx[1024 / kWave];
+  int active = 0, mmax = 0;
   for (int p = threadIdx.x; p < b.P; p += blockDim.x) {
     int* it = pi(b, p);
     if (!it[kBActive]) continue;
-    if (it[kBCtrl] == 1) control_expokit(b, p);
+    if (it[kBCtrl] >= 1) control_expokit(b, p);
     else control_reference(b, p);
     active += it[kBActive];
+    if (it[kBActive]) mmax = max(mmax, it[kBM]);
   }
+  for (int off = kWave / 2; off >= 1; off >>= 1) mmax = max(mmax, __shfl_xor(mmax, off));
+  if (threadIdx.x % kWave == 0) mx[threadIdx.x / kWave] = mmax;
   // count of active problems (fixed-order reduction, no atomics)
   for (int off = kWave / 2; off >= 1; off >>= 1) active += __shfl_xor(active, off);
   if (threadIdx.x % kWave == 0) cnt[threadIdx.x / kWave] = active;
@@ -595,6 +608,9 @@ __global__ __launch_bounds__(1024) void kb_control(BatchBufs b) {
     for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) tot += cnt[w];
     b.counters[0] = tot;
     b.counters[1] += 1;  // cycles
+    int m_act = 0;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) m_act = max(m_act, mx[w]);
+    b.counters[2] = m_act;
   }
 }
 
@@ -612,12 +628,12 @@ void batch_launch_init(const BatchBufs& b, hipStream_t s) {
   hipLaunchKernelGGL(kb_init, dim3(b.P), dim3(kBlock), 0, s, b);
 }
 
-void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int gs_mode, double eta,
+void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int mcols, int gs_mode, double eta,
                         hipStream_t s) {
   const int pg = (b.P + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(kb_restart_copy, dim3(b.nsb), dim3(kBlock), 0, s, b);
   hipLaunchKernelGGL(kb_cycle_begin, dim3(pg), dim3(kBlock), 0, s, b);
-  for (int j = 0; j < b.M; ++j) {
+  for (int j = 0; j < mcols; ++j) {
     // problems are ordered by decreasing m: column j concerns a prefix of them
     const int np = plan.np[j], nsb = plan.nsb[j], nrb = plan.nrb[j];
     const int sg = plan.spmv_grid[j];
@@ -655,7 +671,7 @@ void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int gs_mode, 
   }
   hipLaunchKernelGGL(kb_control, dim3(1), dim3(1024), 0, s, b);
 #define MEK_L(K) hipLaunchKernelGGL(kb_lift<K>, dim3(b.nsb), dim3(kBlock), 0, s, b)
-  MEK_SWEEP_BUCKETS(b.M + 2, MEK_L);
+  MEK_SWEEP_BUCKETS(mcols + 2, MEK_L);
 #undef MEK_L
   hipLaunchKernelGGL(kb_fin_lift, dim3(b.P), dim3(kBlock), 0, s, b);
 }

@@ -25,7 +25,7 @@ enum BatchInt : int {
   kBActive = 0, kBNeedBasis, kBRestart, kBBreakdown, kBReorth, kBKUsed, kBM,
   kBAccepted, kBRejects, kBConverged, kBAborted, kBAttempts, kBColumns,
   kBReorthCount, kBMaxRestarts, kBMaxSteps, kBSubsteps,
-  kBCtrl,      // 0 reference controller, 1 Expokit step control
+  kBCtrl,      // 0 reference controller, 1 Expokit, 2 Expokit with relative local error
   kBIReject,   // Expokit: rejections of the current step
   kBLift,      // 1: lift y in this cycle
   kBKLift,     // basis vectors in the lift
@@ -59,13 +59,15 @@ struct BatchBufs {
   int* pint;                // P x kPInt
   double* partials;         // (M+2) x nsb
   double* expm_ws;          // P x 6 M^2 when M > the LDS limit
-  int* counters;            // [0] active problems, [1] cycles
+  int* counters;            // [0] active problems, [1] cycles, [2] largest m of the active
   int any_expokit;          // 1 if some problem uses Expokit step control
 };
 
 void batch_prepare();
 void batch_launch_init(const BatchBufs& b, hipStream_t s);
-void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int gs_mode, double eta,
+// One cycle that builds mcols Arnoldi columns (the largest m of the problems
+// still active; those with a smaller m skip the later columns).
+void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int mcols, int gs_mode, double eta,
                         hipStream_t s);
 // Largest exponential order of a cycle (M, or M + 2 with Expokit problems).
 __host__ __device__ inline int batch_expm_order(const BatchBufs& b) { return b.any_expokit ? b.M + 2 : b.M; }

@@ -5,23 +5,23 @@ Pipeline (every GPU step runs through libmek.so):
 1. ``training_set``: synthetic matrices (2-D Laplacians, 3-D upwind
    convection-diffusion, random Gershgorin-negative sparse rows) with t chosen
    so that ||tA||_1 is log-uniform in [1, 1e4].
-2. ``label``: one m=60 Arnoldi decomposition of each problem on the GPU; for
-   every m on a grid the largest first sub-step dt with the reference's
-   residual estimate |h_{m+1,m} [exp(dt H_m) e1]_m| <= tol (bisection), hence
-   substeps s_m = ceil(t/dt).  The label is the m (and its s_m) that minimises
-   the modelled HBM bytes of the whole solve, s_m * (m * B_spmv + 3 sweeps of
-   Gram-Schmidt); the SpMV-minimal choice is recorded beside it.
+2. ``label``: one m=61 Arnoldi decomposition of each problem on the GPU; for
+   every m on a grid the largest first sub-step dt that Expokit's step
+   control accepts (local error of the augmented exponential, err_loc <=
+   1.2 dt tol; bisection), hence substeps s_m = ceil(t/dt).  The label is the
+   m (and its s_m) that minimises the modelled HBM bytes of the whole
+   mixed-precision solve, s_m * ((m+1) B_spmv + 2 Gram-Schmidt sweeps per
+   column + lift); the SpMV-minimal choice is recorded beside it.
 3. ``train``: features on the GPU (``mek.features``), standardised; the MLP
    (16 -> 64 -> 64 -> 2) is trained on the GPU by the library's own kernels.
 4. ``Selector.predict``: (m, substeps) per problem; the batched solver runs
-   them with the reference controller, whose a-posteriori residual estimate
-   judges every step (a rejected step shrinks dt), so a poor prediction costs
-   time, not accuracy.
+   them under Expokit step control with t/substeps as the first step, so the
+   local error estimate judges every step and a rejected step falls back to
+   Expokit's own step formula: a poor prediction costs time, not accuracy.
 
-The heuristic it is compared with: m = 30 (the reference CLI's default) and
-the number of sub-steps from the Expokit initial step (Sidje 1998):
-dt = (1/||A||) ((fact tol) / (4 beta ||A||))^(1/m), fact = ((m+1)/e)^(m+1)
-sqrt(2 pi (m+1)).
+The heuristic it is compared with: the problem's own m and Expokit's initial
+step (Sidje 1998): dt = (1/||A||) ((fact tol) / (4 beta ||A||))^(1/m),
+fact = ((m+1)/e)^(m+1) sqrt(2 pi (m+1)), then Expokit's step control.
 """
 from __future__ import annotations
 
@@ -56,7 +56,8 @@ def norm1(rp, ci, va, n):
 
 
 def training_set(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
-    """List of (row_ptr, col_idx, values, t, kind) with ||tA||_1 log-uniform in [1, 1e4]."""
+    """List of (row_ptr, col_idx, values, t, kind) with ||tA||_1 log-uniform in
+    [1, 1e4]; values are rounded to FP32 so FP32 storage is exact."""
     _, gen = _mods()
     rng = np.random.default_rng(seed)
     out = []
@@ -72,6 +73,7 @@ def training_set(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
         else:
             n = int(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))))
             rp, ci, va = gen.random_rows(n, 5, 30, seed=seed * 100003 + i)
+        va = va.astype(np.float32).astype(np.float64)  # FP32-exact values: lossless mixed storage
         n = rp.size - 1
         tau = math.exp(rng.uniform(0.0, math.log(1e4)))
         t = tau / norm1(rp, ci, va, n)
@@ -79,57 +81,77 @@ def training_set(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
     return out
 
 
-def modelled_bytes(m: int, s: int, n: int, nnz: int) -> float:
-    """HBM bytes of s sub-steps with an m-column basis: SpMV (FP64 values,
-    i32 indices) + three Gram-Schmidt sweeps per column + lift."""
-    bspmv = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n
-    gs = 3 * 8.0 * n * m * (m + 1) / 2 + 5 * 8.0 * n * m
-    return s * (m * bspmv + gs + 8.0 * n * (m + 2))
+def modelled_bytes(m: int, s: int, n: int, nnz: int, value_bytes: int = 4, sweeps: int = 2) -> float:
+    """HBM bytes of s sub-steps with an m-column basis: m + 1 SpMVs (i32
+    indices, values of value_bytes), `sweeps` Gram-Schmidt sweeps per column
+    (two: one dot sweep and one update sweep when the conditional second pass
+    is not taken) and the lift."""
+    bspmv = (value_bytes + 4.0) * nnz + 4.0 * (n + 1) + 16.0 * n
+    gs = sweeps * 8.0 * n * m * (m + 1) / 2 + (2 * sweeps + 1) * 8.0 * n * m
+    return s * ((m + 1) * bspmv + gs + 8.0 * n * (m + 2))
 
 
-def max_substep(Hm: np.ndarray, h_next: float, t: float, tol: float) -> float:
-    """Largest dt in (0, t] with |h_next [exp(dt H_m) e1]_{m-1}| <= tol (beta = 1)."""
-    def err(dt):
-        return abs(h_next * scipy.linalg.expm(dt * Hm)[-1, 0])
-    if err(t) <= tol:
+DELTA = 1.2  # Expokit's acceptance safety factor: err_loc <= DELTA * dt * tol
+
+
+def expokit_err(H: np.ndarray, m: int, dt: float, avnorm: float, beta: float = 1.0):
+    """Expokit's local error estimate of one step of length dt with the first m
+    Arnoldi columns of H ((k+1) x k, k > m): exponential of the augmented
+    (m+2) matrix, phi1 = |beta F_m|, phi2 = |beta F_{m+1} ||A v_{m+1}|||.
+    Returns (err_loc, ||F(0:m)||), the second being ||w_end|| / beta."""
+    aug = np.zeros((m + 2, m + 2))
+    aug[:m + 1, :m] = H[:m + 1, :m]
+    aug[m + 1, m] = 1.0
+    F = scipy.linalg.expm(dt * aug)[:, 0]
+    phi1 = abs(beta * F[m])
+    phi2 = abs(beta * F[m + 1] * avnorm)
+    fn = float(np.linalg.norm(F[:m + 1]))
+    if phi1 > 10.0 * phi2:
+        return phi2, fn
+    if phi1 > phi2:
+        return phi1 * phi2 / (phi1 - phi2), fn
+    return phi1, fn
+
+
+def max_substep(H: np.ndarray, m: int, t: float, tol: float) -> float:
+    """Largest dt in (0, t] that Expokit's step control in relative mode
+    accepts as a first step with an m-column basis (geometric bisection to 2
+    %): err_loc <= 1.2 dt tol min(1, ||w_end||) for ||v|| = 1."""
+    avnorm = float(np.linalg.norm(H[:m + 2, m]))  # ||A v_{m+1}|| = ||H[:, m]|| (orthonormal V)
+
+    def ok(dt):
+        err, fn = expokit_err(H, m, dt, avnorm)
+        return err <= DELTA * dt * tol * min(1.0, fn)
+    if ok(t):
         return t
-    lo, hi = 0.0, t
-    for _ in range(40):
-        mid = 0.5 * (lo + hi) if lo > 0 else hi * 1e-3
-        if err(mid) <= tol:
+    lo, hi = t * 1e-9, t
+    if not ok(lo):
+        return lo
+    while hi / lo > 1.02:
+        mid = math.sqrt(lo * hi)
+        if ok(mid):
             lo = mid
         else:
             hi = mid
-        if lo > 0 and hi / lo < 1.02:
-            break
-    return max(lo, t * 1e-9)
+    return lo
 
 
-def label(rp, ci, va, t, tol=1e-8, seed=0, device=0):
-    """(m, substeps) minimising modelled bytes, and the SpMV-minimal (m, s)."""
-    mek, gen = _mods()
-    n = rp.size - 1
-    nnz = int(rp[-1])
-    m_top = min(M_MAX, n - 1)
-    R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=m_top, t=t, tol=tol, device=device)
-    H, k = R.arnoldi(gen.normal_vector(n, seed=seed))
-    R.close()
-    best_b = best_s = None
+STEP_SAFETY = 0.8  # predicted first step = 0.8 x the largest accepted one
+
+
+def substeps_per_m(H: np.ndarray, k: int, t: float, tol: float) -> dict:
+    """{m: substeps} for every m of the grid below the Krylov dimension k
+    reached by the decomposition H; s_m = ceil(t / (0.8 dt_m)) with dt_m the
+    largest first step Expokit accepts at the per-unit-time tolerance tol/t.
+    A happy breakdown (k below the decomposition's size) gives {k: 1}."""
+    out = {}
     for m in M_GRID:
-        if m > k:
+        if m >= k:
+            out[k] = 1
             break
-        Hm = H[:m, :m]
-        h_next = H[m, m - 1] if m < H.shape[0] else 0.0
-        if m == k and k < m_top:  # happy breakdown: exact in one step
-            s = 1
-        else:
-            s = max(1, math.ceil(t / max_substep(Hm, h_next, t, tol) - 1e-12))
-        cb = modelled_bytes(m, s, n, nnz)
-        if best_b is None or cb < best_b[2]:
-            best_b = (m, s, cb)
-        if best_s is None or m * s < best_s[0] * best_s[1]:
-            best_s = (m, s)
-    return best_b[0], best_b[1], best_s
+        dt = max_substep(H, m, t, tol / t)
+        out[m] = max(1, math.ceil(t / (STEP_SAFETY * dt) - 1e-12))
+    return out
 
 
 def heuristic(anorm: float, t: float, tol: float = 1e-8, m: int = 30, beta: float = 1.0):
@@ -172,16 +194,64 @@ class Selector:
         return [decode(y) for y in Y]
 
 
-def build_dataset(problems, tol=1e-8, device=0, log=None):
-    mek, _ = _mods()
+def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
+    """Features and labels of every problem.  For each m of the grid the
+    first sub-step count s_m comes from one m=61 Arnoldi decomposition
+    (substeps_per_m); then every (m, s_m) is actually solved, a chunk of
+    problems at a time, by the batched mixed-precision solver under Expokit
+    step control (tol/t), and the label is the m whose solve moved the fewest
+    modelled HBM bytes, with its s_m.  The fewest-SpMV choice is recorded
+    beside it."""
+    mek, gen = _mods()
     X, T, meta = [], [], []
-    for i, (rp, ci, va, t, kind) in enumerate(problems):
-        m, s, (ms, ss) = label(rp, ci, va, t, tol, seed=i, device=device)
-        X.append(mek.features(mek.CSR.from_arrays(rp, ci, va), t, tol, device=device))
-        T.append(encode(m, s))
-        meta.append((kind, rp.size - 1, t, m, s, ms, ss))
-        if log and (i + 1) % 100 == 0:
-            log(f"labelled {i + 1}/{len(problems)}")
+    for c0 in range(0, len(problems), chunk):
+        part = problems[c0:c0 + chunk]
+        csrs, vs, cand = [], [], []
+        for i, (rp, ci, va, t, kind) in enumerate(part):
+            n = rp.size - 1
+            A = mek.CSR.from_arrays(rp, ci, va)
+            v = gen.normal_vector(n, seed=c0 + i)
+            R = mek.Runner(A, m=min(M_MAX + 1, n - 1), t=t, tol=tol, device=device)
+            H, k = R.arnoldi(v)
+            R.close()
+            X.append(mek.features(A, t, tol, device=device))
+            csrs.append(A)
+            vs.append(v)
+            cand.append(substeps_per_m(H, k, t, tol))
+        ts = [p[3] for p in part]
+        cost = [dict() for _ in part]
+        cols = [dict() for _ in part]
+        for m in M_GRID:
+            idx = [i for i in range(len(part)) if m in cand[i]]
+            if not idx:
+                continue
+            B = mek.Batch([csrs[i] for i in idx], [ts[i] for i in idx], [m] * len(idx),
+                          [tol / ts[i] for i in idx], substeps=[cand[i][m] for i in idx],
+                          values_f32=True, gs_mode=2, step_control="expokit_rel", device=device)
+            B.upload([vs[i] for i in idx])
+            B.solve()
+            for q, i in enumerate(idx):
+                st = B.stats(q)
+                rp = part[i][0]
+                n, nnz = rp.size - 1, int(rp[-1])
+                cols[i][m] = st["spmv_count"]
+                cost[i][m] = modelled_bytes(m, 1, n, nnz) * st["spmv_count"] / m
+            B.close()
+        for i, (rp, ci, va, t, kind) in enumerate(part):
+            # breakdown-sized candidates (m not on the grid) are solved in one step
+            for m, sm in cand[i].items():
+                if m not in cost[i]:
+                    n, nnz = rp.size - 1, int(rp[-1])
+                    cols[i][m] = m
+                    cost[i][m] = modelled_bytes(m, 1, n, nnz)
+            m_b = min(cost[i], key=lambda mm: cost[i][mm])
+            m_s = min(cols[i], key=lambda mm: cols[i][mm])
+            T.append(encode(m_b, cand[i][m_b]))
+            meta.append((kind, rp.size - 1, t, m_b, cand[i][m_b], m_s, cand[i][m_s]))
+        for A in csrs:
+            A.close()
+        if log:
+            log(f"labelled {min(c0 + chunk, len(problems))}/{len(problems)}")
     return np.array(X, np.float32), np.array(T, np.float32), meta
 
 
