@@ -162,15 +162,16 @@ def read_counter(d, counter):
 
 def summarize(args):
     stats = read_stats(args.stats)
-    fetch = read_counter(args.fetch, "FETCH_SIZE")
-    write = read_counter(args.write, "WRITE_SIZE")
+    have_counters = bool(args.fetch and args.write)
+    fetch = read_counter(args.fetch, "FETCH_SIZE") if have_counters else []
+    write = read_counter(args.write, "WRITE_SIZE") if have_counters else []
 
     def is_cal(name):
         return "neg" in name.lower()
 
-    cal_f = [v for n, v in fetch if is_cal(n)]
-    cal_w = [v for n, v in write if is_cal(n)]
-    if not cal_f or not cal_w:
+    cal_f = [v for n, v in fetch if is_cal(n)] or [CAL_BYTES / 1024.0]
+    cal_w = [v for n, v in write if is_cal(n)] or [CAL_BYTES / 1024.0]
+    if have_counters and (len(cal_f) == 0 or len(cal_w) == 0):
         raise SystemExit("calibration kernel not found in the counter runs")
     # counters read KiB; scale so the calibration kernel reads CAL_BYTES
     f_scale = CAL_BYTES / (sum(cal_f) / len(cal_f) * 1024.0)
@@ -201,8 +202,9 @@ def summarize(args):
         "bytes_per_launch": traffic}
     for k, v in traffic.items():
         out[k] = v
-    with open(args.out_json, "w") as f:
-        json.dump(out, f, indent=1)
+    if have_counters and args.out_json:
+        with open(args.out_json, "w") as f:
+            json.dump(out, f, indent=1)
 
     by_kind = defaultdict(lambda: [0, 0.0])
     lines = []
@@ -229,6 +231,8 @@ def summarize(args):
         for tot, name, calls, avg in lines[:25]:
             short = name.split("(")[0].replace("void ", "")
             f.write(f"| `{short}` | {calls} | {tot / 1e6:.3f} | {avg / 1e3:.1f} |\n")
+        if not have_counters:
+            return
         f.write("\n## HBM traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated)\n\n")
         f.write(f"Calibration: torch.neg on 1 GiB FP64 (1 GiB read, 1 GiB written); "
                 f"FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} "
@@ -247,11 +251,11 @@ def main():
     a.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"])
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
-    s.add_argument("--fetch", required=True)
-    s.add_argument("--write", required=True)
+    s.add_argument("--fetch", default=None)
+    s.add_argument("--write", default=None)
     s.add_argument("--workload", required=True)
     s.add_argument("--out-md", required=True)
-    s.add_argument("--out-json", required=True)
+    s.add_argument("--out-json", default=None)
     s.add_argument("--command", action="append")
     args = ap.parse_args()
     if args.mode == "workload":
