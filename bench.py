@@ -42,6 +42,7 @@ def log(msg):
 WORKLOADS = {
     "laplacian4096": dict(kind="laplacian", N=4096, t=10.0, m=40, tol=1e-8, f32=False),
     "batch256": dict(kind="batch", P=256, tol=1e-8),
+    "block2048": dict(kind="block", N=2048, t=10.0, m=16, tol=1e-8),
     "convdiff256_f32": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=True),
     "convdiff256_f64": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=False),
 }
@@ -223,6 +224,109 @@ def run_batch(args, mek, gen):
     print(json.dumps(out), flush=True)
 
 
+def run_block(args, mek, gen):
+    """2-D Laplacian on a 2048 x 2048 grid (n = 4,194,304) with a block of 32
+    N(0,1) right-hand sides (columns normalised, seed 0), t = 10, tol 1e-8:
+    block Arnoldi with 16 block steps (SpMM, block Gram-Schmidt on FP64 MFMA
+    with the conditional second pass, Cholesky QR, Taylor exponential of the
+    512 x 512 block Hessenberg matrix on the MFMA GEMM).  Plain form: the
+    single-vector solver (m = 40, DGKS, Expokit steps) on 8 of the columns in
+    turn.  Errors against the DST-I closed form of every column."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from closed_forms import laplacian_2d_expmv
+    w = WORKLOADS["block2048"]
+    N, t, tol, m = w["N"], w["t"], w["tol"], args.block_m or w["m"]
+    n = N * N
+    log("building the 2048^2 Laplacian and 32 right-hand sides")
+    rp, ci, va = gen.laplacian_2d(N)
+    V = np.random.default_rng(0).standard_normal((n, 32))
+    V /= np.linalg.norm(V, axis=0)
+    A = mek.CSR.from_arrays(rp, ci, va)
+    gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+    B = mek.Block(A, m=m, t=t, tol=tol, gs_mode=gs_mode, timing=True)
+    d_v = mek.DeviceBuffer.from_array(V.ravel())
+    d_y = mek.DeviceBuffer(8 * V.size)
+    for i in range(args.warmup):
+        log(f"warmup {i + 1}/{args.warmup}")
+        B.expmv_device(d_v, d_y)
+    mek.synchronize()
+    kern = {k: {"ms": 0.0, "launches": 0, "bytes": 0.0} for k in mek.BLOCK_KERNEL_KINDS}
+    log(f"timing {args.steps} block solves")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        B.expmv_device(d_v, d_y)
+        st = B.stats()
+        for k in kern:
+            for f in kern[k]:
+                kern[k][f] += st["kernels"][k][f]
+    mek.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1000.0 / args.steps
+    Y = d_y.to_array().reshape(V.shape)
+    B.close()
+    log("plain single-vector solves of 8 columns")
+    R = mek.Runner(A, m=40, t=t, tol=tol, max_restarts=0, step_control="expokit", gs_mode=2)
+    cols = [mek.DeviceBuffer.from_array(np.ascontiguousarray(V[:, c])) for c in range(8)]
+    dy1 = mek.DeviceBuffer(8 * n)
+    R.expmv_device(cols[0], dy1)
+    mek.synchronize()
+    t1 = time.perf_counter()
+    for c in cols:
+        R.expmv_device(c, dy1)
+    mek.synchronize()
+    single_ms = (time.perf_counter() - t1) * 1000.0 / len(cols)
+    single_cols = R.stats()["spmv_count"]
+    R.close()
+    log("closed-form check of the 32 columns")
+    errs = [float(np.linalg.norm(Y[:, c] - laplacian_2d_expmv(N, t, V[:, c]))
+                  / np.linalg.norm(laplacian_2d_expmv(N, t, V[:, c]))) for c in range(32)]
+
+    def roof(k):
+        kt = kern[k]
+        if kt["launches"] == 0 or kt["ms"] <= 0:
+            return None
+        achieved = kt["bytes"] / (kt["ms"] * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "block_" + k,
+                "bytes_per_launch": kt["bytes"] / kt["launches"],
+                "avg_launch_us": kt["ms"] * 1e3 / kt["launches"]}
+
+    dominant = max(("spmm", "dots", "update", "lift"), key=lambda k: kern[k]["ms"])
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("CPU baseline (oracle, one thread, bounded sample)")
+        cpu = cpu_baseline(rp, ci, va, V[:, 0].copy(), single_cols, 40, n, args.cpu_seconds)
+        cpu["unit"] = "rhs/s"  # one right-hand side per single-vector solve
+        cpu["sample"] += "; one right-hand side per solve"
+    rhs_per_s = 32 * 1000.0 / ms_per_step
+    out = {
+        "metric": "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64",
+        "value": round(rhs_per_s, 3),
+        "unit": "rhs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "block2048", "n": n, "nnz": int(rp[-1]), "rhs": 32, "t": t, "block_steps": m,
+                   "tol": tol, "gs": args.gs, "seed": 0},
+        "rel_err": max(errs),
+        "solve": {"substeps": st["accepted"], "attempts": st["attempts"], "block_steps": st["block_steps"],
+                  "second_passes": st["second_passes"], "squarings": st["squarings"]},
+        "plain_single_vector": {"ms_per_rhs": round(single_ms, 3), "rhs_per_s": round(1000.0 / single_ms, 3),
+                                "krylov_columns": single_cols},
+        "speedup_vs_single_vector": round(rhs_per_s / (1000.0 / single_ms), 3),
+        "roofline": roof(dominant),
+        "spmm": roof("spmm"),
+        "kernels_ms_per_solve": {k: round(kern[k]["ms"] / args.steps, 3) for k in kern if kern[k]["launches"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +338,7 @@ def main():
                     help="second GS pass: dgks = only when ||w'|| < 0.1 ||w|| (default), always = reference MGS2 semantics")
     ap.add_argument("--fuse", action="store_true", help="first-pass dots inside the SpMV kernel")
     ap.add_argument("--cpu-seconds", type=float, default=25.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this solver runs on one GPU; multi-GPU runs are not part of this build "
@@ -244,6 +349,9 @@ def main():
     gen = _load("mek_generate", os.path.join(ROOT, "matrix-exp-krylov_amd", "generate.py"))
     if args.workload == "batch256":
         run_batch(args, mek, gen)
+        return
+    if args.workload == "block2048":
+        run_block(args, mek, gen)
         return
     w = WORKLOADS[args.workload]
     log(f"building {args.workload}")

@@ -207,6 +207,49 @@ void mek_spmv_plan_destroy(mek_spmv_plan* P);
  * A.exp() of main.cu:33-36.  Synchronous. */
 int mek_expm_dense(int k, const double* d_A, double* d_E);
 
+/* ------------------------------------------------------- block of RHS */
+
+/* Y = exp(t A) V for a block of 32 right-hand sides on one shared A (V, Y:
+ * n x 32, row-major) by block Arnoldi: SpMM, block classical Gram-Schmidt
+ * with a second pass on FP64 MFMA, Cholesky QR inside each block, and a
+ * Taylor / scaling-and-squaring exponential of the block Hessenberg matrix
+ * on an FP64 MFMA GEMM.  No reference counterpart: the reference solves one
+ * vector per process (main.cu:1411-1571).  m: block steps (basis of 32 m
+ * vectors, 32 (m + 1) < n, m <= 48); substeps > 0 fixes the first sub-step
+ * to t/substeps, 0 uses Expokit's first-step formula; each sub-step passes
+ * ||H(k,k-1) E_k R0||_F <= 1.2 (dt/t) tol min(||V||_F, ||Y||_F).  gs_mode:
+ * 1 second block Gram-Schmidt pass always, 2 only when a column of the new
+ * block keeps less than 0.1 of its norm in the first pass, 0 never. */
+typedef struct mek_block mek_block;
+typedef struct {
+    int converged;
+    int accepted;
+    int attempts;
+    int rejected;
+    int block_steps;
+    int squarings;
+    int second_passes;  /* block steps that took the second Gram-Schmidt pass */
+    double t_done;
+    double last_residual;
+    double solve_ms;
+    double kernel_ms[5];     /* spmm, dots, update, expm, lift (timing on) */
+    double kernel_bytes[5];  /* algorithmic HBM bytes of the same          */
+    long long launches[5];
+} mek_block_stats;
+int mek_block_create(const mek_csr* A, int m, double t, double tol, int substeps, int gs_mode,
+                     int device_id, int timing, mek_block** out);
+int mek_block_expmv(mek_block* B, const double* V, double* Y);
+int mek_block_expmv_device(mek_block* B, const double* d_V, double* d_Y);
+int mek_block_stats_get(const mek_block* B, mek_block_stats* out);
+void mek_block_destroy(mek_block* B);
+
+/* Y = A X for 32 columns (X, Y: n x 32 row-major host arrays), each column
+ * summed in CSR storage order as a sequential loop does. */
+int mek_spmm(const mek_csr* A, const double* X, double* Y, int device_id);
+/* C = A B for N x N column-major host matrices (N a multiple of 64) on the
+ * FP64 MFMA GEMM of the block exponential. */
+int mek_gemm_dense(int N, const double* A, const double* B, double* C, int device_id);
+
 #ifdef __cplusplus
 }
 #endif

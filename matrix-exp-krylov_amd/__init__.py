@@ -126,6 +126,14 @@ SIGNATURES = {
     "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mek_spmv_plan_destroy": (None, [_vp]),
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
+    "mek_block_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mek_block_expmv": (ctypes.c_int, [_vp, _dp, _dp]),
+    "mek_block_expmv_device": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mek_block_stats_get": (ctypes.c_int, [_vp, ctypes.c_void_p]),
+    "mek_block_destroy": (None, [_vp]),
+    "mek_spmm": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int]),
+    "mek_gemm_dense": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
 }
 
 
@@ -514,6 +522,80 @@ class SpmvPlan:
             self.close()
         except Exception:
             pass
+
+
+class BlockStats(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("accepted", ctypes.c_int), ("attempts", ctypes.c_int),
+                ("rejected", ctypes.c_int), ("block_steps", ctypes.c_int), ("squarings", ctypes.c_int),
+                ("second_passes", ctypes.c_int),
+                ("t_done", ctypes.c_double), ("last_residual", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double * 5), ("kernel_bytes", ctypes.c_double * 5),
+                ("launches", ctypes.c_longlong * 5)]
+
+
+BLOCK_KERNEL_KINDS = ("spmm", "dots", "update", "expm", "lift")
+RHS = 32
+
+
+class Block:
+    """Y = exp(t A) V for a block of 32 right-hand sides (block Arnoldi)."""
+
+    def __init__(self, A: CSR, m: int, t: float, tol: float = 1e-8, substeps: int = 0, gs_mode: int = 2,
+                 device: int = 0, timing: bool = False):
+        h = _vp()
+        _check(lib().mek_block_create(A.h, int(m), float(t), float(tol), int(substeps), int(gs_mode),
+                                      int(device), 1 if timing else 0, ctypes.byref(h)))
+        self.h = h
+        self.n = A.info()["rows"]
+
+    def expmv(self, V):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        if V.shape != (self.n, RHS):
+            raise ValueError(f"V must be {self.n} x {RHS}")
+        Y = np.zeros_like(V)
+        _check(lib().mek_block_expmv(self.h, _pd(V.ravel()), _pd(Y.ravel())))
+        return Y
+
+    def expmv_device(self, d_V: DeviceBuffer, d_Y: DeviceBuffer):
+        _check(lib().mek_block_expmv_device(self.h, d_V.ptr, d_Y.ptr))
+
+    def stats(self):
+        s = BlockStats()
+        _check(lib().mek_block_stats_get(self.h, ctypes.byref(s)))
+        out = {f: getattr(s, f) for f, _ in BlockStats._fields_ if f not in ("kernel_ms", "kernel_bytes", "launches")}
+        out["kernels"] = {k: {"ms": s.kernel_ms[i], "bytes": s.kernel_bytes[i], "launches": s.launches[i]}
+                          for i, k in enumerate(BLOCK_KERNEL_KINDS)}
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mek_block_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spmm(A: CSR, X, device: int = 0):
+    """A X for a block of 32 columns (n x 32 row-major) on the device."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.zeros_like(X)
+    _check(lib().mek_spmm(A.h, _pd(X.ravel()), _pd(Y.ravel()), int(device)))
+    return Y
+
+
+def gemm_dense(A, B, device: int = 0):
+    """A B (N x N, N a multiple of 64) on the device FP64 MFMA GEMM."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    N = A.shape[0]
+    C = np.zeros((N, N), np.float64, order="F")
+    _check(lib().mek_gemm_dense(N, _pd(A.ravel(order="F")), _pd(B.ravel(order="F")),
+                                C.ctypes.data_as(_dp), int(device)))
+    return C
 
 
 def expm_dense(A):

@@ -14,6 +14,8 @@
 #include "kernels/launch.hpp"
 #include "mek/arnoldi_runner.hpp"
 #include "mek/batch_runner.hpp"
+#include "mek/block_runner.hpp"
+#include "kernels/block.hpp"
 #include "mek/csr_host.hpp"
 #include "mek/mlp.hpp"
 
@@ -27,6 +29,10 @@ struct mek_runner {
 
 struct mek_batch {
   std::unique_ptr<mek::BatchRunner> B;
+};
+
+struct mek_block {
+  std::unique_ptr<mek::BlockRunner> B;
 };
 
 struct mek_mlp {
@@ -534,6 +540,132 @@ int mek_expm_dense(int k, const double* d_A, double* d_E) {
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(d_scr);
     if (d_ws) (void)hipFree(d_ws);
+    MEK_HIP(e);
+  });
+}
+
+// ------------------------------------------------------- block of RHS
+
+int mek_block_create(const mek_csr* A, int m, double t, double tol, int substeps, int gs_mode,
+                     int device_id, int timing, mek_block** out) {
+  return guarded([&] {
+    need(A && out, "matrix, out");
+    need(substeps >= 0, "substeps >= 0");
+    need(gs_mode >= 0 && gs_mode <= 2, "gs_mode");
+    auto h = std::make_unique<mek_block>();
+    h->B = std::make_unique<mek::BlockRunner>(A->A, m, t, tol, substeps, device_id, timing != 0, gs_mode);
+    *out = h.release();
+  });
+}
+
+int mek_block_expmv(mek_block* B, const double* V, double* Y) {
+  return guarded([&] {
+    need(B && V && Y, "block, V, Y");
+    B->B->expmv(V, Y);
+  });
+}
+
+int mek_block_expmv_device(mek_block* B, const double* d_V, double* d_Y) {
+  return guarded([&] {
+    need(B && d_V && d_Y, "block, V, Y");
+    B->B->expmv_device(d_V, d_Y);
+  });
+}
+
+int mek_block_stats_get(const mek_block* B, mek_block_stats* o) {
+  return guarded([&] {
+    need(B && o, "block, out");
+    const mek::BlockStats& s = B->B->stats();
+    std::memset(o, 0, sizeof(*o));
+    o->converged = s.converged;
+    o->accepted = s.accepted;
+    o->attempts = s.attempts;
+    o->rejected = s.rejected;
+    o->block_steps = s.block_steps;
+    o->squarings = s.squarings;
+    o->second_passes = s.second_passes;
+    o->t_done = s.t_done;
+    o->last_residual = s.last_residual;
+    o->solve_ms = s.solve_ms;
+    for (int k = 0; k < 5; ++k) {
+      o->kernel_ms[k] = s.kernel_ms[k];
+      o->kernel_bytes[k] = s.kernel_bytes[k];
+      o->launches[k] = s.launches[k];
+    }
+  });
+}
+
+void mek_block_destroy(mek_block* B) { delete B; }
+
+int mek_spmm(const mek_csr* A, const double* X, double* Y, int device_id) {
+  return guarded([&] {
+    need(A && X && Y, "matrix, X, Y");
+    const CSRHost& M = A->A;
+    need(M.rows == M.cols, "square matrix");
+    MEK_HIP(hipSetDevice(device_id));
+    const long long n_pad = (static_cast<long long>(M.rows) + mek::kBlkRows - 1) / mek::kBlkRows * mek::kBlkRows;
+    const size_t blk = sizeof(double) * static_cast<size_t>(n_pad) * mek::kRhs;
+    const size_t used = sizeof(double) * static_cast<size_t>(M.rows) * mek::kRhs;
+    std::vector<void*> mem;
+    auto alloc = [&](size_t bytes) {
+      void* p = nullptr;
+      MEK_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+      mem.push_back(p);
+      return p;
+    };
+    try {
+      mek::BlockBufs b{};
+      b.n = M.rows;
+      b.n_pad = n_pad;
+      auto* rp = static_cast<int*>(alloc(sizeof(int) * (M.rows + 1)));
+      auto* ci = static_cast<int*>(alloc(sizeof(int) * M.nnz));
+      auto* va = static_cast<double*>(alloc(sizeof(double) * M.nnz));
+      auto* dx = static_cast<double*>(alloc(blk));
+      auto* dy = static_cast<double*>(alloc(blk));
+      b.flags = static_cast<int*>(alloc(sizeof(int) * 4));
+      MEK_HIP(hipMemset(b.flags, 0, sizeof(int) * 4));
+      MEK_HIP(hipMemset(dx, 0, blk));
+      MEK_HIP(hipMemcpy(rp, M.row_ptr.data(), sizeof(int) * (M.rows + 1), hipMemcpyHostToDevice));
+      if (M.nnz > 0) {
+        MEK_HIP(hipMemcpy(ci, M.col_idx.data(), sizeof(int) * M.nnz, hipMemcpyHostToDevice));
+        MEK_HIP(hipMemcpy(va, M.values.data(), sizeof(double) * M.nnz, hipMemcpyHostToDevice));
+      }
+      MEK_HIP(hipMemcpy(dx, X, used, hipMemcpyHostToDevice));
+      b.row_ptr = rp;
+      b.col_idx = ci;
+      b.vals = va;
+      mek::block_launch_spmm(b, dx, dy, nullptr);
+      MEK_HIP_LAUNCHED();
+      MEK_HIP(hipDeviceSynchronize());
+      MEK_HIP(hipMemcpy(Y, dy, used, hipMemcpyDeviceToHost));
+    } catch (...) {
+      for (void* p : mem) (void)hipFree(p);
+      throw;
+    }
+    for (void* p : mem) (void)hipFree(p);
+  });
+}
+
+int mek_gemm_dense(int N, const double* A, const double* B, double* C, int device_id) {
+  return guarded([&] {
+    need(N >= mek::kGemmTile && N % mek::kGemmTile == 0 && A && B && C, "N a positive multiple of 64, A, B, C");
+    MEK_HIP(hipSetDevice(device_id));
+    const size_t bytes = sizeof(double) * static_cast<size_t>(N) * N;
+    double *da = nullptr, *db = nullptr, *dc = nullptr;
+    MEK_HIP(hipMalloc(&da, bytes));
+    hipError_t e = hipMalloc(&db, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dc, bytes);
+    if (e == hipSuccess) e = hipMemcpy(da, A, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, B, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      mek::dense_gemm(N, da, N, db, N, 1.0, nullptr, N, 0.0, dc, N, nullptr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(C, dc, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dc) (void)hipFree(dc);
     MEK_HIP(e);
   });
 }
