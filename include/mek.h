@@ -88,6 +88,7 @@ typedef struct {
     int verbose;        /* print the reference's per-column / [STEP] / METRIC */
     int timing;         /* HIP-event timing of every kernel launch            */
     int fuse_spmv_dot;  /* first-pass dot products inside the SpMV kernel     */
+    int lanczos;        /* symmetric A: Lanczos three-term recurrence        */
 } mek_solver_options;
 
 void mek_solver_options_default(mek_solver_options* opt);

@@ -66,6 +66,9 @@ struct SolverOptions {
   bool verbose = true;      // print the reference's per-column and [STEP] lines
   bool timing = false;      // HIP-event timing of every kernel launch
   bool fuse_spmv_dot = false;  // first-pass dots inside the SpMV kernel
+  // Symmetric A only (checked at init): the Lanczos three-term recurrence
+  // instead of Gram-Schmidt; H is tridiagonal.
+  bool lanczos = false;
 };
 
 struct StepRecord {
@@ -139,6 +142,7 @@ struct ArnoldiRunner {
  private:
   void solve_on_device(const double* d_v_or_null, const double* h_v_or_null);
   void run_columns(int m_cols);
+  void run_lanczos_columns(int m_cols);
   int read_status();
   void begin_timed(int kind);
   void end_timed(int kind, double bytes);

@@ -41,7 +41,7 @@ class SolverOptions(ctypes.Structure):
     _fields_ = [("step_control", ctypes.c_int), ("gs_mode", ctypes.c_int),
                 ("dgks_eta", ctypes.c_double), ("values_f32", ctypes.c_int),
                 ("verbose", ctypes.c_int), ("timing", ctypes.c_int),
-                ("fuse_spmv_dot", ctypes.c_int)]
+                ("fuse_spmv_dot", ctypes.c_int), ("lanczos", ctypes.c_int)]
 
 
 class ArnoldiParams(ctypes.Structure):
@@ -281,7 +281,7 @@ class Runner:
 
     def __init__(self, A: CSR, m=30, t=1.0, tol=1e-8, max_restarts=10, reorthogonalize=True,
                  step_control="reference", gs_mode=-1, dgks_eta=0.1, values_f32=False,
-                 verbose=False, timing=False, fuse_spmv_dot=False, device=0):
+                 verbose=False, timing=False, fuse_spmv_dot=False, lanczos=False, device=0):
         p = ArnoldiParams(int(m), float(t), float(tol), int(max_restarts), 1 if reorthogonalize else 0)
         o = SolverOptions()
         lib().mek_solver_options_default(ctypes.byref(o))
@@ -292,6 +292,7 @@ class Runner:
         o.verbose = 1 if verbose else 0
         o.timing = 1 if timing else 0
         o.fuse_spmv_dot = 1 if fuse_spmv_dot else 0
+        o.lanczos = 1 if lanczos else 0
         h = _vp()
         _check(lib().mek_runner_create(A.h, ctypes.byref(p), ctypes.byref(o), int(device), ctypes.byref(h)))
         self.h = h

@@ -182,6 +182,7 @@ void mek_solver_options_default(mek_solver_options* o) {
   o->verbose = d.verbose;
   o->timing = d.timing;
   o->fuse_spmv_dot = d.fuse_spmv_dot;
+  o->lanczos = d.lanczos;
 }
 
 int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_solver_options* o,
@@ -203,6 +204,7 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       so.verbose = o->verbose != 0;
       so.timing = o->timing != 0;
       so.fuse_spmv_dot = o->fuse_spmv_dot != 0;
+      so.lanczos = o->lanczos != 0;
     }
     std::vector<CSRHost> parts{A->A};
     std::vector<int> ids{device_id};

@@ -9,9 +9,10 @@
 //   --tol X              tolerance (reference CLI: fixed 1e-6)
 //   --no-t-clamp         use t as given
 //   --v-file PATH        start vector, one value per line (default 0.1)
-//   --step-control reference|expokit
+//   --step-control reference|expokit|expokit-rel
 //   --gs always|never|dgks
 //   --f32-values         store matrix values in FP32 on the device
+//   --lanczos            symmetric A: Lanczos three-term recurrence instead of Gram-Schmidt
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,6 +71,7 @@ int main(int argc, char* argv[]) {
     if (a == "--no-t-clamp") { clamp_t = false; continue; }
     if (a == "--v-file" && has_next) { v_file = args[++i]; continue; }
     if (a == "--f32-values") { opts.values_f32 = true; continue; }
+    if (a == "--lanczos") { opts.lanczos = true; continue; }
     if (a == "--step-control" && has_next) {
       const std::string v = args[++i];
       opts.control = (v == "expokit")       ? mek::StepControl::Expokit

@@ -37,6 +37,37 @@ double now_ms() {
 }
 
 // Round up to two significant digits, as Expokit does for step sizes.
+// A == A^T (structure and values), by a counting-sort transpose compared
+// row by row against each row of A sorted by column.
+bool csr_is_symmetric(const CSRHost& A) {
+  if (A.rows != A.cols) return false;
+  const int n = A.rows;
+  const long long nnz = A.nnz;
+  std::vector<long long> tp(static_cast<size_t>(n) + 1, 0);
+  for (long long k = 0; k < nnz; ++k) tp[A.col_idx[k] + 1]++;
+  for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+  std::vector<long long> pos(tp.begin(), tp.end() - 1);
+  std::vector<int> tc(static_cast<size_t>(nnz));
+  std::vector<double> tv(static_cast<size_t>(nnz));
+  for (int i = 0; i < n; ++i)
+    for (int k = A.row_ptr[i]; k < A.row_ptr[i + 1]; ++k) {
+      const long long p = pos[A.col_idx[k]]++;
+      tc[p] = i;
+      tv[p] = A.values[k];
+    }
+  std::vector<std::pair<int, double>> row;
+  for (int i = 0; i < n; ++i) {
+    const int k0 = A.row_ptr[i], k1 = A.row_ptr[i + 1];
+    if (tp[i + 1] - tp[i] != k1 - k0) return false;
+    row.clear();
+    for (int k = k0; k < k1; ++k) row.emplace_back(A.col_idx[k], A.values[k]);
+    std::sort(row.begin(), row.end());
+    for (int q = 0; q < k1 - k0; ++q)
+      if (row[q].first != tc[tp[i] + q] || row[q].second != tv[tp[i] + q]) return false;
+  }
+  return true;
+}
+
 double round_step(double x) {
   const double s = std::pow(10.0, std::floor(std::log10(x)) - 1.0);
   return std::ceil(x / s) * s;
@@ -76,6 +107,8 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
   if (num_gpus != 1)
     throw mek::Error("ERROR: this build runs the solve on one GPU (num_gpus=" +
                      std::to_string(num_gpus) + ")");
+  if (options.lanczos && !csr_is_symmetric(partitions[0]))
+    throw mek::Error("ERROR: the Lanczos path needs a symmetric matrix");
   if (params.m >= global_rows)
     throw mek::Error("ERROR: Requested Arnoldi m=" + std::to_string(params.m) +
                      " must be less than global_rows=" + std::to_string(global_rows));
@@ -161,6 +194,10 @@ void ArnoldiRunner::init_q1_device(const double* d_v, const double* h_v) {
 }
 
 void ArnoldiRunner::run_columns(int m_cols) {
+  if (options.lanczos) {
+    run_lanczos_columns(m_cols);
+    return;
+  }
   Range r("orthogonalize_mgs");
   DeviceContext& dc = device_contexts[0];
   const mek::ArnoldiBufs b = dc.bufs();
@@ -213,6 +250,30 @@ void ArnoldiRunner::run_columns(int m_cols) {
     MEK_HIP_LAUNCHED();
   }
   stats.spmv_count += m_cols;  // launched; columns after a breakdown exit at once
+}
+
+// Symmetric Lanczos: per column one SpMV (with the fused alpha partial), its
+// finaliser, one three-term update sweep and its finaliser.
+void ArnoldiRunner::run_lanczos_columns(int m_cols) {
+  Range r("lanczos");
+  DeviceContext& dc = device_contexts[0];
+  const mek::ArnoldiBufs b = dc.bufs();
+  hipStream_t s = dc.stream_compute;
+  const double n = dc.local_rows;
+  const double bspmv = static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
+                       4.0 * (n + 1) + 16.0 * n;
+  for (int j = 0; j < m_cols; ++j) {
+    begin_timed(mek::kKSpmv);
+    mek::launch_spmv_lanczos(b, j, s);
+    end_timed(mek::kKSpmv, bspmv + 8.0 * n);  // + q_j for the fused dot
+    mek::launch_fin_h1(b, 1, b.spmv_grid, s);
+    begin_timed(mek::kKUpdateDot);
+    mek::launch_lanczos_update(b, j, s);
+    end_timed(mek::kKUpdateDot, 8.0 * n * (j > 0 ? 4 : 3));
+    mek::launch_lanczos_fin(b, j, s);
+    MEK_HIP_LAUNCHED();
+  }
+  stats.spmv_count += m_cols;
 }
 
 void ArnoldiRunner::launch_expm(double dt, int mode) {

@@ -126,7 +126,8 @@ __device__ __forceinline__ double sweep_combination(const double* __restrict__ V
 template <typename VT, int MAXJ>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double* x_in,
                                                      const double* xs_ptr, double* wout,
-                                                     int ndots, int normq, int check_breakdown) {
+                                                     int ndots, int normq, int check_breakdown,
+                                                     int i0) {
   if (check_breakdown && b.iscal[kBreakdown]) return;
   __shared__ double prod[kNnzCap];
   __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
@@ -135,8 +136,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
   const int grid = gridDim.x;
   const double* __restrict__ x = x_in;
   const double xs = xs_ptr ? *xs_ptr : 1.0;
-  // fully unrolled column loops below: the coefficients stay in scalar registers
-  const double* __restrict__ cs = b.colscale;
+  // fully unrolled column loops below: the coefficients stay in scalar registers;
+  // the dots are against columns i0, i0 + 1, ...
+  const double* __restrict__ cs = b.colscale + i0;
+  const double* __restrict__ Vd = b.V + static_cast<long long>(i0) * b.ld;
   const int* __restrict__ rp = b.row_ptr;
   const int* __restrict__ ci = b.col_idx;
   double* __restrict__ w = wout;
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
         acc[MAXJ] += a * a;
 #pragma unroll
         for (int i = 0; i < MAXJ; ++i)
-          if (i < ndots) acc[i] += (cs[i] * b.V[static_cast<long long>(i) * b.ld + r]) * a;
+          if (i < ndots) acc[i] += (cs[i] * Vd[static_cast<long long>(i) * b.ld + r]) * a;
       }
       __syncthreads();
     } else {
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
         acc[MAXJ] += t * t;
 #pragma unroll
         for (int i = 0; i < MAXJ; ++i)
-          if (i < ndots) acc[i] += (cs[i] * b.V[static_cast<long long>(i) * b.ld + r0]) * t;
+          if (i < ndots) acc[i] += (cs[i] * Vd[static_cast<long long>(i) * b.ld + r0]) * t;
       }
       __syncthreads();
     }
@@ -443,17 +446,70 @@ __global__ __launch_bounds__(kBlock) void k_restart(ArnoldiBufs b) {
   }
 }
 
+// --------------------------------------------------------------- Lanczos
+//
+// Symmetric A: the three-term recurrence replaces Gram-Schmidt.  Column j is
+// the SpMV with the fused partial alpha_j = q_j . A q_j (launch_spmv_lanczos,
+// finalised by k_fin_h1 into h1[0]), then one sweep
+//   w' = A q_j - alpha_j q_j - beta_{j-1} q_{j-1}  ->  V[:, j+1] (unnormalised)
+// with the partial ||w'||^2, and the finaliser below.  H stays tridiagonal
+// and symmetric; the exponential, lift and step control are shared with the
+// Arnoldi path.
+
+__global__ __launch_bounds__(kBlock) void k_lanczos_update(ArnoldiBufs b, int j) {
+  if (b.iscal[kBreakdown]) return;
+  __shared__ double red[kWavesPerBlock];
+  const double alpha = b.h1[0];
+  const double ca = alpha * b.colscale[j];
+  const double cb = (j > 0) ? b.H[static_cast<long long>(j - 1) * b.ldh + j] * b.colscale[j - 1] : 0.0;
+  const double* __restrict__ qj = b.V + static_cast<long long>(j) * b.ld;
+  const double* __restrict__ qm = b.V + static_cast<long long>(j > 0 ? j - 1 : 0) * b.ld;
+  const double* __restrict__ w = b.w;
+  double* __restrict__ out = b.V + static_cast<long long>(j + 1) * b.ld;
+  double acc = 0.0;
+  for (long long r = gtid(); r < b.n; r += gstride()) {
+    const double a = (w[r] - ca * qj[r]) - cb * qm[r];
+    out[r] = a;
+    acc += a * a;
+  }
+  double o[1] = {acc};
+  block_partials<1>(o, 1, red, b.partials, gridDim.x, blockIdx.x);
+}
+
+// H(j, j) = alpha_j, H(j+1, j) = beta_j = ||w'|| (with the Arnoldi path's
+// breakdown test), H(j-1, j) = beta_{j-1}; colscale[j+1] = 1 / beta_j.
+__global__ __launch_bounds__(kBlock) void k_lanczos_fin(ArnoldiBufs b, int j, int pgrid) {
+  if (b.iscal[kBreakdown]) return;
+  __shared__ double red[kWavesPerBlock];
+  const double norm2 = block_sum_quantity(b.partials, pgrid, 0, red);
+  if (threadIdx.x == 0) {
+    const int ldh = b.ldh;
+    double* Hc = b.H + static_cast<long long>(j) * ldh;
+    for (int i = 0; i < ldh; ++i) Hc[i] = 0.0;
+    if (j > 0) Hc[j - 1] = b.H[static_cast<long long>(j - 1) * ldh + j];
+    Hc[j] = b.h1[0];
+    const double nrm = sqrt(norm2);
+    b.iscal[kKUsed] = j + 1;
+    if (nrm < 1e-14 || isnan(nrm) || isinf(nrm)) {
+      b.iscal[kBreakdown] = 1;
+    } else {
+      Hc[j + 1] = nrm;
+      b.colscale[j + 1] = 1.0 / nrm;
+    }
+  }
+}
+
 // ------------------------------------------------------------ launchers
 
 template <int MAXJ>
 static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const double* xs, double* w,
-                              int ndots, int normq, int chk, hipStream_t s) {
+                              int ndots, int normq, int chk, hipStream_t s, int i0 = 0) {
   if (b.vals_f32)
     hipLaunchKernelGGL((k_spmv_dot<float, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
-                       ndots, normq, chk);
+                       ndots, normq, chk, i0);
   else
     hipLaunchKernelGGL((k_spmv_dot<double, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
-                       ndots, normq, chk);
+                       ndots, normq, chk, i0);
 }
 
 // Arnoldi column j: w = A q_j with dots against q_0..q_{ndots-1}; the norm
@@ -467,6 +523,18 @@ void launch_spmv_dot(const ArnoldiBufs& b, int j, int ndots, int normq, hipStrea
   else if (ndots <= 32) launch_spmv_dot_t<32>(b, x, xs, b.w, ndots, normq, 1, s);
   else if (ndots <= 48) launch_spmv_dot_t<48>(b, x, xs, b.w, ndots, normq, 1, s);
   else launch_spmv_dot_t<64>(b, x, xs, b.w, ndots < 64 ? ndots : 64, normq, 1, s);
+}
+
+// Lanczos column j: w = A q_j with the partial q_j . w at quantity 0 and
+// ||w||^2 at quantity 1 (spmv_grid blocks).
+void launch_spmv_lanczos(const ArnoldiBufs& b, int j, hipStream_t s) {
+  launch_spmv_dot_t<1>(b, b.V + static_cast<long long>(j) * b.ld, b.colscale + j, b.w, 1, 1, 1, s, j);
+}
+void launch_lanczos_update(const ArnoldiBufs& b, int j, hipStream_t s) {
+  hipLaunchKernelGGL(k_lanczos_update, dim3(b.grid), dim3(kBlock), 0, s, b, j);
+}
+void launch_lanczos_fin(const ArnoldiBufs& b, int j, hipStream_t s) {
+  hipLaunchKernelGGL(k_lanczos_fin, dim3(1), dim3(kBlock), 0, s, b, j, b.grid);
 }
 
 // Plain y = A*(xs*x) (xs may be null) with the ||y||^2 partial at quantity 0.

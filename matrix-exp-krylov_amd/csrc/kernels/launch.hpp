@@ -29,6 +29,12 @@ void launch_fin_start(const ArnoldiBufs& b, hipStream_t s);
 void launch_lift(const ArnoldiBufs& b, int ncoef, double skip_below, hipStream_t s);
 void launch_fin_norm(const ArnoldiBufs& b, int slot, int pgrid, hipStream_t s);
 void launch_restart(const ArnoldiBufs& b, hipStream_t s);
+// Symmetric Lanczos column j (three-term recurrence): SpMV with the fused
+// alpha partial (finalise with launch_fin_h1(b, 1, b.spmv_grid)), the update
+// sweep into V[:, j+1] and its finaliser (H entries, breakdown, colscale).
+void launch_spmv_lanczos(const ArnoldiBufs& b, int j, hipStream_t s);
+void launch_lanczos_update(const ArnoldiBufs& b, int j, hipStream_t s);
+void launch_lanczos_fin(const ArnoldiBufs& b, int j, hipStream_t s);
 void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, double* y,
                        hipStream_t s);
 

@@ -144,3 +144,28 @@ def test_cli_builtin_matrix(gpu, tmp_path, oracle):
     yref, st, _ = oracle.expmv(rp, np.arange(500, dtype=np.int32), d, np.full(500, 0.1), 1.0, 1e-6, 30, 10)
     assert rel_err(y, yref) <= TOL
     assert ("converged=yes" in p.stdout) == bool(st["converged"])
+
+
+@pytest.mark.parametrize("control", ["expokit", "reference"])
+def test_lanczos_laplacian_dst(gpu, control):
+    """Symmetric Lanczos path on the 2-D Laplacian against the DST-I closed
+    form, and against the Arnoldi path on the same input."""
+    N = 96
+    rp, ci, va = gen.laplacian_2d(N)
+    v = gen.normal_vector(N * N, seed=3)
+    t = 5.0
+    ref = laplacian_2d_expmv(N, t, v)
+    ys = {}
+    for lz in (True, False):
+        R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=30, t=t, tol=1e-10, max_restarts=10,
+                       step_control=control, gs_mode=2, lanczos=lz)
+        ys[lz] = R.expmv(v)
+        R.close()
+    assert rel_err(ys[True], ref) <= 1e-8, rel_err(ys[True], ref)
+    assert rel_err(ys[True], ys[False]) <= 1e-8
+
+
+def test_lanczos_rejects_nonsymmetric(gpu):
+    rp, ci, va = gen.convdiff_3d(8)
+    with pytest.raises(mek.MekError, match="symmetric"):
+        mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=10, t=0.1, tol=1e-8, lanczos=True)
