@@ -123,27 +123,33 @@ __global__ __launch_bounds__(kBlock) void k_bdots(BlockBufs b, const double* __r
     red[wave][(16 + m) * kRhs + 16 + lc] = c11[q];
   }
   __syncthreads();
-  double* out = b.partials + (static_cast<long long>(i) * b.nchunks + g) * (kRhs * kRhs);
+  // partials laid out [block i][element e][chunk g]: the finaliser reads each
+  // element's chunks contiguously
+  double* out = b.partials + static_cast<long long>(i) * (kRhs * kRhs) * b.nchunks + g;
   for (int e = threadIdx.x; e < kRhs * kRhs; e += kBlock)
-    out[e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+    out[static_cast<long long>(e) * b.nchunks] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
 }
 
-// C[i][e] = sum over chunks g (in order) of the partials.
+// C[i][e] = the sum of the chunk partials: one wave per element, lane l
+// summing chunks l, l + 64, ... in order, then the fixed-order wave sum.
 __global__ __launch_bounds__(kBlock) void k_bdots_fin(BlockBufs b, int cond) {
   if (b.flags[0] || (cond && !b.flags[2])) return;
-  const int i = blockIdx.x / 4;
-  const int e = (blockIdx.x % 4) * kBlock + threadIdx.x;
-  const double* p = b.partials + static_cast<long long>(i) * b.nchunks * (kRhs * kRhs) + e;
+  constexpr int kPerBlk = kBlock / kWave;  // elements per workgroup
+  const int i = blockIdx.x / (kRhs * kRhs / kPerBlk);
+  const int e = (blockIdx.x % (kRhs * kRhs / kPerBlk)) * kPerBlk + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const double* p = b.partials + (static_cast<long long>(i) * (kRhs * kRhs) + e) * b.nchunks;
   double s = 0.0;
-  for (int g = 0; g < b.nchunks; ++g) s += p[static_cast<long long>(g) * (kRhs * kRhs)];
-  b.C[static_cast<long long>(i) * kRhs * kRhs + e] = s;
+  for (int g = lane; g < b.nchunks; g += kWave) s += p[g];
+  s = wave_sum(s);
+  if (lane == 0) b.C[static_cast<long long>(i) * kRhs * kRhs + e] = s;
 }
 
 void block_launch_dots(const BlockBufs& b, const double* X1, int nb, const double* X2, int gram, int cond,
                        hipStream_t s) {
   const int ntot = nb + (gram ? 1 : 0);
   hipLaunchKernelGGL(k_bdots, dim3(ntot * b.nchunks), dim3(kBlock), 0, s, b, X1, nb, X2, ntot, cond);
-  hipLaunchKernelGGL(k_bdots_fin, dim3(ntot * 4), dim3(kBlock), 0, s, b, cond);
+  hipLaunchKernelGGL(k_bdots_fin, dim3(ntot * (kRhs * kRhs / (kBlock / kWave))), dim3(kBlock), 0, s, b, cond);
 }
 
 // -------------------------------------------------------- block update
@@ -260,12 +266,25 @@ __global__ __launch_bounds__(kBlock) void k_bgram(BlockBufs b, int j, int gs_mod
   __syncthreads();
   const int rows = kRhs * (j + 1);
   const double* Hc = b.Hb + static_cast<long long>(kRhs * j) * b.ldh;
-  for (int e = threadIdx.x; e < kRhs * kRhs; e += kBlock) {
-    const int a = e / kRhs, c = e % kRhs;
-    double s = 0.0;
-    for (int r = 0; r < rows; ++r)
-      s += Hc[static_cast<long long>(a) * b.ldh + r] * Hc[static_cast<long long>(c) * b.ldh + r];
-    Gp[e] = Gz[e] - s;
+  double acc[kRhs * kRhs / kBlock] = {};
+  for (int r0 = 0; r0 < rows; r0 += kRhs) {  // 32 rows of block column j at a time
+    for (int e = threadIdx.x; e < kRhs * kRhs; e += kBlock) {
+      const int r = e % kRhs, c = e / kRhs;
+      Gp[r * kRhs + c] = Hc[static_cast<long long>(c) * b.ldh + r0 + r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kRhs * kRhs / kBlock; ++u) {
+      const int e = threadIdx.x + u * kBlock;
+      const int a = e / kRhs, c = e % kRhs;
+      for (int r = 0; r < kRhs; ++r) acc[u] += Gp[r * kRhs + a] * Gp[r * kRhs + c];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kRhs * kRhs / kBlock; ++u) {
+    const int e = threadIdx.x + u * kBlock;
+    Gp[e] = Gz[e] - acc[u];
   }
   __syncthreads();
   if (threadIdx.x < kRhs) {
@@ -301,26 +320,30 @@ __global__ __launch_bounds__(kBlock) void k_bchol(BlockBufs b, int j) {
   load32(R, (j >= 0 && !b.flags[2]) ? b.Gm : b.C);
   if (j >= 0) load32(Sj, b.S + static_cast<long long>(j) * kRhs * kRhs);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    // upper Cholesky in place (row-major R[r][c], c >= r)
-    int bad = 0;
-    for (int k = 0; k < kRhs && !bad; ++k) {
-      double d = R[k * kRhs + k];
-      for (int p = 0; p < k; ++p) d -= R[p * kRhs + k] * R[p * kRhs + k];
-      if (!(d > 1e-28) || !isfinite(d)) {
-        bad = 1;
-        break;
-      }
-      const double rkk = sqrt(d);
-      R[k * kRhs + k] = rkk;
-      for (int c = k + 1; c < kRhs; ++c) {
-        double s = R[k * kRhs + c];
-        for (int p = 0; p < k; ++p) s -= R[p * kRhs + k] * R[p * kRhs + c];
-        R[k * kRhs + c] = s / rkk;
-      }
+  // upper Cholesky in place (row-major R[r][c], c >= r), row k at a time:
+  // every thread forms the pivot (same operations, same value), thread c > k
+  // its entry of row k
+  int bad = 0;
+  for (int k = 0; k < kRhs; ++k) {
+    double d = R[k * kRhs + k];
+    for (int p = 0; p < k; ++p) d -= R[p * kRhs + k] * R[p * kRhs + k];
+    if (!(d > 1e-28) || !isfinite(d)) {
+      bad = 1;
+      break;
     }
-    brk = bad;
+    const double rkk = sqrt(d);
+    double s = 0.0;
+    const int c = threadIdx.x;
+    if (c > k && c < kRhs) {
+      s = R[k * kRhs + c];
+      for (int p = 0; p < k; ++p) s -= R[p * kRhs + k] * R[p * kRhs + c];
+    }
+    __syncthreads();  // all pivots read before row k changes
+    if (c == k) R[k * kRhs + k] = rkk;
+    if (c > k && c < kRhs) R[k * kRhs + c] = s / rkk;
+    __syncthreads();
   }
+  if (threadIdx.x == 0) brk = bad;
   __syncthreads();
   if (brk) {
     if (threadIdx.x == 0) {

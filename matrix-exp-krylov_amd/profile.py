@@ -32,6 +32,12 @@ CAL_BYTES = 1 << 30  # calibration buffer size (bytes read = bytes written)
 
 # kernel symbol prefix -> kind reported by the runner's event timers
 KINDS = [
+    ("k_spmm", "block_spmm"),
+    ("k_bdots", "block_dots"),
+    ("k_bupdate", "block_update"),
+    ("k_dgemm", "block_expm_gemm"),
+    ("k_b", "block_other"),
+    ("k_d", "block_expm_other"),
     ("kb_spmv", "spmv"),
     ("kb_dots1", "dots"),
     ("kb_update_dot", "update_dot"),
@@ -96,6 +102,25 @@ def batch_workload(args, mek, gen):
     B.close()
 
 
+def block_workload(args, mek, gen):
+    """2048^2 Laplacian with 32 right-hand sides, 16 block steps, t = 10 (one
+    warm-up solve, one measured)."""
+    import numpy as np
+    N = 2048
+    rp, ci, va = gen.laplacian_2d(N)
+    V = np.random.default_rng(0).standard_normal((N * N, 32))
+    V /= np.linalg.norm(V, axis=0)
+    gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+    B = mek.Block(mek.CSR.from_arrays(rp, ci, va), m=16, t=10.0, tol=1e-8, gs_mode=gs_mode)
+    d_v = mek.DeviceBuffer.from_array(V.ravel())
+    d_y = mek.DeviceBuffer(8 * V.size)
+    for _ in range(2):
+        B.expmv_device(d_v, d_y)
+    mek.synchronize()
+    print("[profile] block2048 solves done", file=sys.stderr, flush=True)
+    B.close()
+
+
 def workload(args):
     import torch
     mek = _load("mek", "__init__.py")
@@ -109,6 +134,9 @@ def workload(args):
     torch.cuda.empty_cache()
     if args.workload == "batch256":
         batch_workload(args, mek, gen)
+        return
+    if args.workload == "block2048":
+        block_workload(args, mek, gen)
         return
     w = WORKLOADS[args.workload]
     if w["kind"] == "laplacian":
@@ -247,7 +275,7 @@ def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
     a = sub.add_parser("workload")
-    a.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS) + ["batch256"])
+    a.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS) + ["batch256", "block2048"])
     a.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"])
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
