@@ -337,6 +337,8 @@ def main():
     ap.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"],
                     help="second GS pass: dgks = only when ||w'|| < 0.1 ||w|| (default), always = reference MGS2 semantics")
     ap.add_argument("--fuse", action="store_true", help="first-pass dots inside the SpMV kernel")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shapes (default: time candidates when the runner is created)")
     ap.add_argument("--lanczos", action="store_true",
                     help="symmetric Lanczos recurrence instead of Gram-Schmidt (laplacian4096 only)")
     ap.add_argument("--cpu-seconds", type=float, default=25.0, help="CPU baseline budget (0: skip)")
@@ -369,7 +371,7 @@ def main():
     A = mek.CSR.from_arrays(rp, ci, va)
     R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
                    gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=True,
-                   fuse_spmv_dot=args.fuse, lanczos=args.lanczos)
+                   fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune)
     A.close()
     d_v = mek.DeviceBuffer.from_array(v)
     d_y = mek.DeviceBuffer(8 * n)
@@ -443,7 +445,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "t": w["t"], "m": w["m"],
                    "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
-                   "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "seed": 0},
+                   "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "autotune": args.autotune, "seed": 0},
         "rel_err": rel,
         "solve": {"substeps": stats["accepted"], "expm_attempts": stats["attempts"],
                   "krylov_columns": stats["spmv_count"], "extra_spmv": stats["extra_spmv"],
@@ -451,6 +453,7 @@ def main():
         "roofline": roofline,
         "spmv": roof("spmv"),
         "kernels_ms_per_solve": {k: round(kern[k]["ms"] / args.steps, 3) for k in kern if kern[k]["launches"]},
+        "launch_shapes": R.tuning() if args.autotune else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

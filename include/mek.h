@@ -89,6 +89,7 @@ typedef struct {
     int timing;         /* HIP-event timing of every kernel launch            */
     int fuse_spmv_dot;  /* first-pass dot products inside the SpMV kernel     */
     int lanczos;        /* symmetric A: Lanczos three-term recurrence        */
+    int autotune;       /* time SpMV / sweep launch shapes at create, keep best */
 } mek_solver_options;
 
 void mek_solver_options_default(mek_solver_options* opt);
@@ -133,6 +134,10 @@ int mek_runner_kernel_time(const mek_runner* R, int kind, double* ms, long long*
 /* One Arnoldi decomposition from v: H ((m+1) x m, column-major).  Returns the
  * number of columns built (< m after a breakdown) in *k_used. */
 int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used);
+/* Launch shapes chosen by the autotuner (autotune option; zeros otherwise):
+ * SpMV row-block nonzero cap and grid, sweep grid, and their timings (us). */
+int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sweep_grid, double* spmv_us,
+                      double* sweep_us);
 /* The runner's stream (hipStream_t). */
 int mek_runner_stream(const mek_runner* R, void** stream);
 void mek_runner_destroy(mek_runner* R);

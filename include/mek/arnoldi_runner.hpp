@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "mek/autotune.hpp"
 #include "mek/csr_host.hpp"
 #include "mek/device_context.hpp"
 
@@ -69,6 +70,9 @@ struct SolverOptions {
   // Symmetric A only (checked at init): the Lanczos three-term recurrence
   // instead of Gram-Schmidt; H is tridiagonal.
   bool lanczos = false;
+  // Time candidate SpMV row-block caps and grids and sweep grids on this
+  // matrix at init and keep the fastest (mek/autotune.hpp).
+  bool autotune = false;
 };
 
 struct StepRecord {
@@ -121,6 +125,7 @@ struct ArnoldiRunner {
   // Additive
   mek::SolverOptions options;
   mek::SolveStats stats;
+  mek::TuneChoice tuning;  // additive: launch shapes chosen when options.autotune
   double anorm_inf = 0.0;   // ||A||_inf, for the Expokit initial step
 
   ArnoldiRunner(int n_gpus, const ArnoldiParams& arnoldi_params, int total_rows, int total_cols,

@@ -41,7 +41,7 @@ class SolverOptions(ctypes.Structure):
     _fields_ = [("step_control", ctypes.c_int), ("gs_mode", ctypes.c_int),
                 ("dgks_eta", ctypes.c_double), ("values_f32", ctypes.c_int),
                 ("verbose", ctypes.c_int), ("timing", ctypes.c_int),
-                ("fuse_spmv_dot", ctypes.c_int), ("lanczos", ctypes.c_int)]
+                ("fuse_spmv_dot", ctypes.c_int), ("lanczos", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
 class ArnoldiParams(ctypes.Structure):
@@ -126,6 +126,7 @@ SIGNATURES = {
     "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mek_spmv_plan_destroy": (None, [_vp]),
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
+    "mek_runner_tuning": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp, _dp]),
     "mek_block_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "mek_block_expmv": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -281,7 +282,7 @@ class Runner:
 
     def __init__(self, A: CSR, m=30, t=1.0, tol=1e-8, max_restarts=10, reorthogonalize=True,
                  step_control="reference", gs_mode=-1, dgks_eta=0.1, values_f32=False,
-                 verbose=False, timing=False, fuse_spmv_dot=False, lanczos=False, device=0):
+                 verbose=False, timing=False, fuse_spmv_dot=False, lanczos=False, autotune=False, device=0):
         p = ArnoldiParams(int(m), float(t), float(tol), int(max_restarts), 1 if reorthogonalize else 0)
         o = SolverOptions()
         lib().mek_solver_options_default(ctypes.byref(o))
@@ -293,6 +294,7 @@ class Runner:
         o.timing = 1 if timing else 0
         o.fuse_spmv_dot = 1 if fuse_spmv_dot else 0
         o.lanczos = 1 if lanczos else 0
+        o.autotune = 1 if autotune else 0
         h = _vp()
         _check(lib().mek_runner_create(A.h, ctypes.byref(p), ctypes.byref(o), int(device), ctypes.byref(h)))
         self.h = h
@@ -332,6 +334,15 @@ class Runner:
         _check(lib().mek_runner_kernel_time(self.h, KERNEL_KINDS[kind], ctypes.byref(ms), ctypes.byref(nl),
                                             ctypes.byref(nb)))
         return {"ms": ms.value, "launches": nl.value, "bytes": nb.value}
+
+    def tuning(self):
+        """Launch shapes the autotuner chose (all zero without autotune)."""
+        c, g, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().mek_runner_tuning(self.h, ctypes.byref(c), ctypes.byref(g), ctypes.byref(w),
+                                       ctypes.byref(a), ctypes.byref(b)))
+        return {"nnz_cap": c.value, "spmv_grid": g.value, "sweep_grid": w.value,
+                "spmv_us": a.value, "sweep_us": b.value}
 
     def arnoldi(self, v):
         v = _f64(v)

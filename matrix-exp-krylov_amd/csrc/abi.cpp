@@ -183,6 +183,7 @@ void mek_solver_options_default(mek_solver_options* o) {
   o->timing = d.timing;
   o->fuse_spmv_dot = d.fuse_spmv_dot;
   o->lanczos = d.lanczos;
+  o->autotune = d.autotune;
 }
 
 int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_solver_options* o,
@@ -205,6 +206,7 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       so.timing = o->timing != 0;
       so.fuse_spmv_dot = o->fuse_spmv_dot != 0;
       so.lanczos = o->lanczos != 0;
+      so.autotune = o->autotune != 0;
     }
     std::vector<CSRHost> parts{A->A};
     std::vector<int> ids{device_id};
@@ -305,6 +307,19 @@ int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used) {
     const int k = R->R->arnoldi_decomposition(vh, Hh);
     std::memcpy(H, Hh.data(), sizeof(double) * Hh.size());
     if (k_used) *k_used = k;
+  });
+}
+
+int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sweep_grid, double* spmv_us,
+                      double* sweep_us) {
+  return guarded([&] {
+    need(R && nnz_cap && spmv_grid && sweep_grid && spmv_us && sweep_us, "runner, outputs");
+    const mek::TuneChoice& t = R->R->tuning;
+    *nnz_cap = t.nnz_cap;
+    *spmv_grid = t.spmv_grid;
+    *sweep_grid = t.sweep_grid;
+    *spmv_us = t.spmv_us;
+    *sweep_us = t.sweep_us;
   });
 }
 

@@ -128,6 +128,7 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
   static const CSRHost::GhostMap kEmpty;
   dc.create_descriptors(partitions[0], global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0,
                         alloc_m);
+  if (options.autotune) tuning = mek::autotune_single(dc, partitions[0], alloc_m);
   anorm_inf = partitions[0].norm_inf();
 }
 

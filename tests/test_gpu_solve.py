@@ -169,3 +169,28 @@ def test_lanczos_rejects_nonsymmetric(gpu):
     rp, ci, va = gen.convdiff_3d(8)
     with pytest.raises(mek.MekError, match="symmetric"):
         mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=10, t=0.1, tol=1e-8, lanczos=True)
+
+
+def test_autotune_shapes_and_result(gpu):
+    """The autotuner picks shapes from its candidate sets and the tuned solve
+    gives the untuned result to rounding (same SpMV sums, other reduction
+    partitions) and the closed form."""
+    N = 512
+    rp, ci, va = gen.laplacian_2d(N)
+    v = gen.normal_vector(N * N, seed=4)
+    ys = {}
+    for at in (True, False):
+        R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=30, t=3.0, tol=1e-10, max_restarts=0,
+                       step_control="expokit", gs_mode=2, autotune=at)
+        ys[at] = R.expmv(v)
+        tn = R.tuning()
+        R.close()
+        if at:
+            print(f"[autotune] {tn}", flush=True)
+            assert tn["nnz_cap"] in (512, 1024, 2048)
+            assert tn["spmv_grid"] in (8, 1024, 2048, 4096) and tn["sweep_grid"] in (1024, 2048, 4096)
+            assert tn["spmv_us"] > 0 and tn["sweep_us"] > 0
+        else:
+            assert tn["nnz_cap"] == 0
+    assert rel_err(ys[True], ys[False]) <= 1e-12
+    assert rel_err(ys[True], laplacian_2d_expmv(N, 3.0, v)) <= 1e-8
