@@ -166,7 +166,7 @@ def run_batch(args, mek, gen):
     def batched(name, m_list, s_list, f32, select_s=0.0):
         log(f"batched: {name}")
         B = mek.Batch(csrs, ts, m_list, tols, substeps=s_list, values_f32=f32, gs_mode=gs_mode,
-                      step_control="expokit_rel")
+                      step_control="expokit_rel", autotune=args.autotune)
         B.upload(vs)
         B.solve()  # warm-up: captures the cycle graph
         mek.synchronize()
@@ -178,6 +178,7 @@ def run_batch(args, mek, gen):
         ys = B.download()
         st = [B.stats(p) for p in range(P)]
         out = {"problems_per_s": P / (el + select_s), "solve_s": el, "select_s": select_s,
+               "spmv_grid": B.spmv_grid() if args.autotune else None,
                "cycles": B.cycles(), "krylov_columns": int(sum(x["spmv_count"] for x in st)),
                "converged": int(sum(x["converged"] for x in st)), "max_rel_err": max_err(ys)}
         B.close()
@@ -243,7 +244,7 @@ def run_block(args, mek, gen):
     V /= np.linalg.norm(V, axis=0)
     A = mek.CSR.from_arrays(rp, ci, va)
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
-    B = mek.Block(A, m=m, t=t, tol=tol, gs_mode=gs_mode, timing=True)
+    B = mek.Block(A, m=m, t=t, tol=tol, gs_mode=gs_mode, timing=True, autotune=args.autotune)
     d_v = mek.DeviceBuffer.from_array(V.ravel())
     d_y = mek.DeviceBuffer(8 * V.size)
     for i in range(args.warmup):
@@ -316,6 +317,7 @@ def run_block(args, mek, gen):
         "rel_err": max(errs),
         "solve": {"substeps": st["accepted"], "attempts": st["attempts"], "block_steps": st["block_steps"],
                   "second_passes": st["second_passes"], "squarings": st["squarings"]},
+        "launch_shapes": {"spmm_grid": st["spmm_grid"], "update_grid": st["update_grid"]},
         "plain_single_vector": {"ms_per_rhs": round(single_ms, 3), "rhs_per_s": round(1000.0 / single_ms, 3),
                                 "krylov_columns": single_cols},
         "speedup_vs_single_vector": round(rhs_per_s / (1000.0 / single_ms), 3),

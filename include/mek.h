@@ -158,7 +158,9 @@ typedef struct mek_batch mek_batch;
 int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
                      const double* tol, const int* max_restarts, const int* substeps,
                      int step_control, int values_f32, int gs_mode, double dgks_eta, int use_graph,
-                     int device_id, mek_batch** out);
+                     int autotune, int device_id, mek_batch** out);
+/* SpMV grid the autotuner chose at the first solve (0 without autotune). */
+int mek_batch_tuning(const mek_batch* B, int* spmv_grid);
 /* v[p]: host arrays of each problem's rows, copied to the device. */
 int mek_batch_upload_v(mek_batch* B, const double* const* v);
 /* Solve every problem from the uploaded v; y stays in device memory. */
@@ -235,6 +237,8 @@ typedef struct {
     int block_steps;
     int squarings;
     int second_passes;  /* block steps that took the second Gram-Schmidt pass */
+    int spmm_grid;      /* autotuned SpMM / update grids (0 without autotune) */
+    int update_grid;
     double t_done;
     double last_residual;
     double solve_ms;
@@ -243,7 +247,7 @@ typedef struct {
     long long launches[5];
 } mek_block_stats;
 int mek_block_create(const mek_csr* A, int m, double t, double tol, int substeps, int gs_mode,
-                     int device_id, int timing, mek_block** out);
+                     int device_id, int timing, int autotune, mek_block** out);
 int mek_block_expmv(mek_block* B, const double* V, double* Y);
 int mek_block_expmv_device(mek_block* B, const double* d_V, double* d_Y);
 int mek_block_stats_get(const mek_block* B, mek_block_stats* out);

@@ -39,8 +39,9 @@ struct BatchProblemStats {
 class BatchRunner {
  public:
   // gs_mode: 0 never, 1 always (reference), 2 DGKS with eta.
+  // autotune: time the SpMV grid at the first solve and keep the fastest.
   BatchRunner(int device_id, const std::vector<BatchProblem>& problems, bool values_f32,
-              int gs_mode, double dgks_eta, bool use_graph);
+              int gs_mode, double dgks_eta, bool use_graph, bool autotune = false);
   ~BatchRunner();
   BatchRunner(const BatchRunner&) = delete;
   BatchRunner& operator=(const BatchRunner&) = delete;
@@ -60,6 +61,7 @@ class BatchRunner {
   // The 16 learned-selector features of every problem (P x 16, host).
   void features(float* out);
   int cycles() const { return cycles_; }
+  int spmv_grid() const { return tuned_spmv_grid_; }
   double solve_ms() const { return solve_ms_; }
   hipStream_t stream() const { return stream_; }
   void destroy();
@@ -75,6 +77,8 @@ class BatchRunner {
   bool use_graph_ = true;
   int gs_mode_ = 1;
   double eta_ = 0.1;
+  bool autotune_ = false;
+  int tuned_spmv_grid_ = 0;
   std::vector<int> ext2int_;   // caller's problem index -> storage order
   BatchPlan plan_;
   hipStream_t stream_ = nullptr;

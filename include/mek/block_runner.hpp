@@ -41,8 +41,9 @@ class BlockRunner {
   // min(||V_step||_F, ||Y_step||_F) per sub-step.
   // gs_mode: 1 second Gram-Schmidt pass always, 2 only when a column of the
   // new block keeps less than eta of its norm in the first pass (DGKS), 0 never.
+  // autotune: time the SpMM and update-sweep grids at the first solve.
   BlockRunner(const CSRHost& A, int m, double t, double tol, int substeps, int device_id,
-              bool timing = false, int gs_mode = 2, double eta = 0.1);
+              bool timing = false, int gs_mode = 2, double eta = 0.1, bool autotune = false);
   ~BlockRunner();
   BlockRunner(const BlockRunner&) = delete;
   BlockRunner& operator=(const BlockRunner&) = delete;
@@ -52,11 +53,14 @@ class BlockRunner {
   void expmv(const double* h_V, double* h_Y);
   void expmv_device(const double* d_V, double* d_Y);
   const BlockStats& stats() const { return stats_; }
+  int spmm_grid() const { return tuned_spmm_grid_; }
+  int update_grid() const { return tuned_update_grid_; }
   hipStream_t stream() const { return stream_; }
   void destroy();
 
  private:
   void start_block(const double* d_src, bool device_src, const double* h_src);
+  void autotune();
   int build_basis();
   double exponential(double dt, int k);  // returns ||dt H||_1; F in the work matrix
   void timed_begin(int kind);
@@ -73,6 +77,8 @@ class BlockRunner {
   bool timing_ = false;
   int gs_mode_ = 2;
   double eta_ = 0.1;
+  bool autotune_ = false;
+  int tuned_spmm_grid_ = 0, tuned_update_grid_ = 0;
   hipStream_t stream_ = nullptr;
   void* bufs_ = nullptr;  // BlockBufs
   // device memory

@@ -98,7 +98,8 @@ SIGNATURES = {
     "mek_runner_destroy": (None, [_vp]),
     "mek_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _dp, _ip, _dp, _ip, _ip,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mek_batch_tuning": (ctypes.c_int, [_vp, _ip]),
     "mek_batch_upload_v": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
     "mek_batch_solve": (ctypes.c_int, [_vp]),
     "mek_batch_download_y": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
@@ -128,7 +129,8 @@ SIGNATURES = {
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
     "mek_runner_tuning": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp, _dp]),
     "mek_block_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(_vp)]),
     "mek_block_expmv": (ctypes.c_int, [_vp, _dp, _dp]),
     "mek_block_expmv_device": (ctypes.c_int, [_vp, _vp, _vp]),
     "mek_block_stats_get": (ctypes.c_int, [_vp, ctypes.c_void_p]),
@@ -369,7 +371,7 @@ class Batch:
     """Many independent exp(t_p A_p) v_p solves in lockstep on one GPU."""
 
     def __init__(self, mats, t, m, tol, max_restarts=None, substeps=None, values_f32=False,
-                 gs_mode=1, dgks_eta=0.1, use_graph=True, device=0, step_control="reference"):
+                 gs_mode=1, dgks_eta=0.1, use_graph=True, device=0, step_control="reference", autotune=False):
         P = len(mats)
         self.csrs = [A if isinstance(A, CSR) else CSR.from_arrays(*A) for A in mats]
         self.rows = [c.info()["rows"] for c in self.csrs]
@@ -383,7 +385,7 @@ class Batch:
         ctrl = STEP_CONTROL[step_control]
         _check(lib().mek_batch_create(P, handles, _pd(t), _pi(m), _pd(tol), _pi(mr), _pi(ss), ctrl,
                                       1 if values_f32 else 0, int(gs_mode), float(dgks_eta),
-                                      1 if use_graph else 0, int(device), ctypes.byref(h)))
+                                      1 if use_graph else 0, 1 if autotune else 0, int(device), ctypes.byref(h)))
         self.h = h
         self.P = P
 
@@ -415,6 +417,12 @@ class Batch:
         c = ctypes.c_int()
         _check(lib().mek_batch_cycles(self.h, ctypes.byref(c)))
         return c.value
+
+    def spmv_grid(self):
+        """SpMV grid chosen by the autotuner (0 without autotune)."""
+        g = ctypes.c_int()
+        _check(lib().mek_batch_tuning(self.h, ctypes.byref(g)))
+        return g.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -539,7 +547,7 @@ class SpmvPlan:
 class BlockStats(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int), ("accepted", ctypes.c_int), ("attempts", ctypes.c_int),
                 ("rejected", ctypes.c_int), ("block_steps", ctypes.c_int), ("squarings", ctypes.c_int),
-                ("second_passes", ctypes.c_int),
+                ("second_passes", ctypes.c_int), ("spmm_grid", ctypes.c_int), ("update_grid", ctypes.c_int),
                 ("t_done", ctypes.c_double), ("last_residual", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double * 5), ("kernel_bytes", ctypes.c_double * 5),
                 ("launches", ctypes.c_longlong * 5)]
@@ -553,10 +561,10 @@ class Block:
     """Y = exp(t A) V for a block of 32 right-hand sides (block Arnoldi)."""
 
     def __init__(self, A: CSR, m: int, t: float, tol: float = 1e-8, substeps: int = 0, gs_mode: int = 2,
-                 device: int = 0, timing: bool = False):
+                 device: int = 0, timing: bool = False, autotune: bool = False):
         h = _vp()
         _check(lib().mek_block_create(A.h, int(m), float(t), float(tol), int(substeps), int(gs_mode),
-                                      int(device), 1 if timing else 0, ctypes.byref(h)))
+                                      int(device), 1 if timing else 0, 1 if autotune else 0, ctypes.byref(h)))
         self.h = h
         self.n = A.info()["rows"]
 

@@ -340,8 +340,8 @@ void mek_runner_destroy(mek_runner* R) {
 
 int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
                      const double* tol, const int* max_restarts, const int* substeps, int step_control,
-                     int values_f32, int gs_mode, double dgks_eta, int use_graph, int device_id,
-                     mek_batch** out) {
+                     int values_f32, int gs_mode, double dgks_eta, int use_graph, int autotune,
+                     int device_id, mek_batch** out) {
   return guarded([&] {
     need(nprob >= 1 && A && t && m && tol && out, "batch arguments");
     need(gs_mode >= 0 && gs_mode <= 2, "gs_mode");
@@ -354,8 +354,15 @@ int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const 
     }
     auto h = std::make_unique<mek_batch>();
     h->B = std::make_unique<mek::BatchRunner>(device_id, probs, values_f32 != 0, gs_mode, dgks_eta,
-                                             use_graph != 0);
+                                             use_graph != 0, autotune != 0);
     *out = h.release();
+  });
+}
+
+int mek_batch_tuning(const mek_batch* B, int* spmv_grid) {
+  return guarded([&] {
+    need(B && spmv_grid, "batch, out");
+    *spmv_grid = B->B->spmv_grid();
   });
 }
 
@@ -564,13 +571,14 @@ int mek_expm_dense(int k, const double* d_A, double* d_E) {
 // ------------------------------------------------------- block of RHS
 
 int mek_block_create(const mek_csr* A, int m, double t, double tol, int substeps, int gs_mode,
-                     int device_id, int timing, mek_block** out) {
+                     int device_id, int timing, int autotune, mek_block** out) {
   return guarded([&] {
     need(A && out, "matrix, out");
     need(substeps >= 0, "substeps >= 0");
     need(gs_mode >= 0 && gs_mode <= 2, "gs_mode");
     auto h = std::make_unique<mek_block>();
-    h->B = std::make_unique<mek::BlockRunner>(A->A, m, t, tol, substeps, device_id, timing != 0, gs_mode);
+    h->B = std::make_unique<mek::BlockRunner>(A->A, m, t, tol, substeps, device_id, timing != 0, gs_mode, 0.1,
+                                             autotune != 0);
     *out = h.release();
   });
 }
@@ -601,6 +609,8 @@ int mek_block_stats_get(const mek_block* B, mek_block_stats* o) {
     o->block_steps = s.block_steps;
     o->squarings = s.squarings;
     o->second_passes = s.second_passes;
+    o->spmm_grid = B->B->spmm_grid();
+    o->update_grid = B->B->update_grid();
     o->t_done = s.t_done;
     o->last_residual = s.last_residual;
     o->solve_ms = s.solve_ms;

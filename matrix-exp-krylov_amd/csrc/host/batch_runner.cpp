@@ -9,6 +9,7 @@
 #include "../kernels/batch.hpp"
 #include "../kernels/launch.hpp"
 #include "../kernels/learn.hpp"
+#include "mek/autotune.hpp"
 #include "hip_check.hpp"
 #include "mek/device_context.hpp"
 
@@ -32,9 +33,9 @@ T* upload(const std::vector<T>& h) {
 }  // namespace
 
 BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_in, bool values_f32,
-                         int gs_mode, double dgks_eta, bool use_graph)
+                         int gs_mode, double dgks_eta, bool use_graph, bool autotune)
     : P_(static_cast<int>(probs_in.size())), device_(device_id), use_graph_(use_graph),
-      gs_mode_(gs_mode), eta_(dgks_eta) {
+      gs_mode_(gs_mode), eta_(dgks_eta), autotune_(autotune) {
   if (P_ < 1) throw Error("ERROR: a batch needs at least one problem");
   MEK_HIP(hipSetDevice(device_));
   // internal order: decreasing m (stable), so Arnoldi column j concerns a
@@ -229,6 +230,17 @@ void BatchRunner::solve_device() {
   batch_prepare();
   batch_launch_init(b, stream_);
   MEK_HIP_LAUNCHED();
+  if (autotune_ && tuned_spmv_grid_ == 0) {
+    // every problem is active with its start block: time the column-0 SpMV
+    // over all row blocks for each grid and scale the per-column grids
+    const std::vector<int> grids = {1024, 2048, 4096, 8192};
+    std::vector<double> us;
+    const int i = time_candidates(
+        grids, [&](int g) { batch_launch_spmv(b, 0, plan_.nrb[0], g, stream_); }, stream_, 5, &us);
+    tuned_spmv_grid_ = grids[i];
+    for (int j = 0; j < M_; ++j)
+      plan_.spmv_grid[j] = std::max(8, (std::min(plan_.nrb[j], tuned_spmv_grid_) + 7) / 8 * 8);
+  }
   cycles_ = 0;
   int mcols = M_;
   const int max_cycles = 1000000;

@@ -9,6 +9,7 @@
 
 #include "../kernels/block.hpp"
 #include "hip_check.hpp"
+#include "mek/autotune.hpp"
 
 namespace mek {
 
@@ -35,9 +36,9 @@ enum Kind { kSpmm = 0, kDots = 1, kUpdate = 2, kExpm = 3, kLift = 4 };
 }  // namespace
 
 BlockRunner::BlockRunner(const CSRHost& A, int m, double t, double tol, int substeps, int device_id,
-                         bool timing, int gs_mode, double eta)
+                         bool timing, int gs_mode, double eta, bool autotune)
     : n_(A.rows), nnz_(A.nnz), m_(m), t_(t), tol_(tol), substeps_(substeps), device_(device_id),
-      timing_(timing), gs_mode_(gs_mode), eta_(eta) {
+      timing_(timing), gs_mode_(gs_mode), eta_(eta), autotune_(autotune) {
   if (gs_mode < 0 || gs_mode > 2) throw Error("ERROR: gs_mode must be 0, 1 or 2");
   if (A.rows != A.cols) throw Error("ERROR: Matrix must be square");
   if (m < 1 || m > kBlkMaxSteps) throw Error("ERROR: block steps m=" + std::to_string(m) + " outside [1, " +
@@ -96,7 +97,36 @@ BlockRunner::BlockRunner(const CSRHost& A, int m, double t, double tol, int subs
   b->partials = d_partials_;
   b->nchunks = nchunks;
   b->flags = d_flags_;
+  b->spmm_grid = 4096;
+  b->update_grid = 4096;
   bufs_ = b;
+}
+
+// Time the SpMM and update-sweep grids on this matrix (the start block is in
+// P_0; W is scratch until the first SpMM) and keep the fastest.
+void BlockRunner::autotune() {
+  BlockBufs& b = *static_cast<BlockBufs*>(bufs_);
+  const std::vector<int> grids = {1024, 2048, 4096, 8192};
+  std::vector<double> us;
+  int i = time_candidates(
+      grids,
+      [&](int g) {
+        b.spmm_grid = g;
+        block_launch_spmm(b, d_P_, d_W_, stream_);
+      },
+      stream_, 5, &us);
+  b.spmm_grid = grids[i];
+  tuned_spmm_grid_ = grids[i];
+  i = time_candidates(
+      grids,
+      [&](int g) {
+        b.update_grid = g;
+        block_launch_update(b, 1, d_W_, d_W_, 0, stream_);
+      },
+      stream_, 5, &us);
+  b.update_grid = grids[i];
+  tuned_update_grid_ = grids[i];
+  MEK_HIP(hipStreamSynchronize(stream_));
 }
 
 BlockRunner::~BlockRunner() { destroy(); }
@@ -271,6 +301,7 @@ void BlockRunner::expmv_device(const double* d_V, double* d_Y) {
   stats_ = BlockStats();
   ev_used_ = 0;
   start_block(d_V, true, nullptr);
+  if (autotune_ && tuned_spmm_grid_ == 0) autotune();
   const double blk = 8.0 * static_cast<double>(n_pad_) * kRhs;
   const size_t full = sizeof(double) * static_cast<size_t>(n_pad_) * kRhs;
   double t_now = 0.0;

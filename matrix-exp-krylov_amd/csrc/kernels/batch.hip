@@ -628,6 +628,11 @@ void batch_launch_init(const BatchBufs& b, hipStream_t s) {
   hipLaunchKernelGGL(kb_init, dim3(b.P), dim3(kBlock), 0, s, b);
 }
 
+void batch_launch_spmv(const BatchBufs& b, int j, int nrb, int grid, hipStream_t s) {
+  if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(grid), dim3(kBlock), 0, s, b, j, 0, nrb);
+  else hipLaunchKernelGGL(kb_spmv<double>, dim3(grid), dim3(kBlock), 0, s, b, j, 0, nrb);
+}
+
 void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int mcols, int gs_mode, double eta,
                         hipStream_t s) {
   const int pg = (b.P + kBlock - 1) / kBlock;

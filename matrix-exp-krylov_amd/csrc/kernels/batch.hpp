@@ -65,6 +65,8 @@ struct BatchBufs {
 
 void batch_prepare();
 void batch_launch_init(const BatchBufs& b, hipStream_t s);
+// The column-j SpMV over row blocks [0, nrb) with `grid` workgroups.
+void batch_launch_spmv(const BatchBufs& b, int j, int nrb, int grid, hipStream_t s);
 // One cycle that builds mcols Arnoldi columns (the largest m of the problems
 // still active; those with a smaller m skip the later columns).
 void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int mcols, int gs_mode, double eta,

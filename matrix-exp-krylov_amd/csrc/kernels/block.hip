@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(BlockBufs b, const double* __re
 }
 
 void block_launch_spmm(const BlockBufs& b, const double* X, double* Out, hipStream_t s) {
-  hipLaunchKernelGGL(k_spmm, dim3(4096), dim3(kBlock), 0, s, b, X, Out);
+  hipLaunchKernelGGL(k_spmm, dim3(b.spmm_grid > 0 ? b.spmm_grid : 4096), dim3(kBlock), 0, s, b, X, Out);
 }
 
 // ---------------------------------------------------------- block dots
@@ -210,7 +210,8 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(BlockBufs b, int nb, const d
 }
 
 void block_launch_update(const BlockBufs& b, int nb, const double* X, double* Out, int cond, hipStream_t s) {
-  hipLaunchKernelGGL(k_bupdate, dim3(4096), dim3(kBlock), 0, s, b, nb, X, Out, cond);
+  hipLaunchKernelGGL(k_bupdate, dim3(b.update_grid > 0 ? b.update_grid : 4096), dim3(kBlock), 0, s, b, nb, X,
+                     Out, cond);
 }
 
 // ------------------------------------------------- small block algebra

@@ -36,6 +36,8 @@ struct BlockBufs {
                           // [2] second pass needed for the current block, [3] second passes
   double* Gm;             // 32 x 32 Gram block of the new block from the first pass
   double* scal;           // [0] ||Y||_F, [1] residual estimate, [2] ||V||_F, [3] 1-norm
+  int spmm_grid;          // workgroups of the SpMM and of the update sweep
+  int update_grid;
 };
 
 // W = A * P_j (raw block j), rows < n; rows >= n of W are zero.

@@ -102,3 +102,19 @@ def test_batch_expokit_matches_single_runner(gpu, values_f32):
         A = sp.csr_matrix((va, ci, rp))
         assert rel_err(y, spla.expm_multiply(t * A, v)) <= 1e-8, p
     B.close()
+
+
+def test_batch_autotune_same_result(gpu):
+    """The autotuned SpMV grid changes only the launch shape: identical y."""
+    mats, ts, ms, vs = _problems(P=8, seed=7)
+    ys = {}
+    for at in (False, True):
+        B = mek.Batch(mats, ts, ms, 1e-9, step_control="expokit", gs_mode=2, autotune=at)
+        B.upload(vs)
+        B.solve()
+        ys[at] = B.download()
+        if at:
+            assert B.spmv_grid() in (1024, 2048, 4096, 8192)
+        B.close()
+    for a, b in zip(ys[False], ys[True]):
+        np.testing.assert_array_equal(a, b)

@@ -111,3 +111,19 @@ def test_block_rejects_bad_sizes(gpu):
     rp, ci, va = gen.laplacian_2d(10)  # 100 rows: a 32 (m + 1) basis does not fit
     with pytest.raises(mek.MekError):
         mek.Block(mek.CSR.from_arrays(rp, ci, va), m=4, t=1.0)
+
+
+def test_block_autotune_same_result(gpu):
+    N = 64
+    rp, ci, va = gen.laplacian_2d(N)
+    V = np.random.default_rng(12).standard_normal((N * N, 32))
+    A = mek.CSR.from_arrays(rp, ci, va)
+    ys = {}
+    for at in (False, True):
+        B = mek.Block(A, m=10, t=1.0, tol=1e-9, autotune=at)
+        ys[at] = B.expmv(V)
+        st = B.stats()
+        if at:
+            assert st["spmm_grid"] in (1024, 2048, 4096, 8192) and st["update_grid"] in (1024, 2048, 4096, 8192)
+        B.close()
+    np.testing.assert_allclose(ys[True], ys[False], rtol=0, atol=1e-12 * np.abs(ys[False]).max())
