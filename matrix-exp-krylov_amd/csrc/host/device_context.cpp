@@ -85,9 +85,14 @@ void DeviceContext::alloc_from(const CSRHost& part, int /*global_cols*/, int m_a
   allocated_m_max = std::max(1, m_alloc_max);
   ld = (static_cast<long long>(local_rows) + 63) / 64 * 64;
   const size_t ncols = static_cast<size_t>(allocated_m_max) + 2;
+  // V, w and y carry ld rows; rows n..ld-1 stay zero (no kernel writes them):
+  // the sweeps read rows in pairs and may touch row n when n is odd
   MEK_HIP(hipMalloc(&d_V_m, sizeof(double) * static_cast<size_t>(ld) * ncols));
-  MEK_HIP(hipMalloc(&d_w, sizeof(double) * std::max(local_rows, 1)));
-  MEK_HIP(hipMalloc(&d_y, sizeof(double) * std::max(local_rows, 1)));
+  MEK_HIP(hipMemsetAsync(d_V_m, 0, sizeof(double) * static_cast<size_t>(ld) * ncols, s));
+  MEK_HIP(hipMalloc(&d_w, sizeof(double) * static_cast<size_t>(std::max<long long>(ld, 64))));
+  MEK_HIP(hipMemsetAsync(d_w, 0, sizeof(double) * static_cast<size_t>(std::max<long long>(ld, 64)), s));
+  MEK_HIP(hipMalloc(&d_y, sizeof(double) * static_cast<size_t>(std::max<long long>(ld, 64))));
+  MEK_HIP(hipMemsetAsync(d_y, 0, sizeof(double) * static_cast<size_t>(std::max<long long>(ld, 64)), s));
   MEK_HIP(hipMalloc(&d_small, sizeof(double) * small_doubles()));
   MEK_HIP(hipMemsetAsync(d_small, 0, sizeof(double) * small_doubles(), s));
   grid = mek::streaming_grid(local_rows);

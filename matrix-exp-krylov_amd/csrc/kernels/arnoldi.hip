@@ -204,9 +204,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
 template <int MAXK>
 __global__ __launch_bounds__(kBlock) void k_update_dot_cs(ArnoldiBufs b, int j) {
   if (b.iscal[kBreakdown]) return;
-  sweep_update_dot<MAXK>(b.V, b.ld, j + 1, b.h1, b.colscale, b.w,
-                         b.V + static_cast<long long>(j + 1) * b.ld, grid_tiles(b.n), b.partials,
-                         gridDim.x, blockIdx.x);
+  sweep_update_dot<MAXK, 2>(b.V, b.ld, j + 1, b.h1, b.colscale, b.w,
+                            b.V + static_cast<long long>(j + 1) * b.ld, grid_tiles<2>(b.n), b.partials,
+                            gridDim.x, blockIdx.x);
 }
 
 // partials of q_i . x for i in [i0, i0+nq) (quantity i - i0 + qoff) and, if
@@ -215,7 +215,7 @@ template <int MAXK>
 __global__ __launch_bounds__(kBlock) void k_dots_cs(ArnoldiBufs b, const double* xv, int i0, int nq,
                                                     int with_norm, int qoff) {
   if (b.iscal[kBreakdown]) return;
-  sweep_dots<MAXK>(b.V, b.ld, i0, nq, b.colscale + i0, xv, grid_tiles(b.n), with_norm != 0,
+  sweep_dots<MAXK, 2>(b.V, b.ld, i0, nq, b.colscale + i0, xv, grid_tiles<2>(b.n), with_norm != 0,
                    b.partials + static_cast<long long>(qoff) * gridDim.x, gridDim.x, blockIdx.x);
 }
 
@@ -249,9 +249,9 @@ template <int MAXK>
 __global__ __launch_bounds__(kBlock) void k_update_norm_cs(ArnoldiBufs b, int j, int gs_mode,
                                                            double eta) {
   if (b.iscal[kBreakdown] || !reorth_needed(b, gs_mode, eta)) return;
-  sweep_update_norm<MAXK>(b.V, b.ld, j + 1, b.h2, b.colscale,
-                          b.V + static_cast<long long>(j + 1) * b.ld, grid_tiles(b.n), b.partials,
-                          blockIdx.x);
+  sweep_update_norm<MAXK, 2>(b.V, b.ld, j + 1, b.h2, b.colscale,
+                             b.V + static_cast<long long>(j + 1) * b.ld, grid_tiles<2>(b.n), b.partials,
+                             blockIdx.x);
 }
 
 // --------------------------------------------- generic kernels (j+1 > 64)
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_lift_cs(ArnoldiBufs b, int ncoef, do
   // one combined coefficient per kept column, in kept order
   for (int u = threadIdx.x; u < nu; u += kBlock) coef[u] = coef[u] * cs[u];
   __syncthreads();
-  sweep_lift<MAXK>(b.V, b.ld, nu, used, coef, b.y, grid_tiles(b.n), b.partials, blockIdx.x);
+  sweep_lift<MAXK, 2>(b.V, b.ld, nu, used, coef, b.y, grid_tiles<2>(b.n), b.partials, blockIdx.x);
 }
 
 // Generic lift for more than 64 terms (chunked loads, LDS coefficients).

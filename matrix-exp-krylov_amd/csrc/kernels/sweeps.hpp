@@ -2,25 +2,32 @@
 // single-problem kernels (arnoldi.hip) and the batched ones (batch.hip).
 //
 // Wave w of a 256-thread workgroup owns basis columns w, w+4, w+8, ... (at
-// most MAXK of them, so up to 4*MAXK columns); a tile is 64 consecutive rows,
-// one per lane.  Column indices past the count are clamped to the last column
-// with a zero coefficient, so every load of a tile is issued back-to-back with
-// no branch, and the per-column coefficients are wave-uniform registers
-// computed once.  Sweeps that produce a combination (update, lift) meet the
-// four waves' partial sums in double-buffered LDS, one barrier per tile.
+// most MAXK of them, so up to 4*MAXK columns); a tile is 64 R consecutive rows,
+// R per lane (R = 2: a lane loads a 16-byte pair of rows).  Column indices past
+// the count are clamped to the last column with a zero coefficient, so every
+// load of a tile is issued back-to-back with no branch, and the per-column
+// coefficients are wave-uniform registers computed once.  Basis columns are
+// read with non-temporal loads: each is streamed once per sweep and should not
+// displace the vectors that are reused (x, w) from the caches.  Sweeps that
+// produce a combination (update, lift) meet the four waves' partial sums in
+// double-buffered LDS, one barrier per tile.
 //
 // Per-workgroup results go to partials[q * pstride + pidx] for quantity q, in
 // a fixed order (DPP wave sums, then the owning wave's lane 0), so the
 // reductions stay deterministic.
+//
+// R = 2 needs an even tile base and even column offsets (ld even), and reads
+// (never writes) one row past an odd range end: the vectors must have that
+// padding row (the basis columns have it whenever ld > n).
 #pragma once
 
 #include "common.hpp"
 
 namespace mek {
 
-// Rows base + t*64 + lane for t = t0, t0+ts, ... < ntiles; rows >= rend are
-// outside the range (their loads are clamped to rend-1 and their results
-// dropped).  Requires rend > base.
+// Rows base + t*64*R + R*lane (+ 0..R-1) for t = t0, t0+ts, ... < ntiles;
+// rows >= rend are outside the range (their loads are clamped and their
+// results dropped).  Requires rend > base.
 struct TileRange {
   long long base;
   long long t0;
@@ -29,13 +36,58 @@ struct TileRange {
   long long rend;
 };
 
+template <int R = 1>
 __device__ __forceinline__ TileRange grid_tiles(long long n) {
-  return TileRange{0, static_cast<long long>(blockIdx.x), (n + kWave - 1) / kWave,
+  return TileRange{0, static_cast<long long>(blockIdx.x), (n + kWave * R - 1) / (kWave * R),
                    static_cast<long long>(gridDim.x), n};
 }
 
 __device__ __forceinline__ TileRange block_tiles(long long row0, long long row1) {
   return TileRange{row0, 0, (row1 - row0 + kWave - 1) / kWave, 1, row1};
+}
+
+// R rows of one column starting at p (non-temporal; p 16-byte aligned for R = 2)
+template <int R>
+__device__ __forceinline__ void load_rows_nt(const double* p, double (&v)[R]) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = __builtin_nontemporal_load(p + q);
+}
+
+template <int R>
+__device__ __forceinline__ void load_rows(const double* p, double (&v)[R]) {
+  if constexpr (R == 2) {
+    const double2 t = *reinterpret_cast<const double2*>(p);
+    v[0] = t.x;
+    v[1] = t.y;
+  } else {
+    v[0] = p[0];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void store_rows(double* p, const double (&v)[R], bool all) {
+  if constexpr (R == 2) {
+    if (all) {
+      *reinterpret_cast<double2*>(p) = double2{v[0], v[1]};
+    } else {
+      p[0] = v[0];
+    }
+  } else {
+    p[0] = v[0];
+  }
+}
+
+// First row of this lane's share of tile t, and the row its loads use (the
+// last valid R-row group when the tile runs past the range end).
+template <int R>
+__device__ __forceinline__ long long tile_row(const TileRange& tr, long long t, int lane) {
+  return tr.base + t * (kWave * R) + R * lane;
+}
+template <int R>
+__device__ __forceinline__ long long clamp_row(const TileRange& tr, long long r) {
+  if (r < tr.rend) return r;
+  const long long last = tr.rend - 1;
+  return (R == 2) ? tr.base + ((last - tr.base) & ~1LL) : last;
 }
 
 // Wave-owned columns: col[k] = first row of column (i0 + wave + 4k), clamped.
@@ -75,7 +127,7 @@ __device__ __forceinline__ void write_column_partials(const double (&acc)[MAXK],
 
 // Partials of q_i . x = cs_i * sum_r V[r, i0+i] x[r] for i < nq (quantity i)
 // and, with_norm, of ||x||^2 (quantity nq).  cs points at column i0's scale.
-template <int MAXK>
+template <int MAXK, int R = 1>
 __device__ __forceinline__ void sweep_dots(const double* __restrict__ V, long long ld, int i0, int nq,
                                            const double* cs, const double* __restrict__ x,
                                            TileRange tr, bool with_norm, double* partials,
@@ -90,15 +142,22 @@ __device__ __forceinline__ void sweep_dots(const double* __restrict__ V, long lo
   for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
   double nrm = 0.0;
   for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
-    const long long r = tr.base + t * kWave + lane;
+    const long long r = tile_row<R>(tr, t, lane);
     if (r >= tr.rend) continue;
-    double v[MAXK];
+    const bool all = (r + R - 1) < tr.rend;
+    double v[MAXK][R];
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + r];
-    const double a = x[r];
+    for (int k = 0; k < MAXK; ++k) load_rows_nt<R>(V + col[k] + r, v[k]);
+    double a[R];
+    load_rows<R>(x + r, a);
+    if (!all) a[R - 1] = 0.0;  // the padding row past the range end
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
-    if (wave == 0) nrm += a * a;
+    for (int k = 0; k < MAXK; ++k)
+#pragma unroll
+      for (int q = 0; q < R; ++q) acc[k] += v[k][q] * a[q];
+    if (wave == 0)
+#pragma unroll
+      for (int q = 0; q < R; ++q) nrm += a[q] * a[q];
   }
   write_column_partials<MAXK>(acc, nrm, nq, with_norm, cs, red, partials, pstride, pidx);
 }
@@ -106,13 +165,13 @@ __device__ __forceinline__ void sweep_dots(const double* __restrict__ V, long lo
 // out = x - sum_{i<nq} c_i V[:, i] with c_i = h_i cs_i; partials of
 // q_i . out (quantity i) and ||out||^2 (quantity nq).  out is not a column
 // in [0, nq).
-template <int MAXK>
+template <int MAXK, int R = 1>
 __device__ __forceinline__ void sweep_update_dot(const double* __restrict__ V, long long ld, int nq,
                                                  const double* h, const double* cs,
                                                  const double* __restrict__ x, double* __restrict__ out,
                                                  TileRange tr, double* partials, long long pstride,
                                                  long long pidx) {
-  __shared__ double part[2][kWavesPerBlock][kWave];
+  __shared__ double part[2][kWavesPerBlock][kWave * R];
   __shared__ double red[4 * MAXK + 1];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
@@ -130,26 +189,38 @@ __device__ __forceinline__ void sweep_update_dot(const double* __restrict__ V, l
   double nrm = 0.0;
   int buf = 0;
   for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
-    const long long r = tr.base + t * kWave + lane;
-    const long long rc = r < tr.rend ? r : tr.rend - 1;
-    double v[MAXK];
+    const long long r = tile_row<R>(tr, t, lane);
+    const long long rc = clamp_row<R>(tr, r);
+    double v[MAXK][R];
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
-    const double xr = x[rc];
-    double p = 0.0;
+    for (int k = 0; k < MAXK; ++k) load_rows_nt<R>(V + col[k] + rc, v[k]);
+    double xr[R];
+    load_rows<R>(x + rc, xr);
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
-    part[buf][wave][lane] = p;
+    for (int q = 0; q < R; ++q) {
+      double p = 0.0;
+#pragma unroll
+      for (int k = 0; k < MAXK; ++k) p += c[k] * v[k][q];
+      part[buf][wave][R * lane + q] = p;
+    }
     __syncthreads();
-    const double a = xr - ((part[buf][0][lane] + part[buf][1][lane]) +
-                           (part[buf][2][lane] + part[buf][3][lane]));
+    double a[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      a[q] = xr[q] - ((part[buf][0][R * lane + q] + part[buf][1][R * lane + q]) +
+                      (part[buf][2][R * lane + q] + part[buf][3][R * lane + q]));
     if (r < tr.rend) {
+      const bool all = (r + R - 1) < tr.rend;
+      if (!all) a[R - 1] = 0.0;
       if (wave == 0) {
-        out[r] = a;
-        nrm += a * a;
+        store_rows<R>(out + r, a, all);
+#pragma unroll
+        for (int q = 0; q < R; ++q) nrm += a[q] * a[q];
       }
 #pragma unroll
-      for (int k = 0; k < MAXK; ++k) acc[k] += v[k] * a;
+      for (int k = 0; k < MAXK; ++k)
+#pragma unroll
+        for (int q = 0; q < R; ++q) acc[k] += v[k][q] * a[q];
     }
     buf ^= 1;
   }
@@ -157,12 +228,12 @@ __device__ __forceinline__ void sweep_update_dot(const double* __restrict__ V, l
 }
 
 // io = io - sum_{i<nq} h_i cs_i V[:, i] in place; partial ||io||^2 (quantity 0).
-template <int MAXK>
+template <int MAXK, int R = 1>
 __device__ __forceinline__ void sweep_update_norm(const double* __restrict__ V, long long ld, int nq,
                                                   const double* h, const double* cs,
                                                   double* __restrict__ io, TileRange tr,
                                                   double* partials, long long pidx) {
-  __shared__ double part[2][kWavesPerBlock][kWave];
+  __shared__ double part[2][kWavesPerBlock][kWave * R];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   long long col[MAXK];
@@ -176,22 +247,32 @@ __device__ __forceinline__ void sweep_update_norm(const double* __restrict__ V, 
   double nrm = 0.0;
   int buf = 0;
   for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
-    const long long r = tr.base + t * kWave + lane;
-    const long long rc = r < tr.rend ? r : tr.rend - 1;
-    double v[MAXK];
+    const long long r = tile_row<R>(tr, t, lane);
+    const long long rc = clamp_row<R>(tr, r);
+    double v[MAXK][R];
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
-    const double xr = (wave == 0) ? io[rc] : 0.0;
-    double p = 0.0;
+    for (int k = 0; k < MAXK; ++k) load_rows_nt<R>(V + col[k] + rc, v[k]);
+    double xr[R];
+    if (wave == 0) load_rows<R>(io + rc, xr);
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
-    part[buf][wave][lane] = p;
+    for (int q = 0; q < R; ++q) {
+      double p = 0.0;
+#pragma unroll
+      for (int k = 0; k < MAXK; ++k) p += c[k] * v[k][q];
+      part[buf][wave][R * lane + q] = p;
+    }
     __syncthreads();
     if (wave == 0 && r < tr.rend) {
-      const double a = xr - ((part[buf][0][lane] + part[buf][1][lane]) +
-                             (part[buf][2][lane] + part[buf][3][lane]));
-      io[r] = a;
-      nrm += a * a;
+      const bool all = (r + R - 1) < tr.rend;
+      double a[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        a[q] = xr[q] - ((part[buf][0][R * lane + q] + part[buf][1][R * lane + q]) +
+                        (part[buf][2][R * lane + q] + part[buf][3][R * lane + q]));
+      if (!all) a[R - 1] = 0.0;
+      store_rows<R>(io + r, a, all);
+#pragma unroll
+      for (int q = 0; q < R; ++q) nrm += a[q] * a[q];
     }
     buf ^= 1;
   }
@@ -202,12 +283,12 @@ __device__ __forceinline__ void sweep_update_norm(const double* __restrict__ V, 
 
 // y = sum_{u<nu} coef_u V[:, used_u] (coefficients already include the
 // column scales); partial ||y||^2 (quantity 0).  used and coef live in LDS.
-template <int MAXK>
+template <int MAXK, int R = 1>
 __device__ __forceinline__ void sweep_lift(const double* __restrict__ V, long long ld, int nu,
                                            const int* used, const double* coef,
                                            double* __restrict__ y, TileRange tr, double* partials,
                                            long long pidx) {
-  __shared__ double part[2][kWavesPerBlock][kWave];
+  __shared__ double part[2][kWavesPerBlock][kWave * R];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   long long col[MAXK];
@@ -222,21 +303,30 @@ __device__ __forceinline__ void sweep_lift(const double* __restrict__ V, long lo
   double nrm = 0.0;
   int buf = 0;
   for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
-    const long long r = tr.base + t * kWave + lane;
-    const long long rc = r < tr.rend ? r : tr.rend - 1;
-    double v[MAXK];
+    const long long r = tile_row<R>(tr, t, lane);
+    const long long rc = clamp_row<R>(tr, r);
+    double v[MAXK][R];
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) v[k] = V[col[k] + rc];
-    double p = 0.0;
+    for (int k = 0; k < MAXK; ++k) load_rows_nt<R>(V + col[k] + rc, v[k]);
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) p += c[k] * v[k];
-    part[buf][wave][lane] = p;
+    for (int q = 0; q < R; ++q) {
+      double p = 0.0;
+#pragma unroll
+      for (int k = 0; k < MAXK; ++k) p += c[k] * v[k][q];
+      part[buf][wave][R * lane + q] = p;
+    }
     __syncthreads();
     if (wave == 0 && r < tr.rend) {
-      const double a = (part[buf][0][lane] + part[buf][1][lane]) +
-                       (part[buf][2][lane] + part[buf][3][lane]);
-      y[r] = a;
-      nrm += a * a;
+      const bool all = (r + R - 1) < tr.rend;
+      double a[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        a[q] = (part[buf][0][R * lane + q] + part[buf][1][R * lane + q]) +
+               (part[buf][2][R * lane + q] + part[buf][3][R * lane + q]);
+      if (!all) a[R - 1] = 0.0;
+      store_rows<R>(y + r, a, all);
+#pragma unroll
+      for (int q = 0; q < R; ++q) nrm += a[q] * a[q];
     }
     buf ^= 1;
   }

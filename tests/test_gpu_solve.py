@@ -194,3 +194,24 @@ def test_autotune_shapes_and_result(gpu):
             assert tn["nnz_cap"] == 0
     assert rel_err(ys[True], ys[False]) <= 1e-12
     assert rel_err(ys[True], laplacian_2d_expmv(N, 3.0, v)) <= 1e-8
+
+
+@pytest.mark.parametrize("gs_mode", [1, 2])
+def test_odd_row_count(gpu, oracle, gs_mode):
+    """Odd n: the sweeps read rows in pairs and the last pair includes the
+    zero padding row; results must not depend on it."""
+    N = 45  # n = 2025
+    rp, ci, va = gen.laplacian_2d(N)
+    v = gen.normal_vector(N * N, seed=6)
+    R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=30, t=2.0, tol=1e-10, max_restarts=0,
+                   step_control="expokit", gs_mode=gs_mode)
+    y = R.expmv(v)
+    R.close()
+    assert rel_err(y, laplacian_2d_expmv(N, 2.0, v)) <= 1e-8
+    rp, ci, va = gen.random_rows(3001, 4, 20, seed=8)
+    v = gen.normal_vector(3001, seed=9)
+    R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=20, t=0.5, tol=1e-10, max_restarts=10, gs_mode=gs_mode)
+    y = R.expmv(v)
+    R.close()
+    yref, st, _log = oracle.expmv(rp, ci, va, v, 0.5, 1e-10, 20, 10)
+    assert rel_err(y, yref) <= 1e-8
