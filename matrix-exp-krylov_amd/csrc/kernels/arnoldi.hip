@@ -37,6 +37,14 @@ __device__ __forceinline__ double load_val(const void* vals, long long k) {
   return static_cast<double>(static_cast<const VT*>(vals)[k]);
 }
 
+// The matrix is streamed once per SpMV: values and column indices are read
+// with non-temporal loads so that they do not displace x from the caches.
+template <typename VT>
+__device__ __forceinline__ double load_val_nt(const void* vals, long long k) {
+  return static_cast<double>(__builtin_nontemporal_load(static_cast<const VT*>(vals) + k));
+}
+__device__ __forceinline__ int load_idx_nt(const int* ci, long long k) { return __builtin_nontemporal_load(ci + k); }
+
 // Block-reduce NQ per-thread values (uniform control flow) and write them as
 // partials[q*grid + blk].  `red` is LDS scratch of kWavesPerBlock*NQ doubles.
 template <int NQ>
@@ -158,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
     const int k1 = rp[r1];
     if (k1 - k0 <= kNnzCap) {
       for (int k = k0 + tid; k < k1; k += kBlock)
-        prod[k - k0] = load_val<VT>(b.vals, k) * (xs * x[ci[k]]);
+        prod[k - k0] = load_val_nt<VT>(b.vals, k) * (xs * x[load_idx_nt(ci, k)]);
       __syncthreads();
       for (int r = r0 + tid; r < r1; r += kBlock) {
         const int e0 = rp[r] - k0, e1 = rp[r + 1] - k0;
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
       // one long row: the block strides through it and reduces
       double a = 0.0;
       for (int k = k0 + tid; k < k1; k += kBlock)
-        a += load_val<VT>(b.vals, k) * (xs * x[ci[k]]);
+        a += load_val_nt<VT>(b.vals, k) * (xs * x[load_idx_nt(ci, k)]);
       const double s = wave_sum(a);
       if (tid % kWave == 0) prod[tid / kWave] = s;
       __syncthreads();
