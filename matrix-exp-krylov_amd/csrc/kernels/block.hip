@@ -154,27 +154,35 @@ void block_launch_dots(const BlockBufs& b, const double* X1, int nb, const doubl
 
 // -------------------------------------------------------- block update
 
-// Out = X - sum_{i<nb} P_i G_i (X null: Out = -sum ...).  A wave owns 64
-// rows (four 16-row tiles) at a time; per basis block it loads G_i once as
-// eight 4 x 32 B-operand slabs and runs the 16 MFMAs of each row tile.  In
-// place (Out == X) is allowed: every element is read and written by one lane.
+// Out = X - sum_{i<nb} P_i G_i (X null: Out = -sum ...).  A workgroup takes
+// 256 rows at a time, a wave 64 of them (four 16-row MFMA tiles).  Per basis
+// block the wave's 64 x 32 slab of P_i (16 KB, contiguous) is loaded with
+// fully coalesced 16-byte loads into LDS (rows padded to 33 doubles against
+// bank conflicts), and the MFMA A operands are read from there; G_i is loaded
+// once as eight 4 x 32 B-operand slabs.  In place (Out == X) is allowed:
+// every element is read and written by one lane.
+constexpr int kUpdPad = kRhs + 1;
+
 __global__ __launch_bounds__(kBlock) void k_bupdate(BlockBufs b, int nb, const double* X, double* Out,
                                                     int cond) {
   if (b.flags[0] || (cond && !b.flags[2])) return;
+  __shared__ double tile[kWavesPerBlock][64 * kUpdPad];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   const int lr = lane / 16, lc = lane % 16;
-  const long long nwaves = static_cast<long long>(gridDim.x) * kWavesPerBlock;
-  const long long ntile64 = b.n_pad / 64;
-  for (long long t64 = static_cast<long long>(blockIdx.x) * kWavesPerBlock + wave; t64 < ntile64;
-       t64 += nwaves) {
-    const long long rbase = t64 * 64;
+  const long long ntile256 = b.n_pad / 256;
+  double* my = tile[wave];
+  for (long long T = blockIdx.x; T < ntile256; T += gridDim.x) {
+    const long long rbase = T * 256 + 64LL * wave;
     dbl4 acc[4][2];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = zero4();
     for (int i = 0; i < nb; ++i) {
       const double* __restrict__ Gi = b.G + static_cast<long long>(i) * kRhs * kRhs;
-      const double* __restrict__ Pi = b.P + static_cast<long long>(i) * b.n_pad * kRhs;
+      const double* __restrict__ src = b.P + static_cast<long long>(i) * b.n_pad * kRhs + rbase * kRhs;
+      double2 v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = *reinterpret_cast<const double2*>(src + u * 128 + 2 * lane);
       double g0[8], g1[8];
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks) {
@@ -182,17 +190,24 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(BlockBufs b, int nb, const d
         g1[ks] = Gi[(4 * ks + lr) * kRhs + 16 + lc];
       }
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const double* prow = Pi + (rbase + 16 * t + lc) * kRhs + lr;
-        double a[8];
+      for (int u = 0; u < 16; ++u) {
+        const int e = u * 128 + 2 * lane;  // element of the 64 x 32 slab
+        const int row = e / kRhs, col = e % kRhs;
+        my[row * kUpdPad + col] = v[u].x;
+        my[row * kUpdPad + col + 1] = v[u].y;
+      }
+      __syncthreads();
 #pragma unroll
-        for (int ks = 0; ks < 8; ++ks) a[ks] = prow[4 * ks];
+      for (int t = 0; t < 4; ++t) {
+        const double* prow = my + (16 * t + lc) * kUpdPad + lr;
 #pragma unroll
         for (int ks = 0; ks < 8; ++ks) {
-          acc[t][0] = mfma(a[ks], g0[ks], acc[t][0]);
-          acc[t][1] = mfma(a[ks], g1[ks], acc[t][1]);
+          const double a = prow[4 * ks];
+          acc[t][0] = mfma(a, g0[ks], acc[t][0]);
+          acc[t][1] = mfma(a, g1[ks], acc[t][1]);
         }
       }
+      __syncthreads();
     }
     // acc[t][nh] element q: row 16 t + lr + 4 q, column 16 nh + lc
 #pragma unroll
