@@ -347,8 +347,9 @@ def main():
     ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this solver runs on one GPU; multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py: the solver runs on one GPU; use --gpus 1 (each problem's rows would be split "
+              "across GPUs with a halo exchange per SpMV and one all-reduce per Gram-Schmidt pass)",
+              file=sys.stderr)
         sys.exit(2)
 
     mek = _load("mek", os.path.join(ROOT, "matrix-exp-krylov_amd", "__init__.py"))

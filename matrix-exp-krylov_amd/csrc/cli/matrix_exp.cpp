@@ -151,7 +151,7 @@ int main(int argc, char* argv[]) {
       std::cout << "Created test matrix: " << A.rows << "x" << A.cols << " with " << A.nnz
                 << " non-zeros" << std::endl;
     }
-    const int num_gpus_to_use = 1;  // this build runs the solve on one GPU
+    const int num_gpus_to_use = 1;  // the solve runs on one GPU
 
     double max_abs = A.max_abs_value();
     if (max_abs <= 0.0) max_abs = 1.0;

@@ -105,7 +105,7 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
     throw mek::Error("ERROR: device_ids size (" + std::to_string(device_ids.size()) +
                      ") must match num_gpus (" + std::to_string(num_gpus) + ")");
   if (num_gpus != 1)
-    throw mek::Error("ERROR: this build runs the solve on one GPU (num_gpus=" +
+    throw mek::Error("ERROR: the solve runs on one GPU (num_gpus=" +
                      std::to_string(num_gpus) + ")");
   if (options.lanczos && !csr_is_symmetric(partitions[0]))
     throw mek::Error("ERROR: the Lanczos path needs a symmetric matrix");

@@ -16,7 +16,7 @@ Pipeline (every GPU step runs through libmek.so):
    (16 -> 64 -> 64 -> 2) is trained on the GPU by the library's own kernels.
 4. ``Selector.predict``: (m, substeps) per problem; the batched solver runs
    them under Expokit step control with t/substeps as the first step, so the
-   local error estimate judges every step and a rejected step falls back to
+   local error estimate checks every step and a rejected step falls back to
    Expokit's own step formula: a poor prediction costs time, not accuracy.
 
 The heuristic it is compared with: the problem's own m and Expokit's initial

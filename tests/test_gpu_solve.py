@@ -19,7 +19,7 @@ def _runner(rp, ci, va, **kw):
     return mek.Runner(mek.CSR.from_arrays(rp, ci, va), **kw)
 
 
-def test_cfg1_random_matches_oracle(gpu, oracle):
+def test_random_n2000_matches_oracle(gpu, oracle):
     """n=2000 random Gershgorin-negative matrix, v=0.1, t=1, m=30, tol 1e-6, R=10."""
     rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
     v = np.full(2000, 0.1)
