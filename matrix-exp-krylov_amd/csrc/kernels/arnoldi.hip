@@ -130,13 +130,17 @@ __device__ __forceinline__ double sweep_combination(const double* __restrict__ V
 // workgroup owns a contiguous range of SpMV row blocks (XCD-aware), so x
 // windows of neighbouring rows stay in one L2.  Short-row blocks stage the
 // products val*x in LDS and sum each row in storage order, the same
-// operations as a sequential CSR loop.
+// operations as a sequential CSR loop.  All nonzero loads of a block are
+// issued together (kNnzCap / kBlock per thread), and the next block's bounds
+// are loaded while the current one is processed, so a block costs about two
+// dependent memory latencies (nonzeros, then x).
 template <typename VT, int MAXJ>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double* x_in,
                                                      const double* xs_ptr, double* wout,
                                                      int ndots, int normq, int check_breakdown,
                                                      int i0) {
   if (check_breakdown && b.iscal[kBreakdown]) return;
+  constexpr int U = kNnzCap / kBlock;
   __shared__ double prod[kNnzCap];
   __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
   const int lb = xcd_block_id();
@@ -150,6 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
   const double* __restrict__ Vd = b.V + static_cast<long long>(i0) * b.ld;
   const int* __restrict__ rp = b.row_ptr;
   const int* __restrict__ ci = b.col_idx;
+  const int* __restrict__ rbk = b.row_blocks;
   double* __restrict__ w = wout;
 
   double acc[MAXJ + 1];
@@ -159,17 +164,51 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
   const long long nrb = b.n_row_blocks;
   const int rb0 = static_cast<int>(nrb * lb / grid);
   const int rb1 = static_cast<int>(nrb * (lb + 1) / grid);
+  // block rb spans rows [r0, r1) and nonzeros [k0, k1); r2 ends block rb + 1
+  int r0 = 0, r1 = 0, k0 = 0, k1 = 0, r2 = 0;
+  if (rb0 < rb1) {
+    r0 = rbk[rb0];
+    r1 = rbk[rb0 + 1];
+    k0 = rp[r0];
+    k1 = rp[r1];
+    r2 = (rb0 + 1 < rb1) ? rbk[rb0 + 2] : r1;
+  }
   for (int rb = rb0; rb < rb1; ++rb) {
-    const int r0 = b.row_blocks[rb];
-    const int r1 = b.row_blocks[rb + 1];
-    const int k0 = rp[r0];
-    const int k1 = rp[r1];
+    const int k2 = (rb + 1 < rb1) ? rp[r2] : k1;
+    const int r3 = (rb + 2 < rb1) ? rbk[rb + 3] : r2;
     if (k1 - k0 <= kNnzCap) {
-      for (int k = k0 + tid; k < k1; k += kBlock)
-        prod[k - k0] = load_val_nt<VT>(b.vals, k) * (xs * x[load_idx_nt(ci, k)]);
+      int idx[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + tid + u * kBlock;
+        idx[u] = 0;
+        v[u] = 0.0;
+        if (k < k1) {
+          idx[u] = load_idx_nt(ci, k);
+          v[u] = load_val_nt<VT>(b.vals, k);
+        }
+      }
+      const int rr = r0 + tid;
+      int e0 = 0, e1 = 0;
+      if (rr < r1) {
+        e0 = rp[rr] - k0;
+        e1 = rp[rr + 1] - k0;
+      }
+      double pr[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) pr[u] = v[u] * (xs * x[idx[u]]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + tid + u * kBlock;
+        if (k < k1) prod[k - k0] = pr[u];
+      }
       __syncthreads();
-      for (int r = r0 + tid; r < r1; r += kBlock) {
-        const int e0 = rp[r] - k0, e1 = rp[r + 1] - k0;
+      for (int r = rr; r < r1; r += kBlock) {
+        if (r != rr) {
+          e0 = rp[r] - k0;
+          e1 = rp[r + 1] - k0;
+        }
         double a = 0.0;
         for (int e = e0; e < e1; ++e) a += prod[e];
         w[r] = a;
@@ -198,6 +237,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
       }
       __syncthreads();
     }
+    r0 = r1;
+    k0 = k1;
+    r1 = r2;
+    k1 = k2;
+    r2 = r3;
   }
   block_partials<MAXJ + 1>(acc, ndots, red, b.partials, grid, lb);
   __syncthreads();

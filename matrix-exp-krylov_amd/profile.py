@@ -37,7 +37,11 @@ KINDS = [
     ("k_bupdate", "block_update"),
     ("k_dgemm", "block_expm_gemm"),
     ("k_b", "block_other"),
-    ("k_d", "block_expm_other"),
+    ("k_dcombine", "block_expm_other"),
+    ("k_dload", "block_expm_other"),
+    ("k_dnorm1", "block_expm_other"),
+    ("k_lanczos_update", "lanczos_update"),
+    ("k_lanczos_fin", "finalise"),
     ("kb_spmv", "spmv"),
     ("kb_dots1", "dots"),
     ("kb_update_dot", "update_dot"),
