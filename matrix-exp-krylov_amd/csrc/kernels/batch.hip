@@ -203,8 +203,12 @@ __global__ __launch_bounds__(kBlock) void kb_cycle_begin(BatchBufs b) {
 // blocks [0, nrb_lim) are split into one contiguous range per XCD, and the
 // workgroups of an XCD stride through its range together, so the rows in
 // flight on an XCD (and the x entries they gather) stay close and in its L2.
+// Short blocks stage val*x in LDS and sum each row in storage order.  All
+// nonzero loads and x gathers of a block are issued together, and the next
+// block's problem and bounds are loaded one iteration ahead.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, int nrb_lim) {
+  constexpr int U = kBatchNnzCap / kBlock;
   __shared__ double prod[kBatchNnzCap];
   int xcd = 0, li = blockIdx.x, gx = gridDim.x, nx = 1;
   if (gridDim.x % kNumXcd == 0) {
@@ -213,36 +217,99 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, in
     gx = gridDim.x / kNumXcd;
     nx = kNumXcd;
   }
+  const int tid = threadIdx.x;
+  const int* __restrict__ rp = b.row_ptr;
+  const int* __restrict__ ci = b.col_idx;
   const long long nrb = nrb_lim;
   const int rb0 = static_cast<int>(nrb * xcd / nx);
   const int rb1 = static_cast<int>(nrb * (xcd + 1) / nx);
-  for (int rb = rb0 + li; rb < rb1; rb += gx) {
-    const int p = b.rb_prob[rb];
-    if (av ? !av_active(b, p) : !col_active(b, p, j)) continue;  // uniform over the block
-    const int jc = av ? pi(b, p)[kBM] : j;
-    const double* __restrict__ x = b.V + static_cast<long long>(jc) * b.ld;
-    const double xs = pcs(b, p)[jc];
-    const int r0 = b.rb_start[rb], r1 = b.rb_start[rb + 1];
-    const int k0 = b.row_ptr[r0], k1 = b.row_ptr[r1];
-    if (k1 - k0 <= kBatchNnzCap) {
-      for (int k = k0 + threadIdx.x; k < k1; k += kBlock)
-        prod[k - k0] = ldv<VT>(b.vals, k) * (xs * x[b.col_idx[k]]);
-      __syncthreads();
-      for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
-        double a = 0.0;
-        for (int e = b.row_ptr[r] - k0; e < b.row_ptr[r + 1] - k0; ++e) a += prod[e];
-        b.w[r] = a;
-      }
-      __syncthreads();
-    } else {
-      double a = 0.0;
-      for (int k = k0 + threadIdx.x; k < k1; k += kBlock) a += ldv<VT>(b.vals, k) * (xs * x[b.col_idx[k]]);
-      const double s = wave_sum(a);
-      if (threadIdx.x % kWave == 0) prod[threadIdx.x / kWave] = s;
-      __syncthreads();
-      if (threadIdx.x == 0) b.w[r0] = (prod[0] + prod[1]) + (prod[2] + prod[3]);
-      __syncthreads();
+  int rb = rb0 + li;
+  // current block: problem p, rows [r0, r1), nonzeros [k0, k1); next block:
+  // problem np, rows [nr0, nr1)
+  int p = 0, r0 = 0, r1 = 0, k0 = 0, k1 = 0, np = 0, nr0 = 0, nr1 = 0;
+  if (rb < rb1) {
+    p = b.rb_prob[rb];
+    r0 = b.rb_start[rb];
+    r1 = b.rb_start[rb + 1];
+    k0 = rp[r0];
+    k1 = rp[r1];
+  }
+  if (rb + gx < rb1) {
+    np = b.rb_prob[rb + gx];
+    nr0 = b.rb_start[rb + gx];
+    nr1 = b.rb_start[rb + gx + 1];
+  }
+  for (; rb < rb1; rb += gx) {
+    int nk0 = 0, nk1 = 0, mp = 0, mr0 = 0, mr1 = 0;
+    if (rb + gx < rb1) {
+      nk0 = rp[nr0];
+      nk1 = rp[nr1];
     }
+    if (rb + 2 * gx < rb1) {
+      mp = b.rb_prob[rb + 2 * gx];
+      mr0 = b.rb_start[rb + 2 * gx];
+      mr1 = b.rb_start[rb + 2 * gx + 1];
+    }
+    if (av ? av_active(b, p) : col_active(b, p, j)) {  // uniform over the block
+      const int jc = av ? pi(b, p)[kBM] : j;
+      const double* __restrict__ x = b.V + static_cast<long long>(jc) * b.ld;
+      const double xs = pcs(b, p)[jc];
+      if (k1 - k0 <= kBatchNnzCap) {
+        int idx[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + tid + u * kBlock;
+          idx[u] = 0;
+          v[u] = 0.0;
+          if (k < k1) {
+            idx[u] = ci[k];
+            v[u] = ldv<VT>(b.vals, k);
+          }
+        }
+        const int rr = r0 + tid;
+        int e0 = 0, e1 = 0;
+        if (rr < r1) {
+          e0 = rp[rr] - k0;
+          e1 = rp[rr + 1] - k0;
+        }
+        double pr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pr[u] = v[u] * (xs * x[idx[u]]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + tid + u * kBlock;
+          if (k < k1) prod[k - k0] = pr[u];
+        }
+        __syncthreads();
+        for (int r = rr; r < r1; r += kBlock) {
+          if (r != rr) {
+            e0 = rp[r] - k0;
+            e1 = rp[r + 1] - k0;
+          }
+          double a = 0.0;
+          for (int e = e0; e < e1; ++e) a += prod[e];
+          b.w[r] = a;
+        }
+        __syncthreads();
+      } else {
+        double a = 0.0;
+        for (int k = k0 + tid; k < k1; k += kBlock) a += ldv<VT>(b.vals, k) * (xs * x[ci[k]]);
+        const double s = wave_sum(a);
+        if (tid % kWave == 0) prod[tid / kWave] = s;
+        __syncthreads();
+        if (tid == 0) b.w[r0] = (prod[0] + prod[1]) + (prod[2] + prod[3]);
+        __syncthreads();
+      }
+    }
+    p = np;
+    r0 = nr0;
+    r1 = nr1;
+    k0 = nk0;
+    k1 = nk1;
+    np = mp;
+    nr0 = mr0;
+    nr1 = mr1;
   }
 }
 
