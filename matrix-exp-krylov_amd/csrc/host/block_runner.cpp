@@ -172,7 +172,11 @@ void BlockRunner::start_block(const double* d_src, bool device_src, const double
   const size_t bytes = sizeof(double) * static_cast<size_t>(n_) * kRhs;
   if (device_src) MEK_HIP(hipMemcpyAsync(d_P_, d_src, bytes, hipMemcpyDeviceToDevice, stream_));
   else MEK_HIP(hipMemcpyAsync(d_P_, h_src, bytes, hipMemcpyHostToDevice, stream_));
-  block_launch_dots(b, d_P_, 1, d_P_, 0, 0, stream_);
+  const double blk = 8.0 * static_cast<double>(n_pad_) * kRhs;
+  timed_begin(kDots);
+  block_launch_dots_partial(b, d_P_, 1, d_P_, 0, 0, stream_);
+  timed_end(kDots, blk);  // X1 = X2 = P_0: one block read
+  block_launch_dots_fin(b, 1, 0, 0, stream_);
   block_launch_chol(b, -1, stream_);
   MEK_HIP_LAUNCHED();
 }
@@ -193,8 +197,9 @@ int BlockRunner::build_basis() {
     timed_end(kSpmm, bspmm);
     // first pass: Q^T Z and Z^T Z in one sweep, Z' = Z - Q (Q^T Z) into P_{j+1}
     timed_begin(kDots);
-    block_launch_dots(b, d_P_, nb, d_W_, 1, 0, stream_);
+    block_launch_dots_partial(b, d_P_, nb, d_W_, 1, 0, stream_);
     timed_end(kDots, blk * (nb + 1));
+    block_launch_dots_fin(b, nb, 1, 0, stream_);
     block_launch_coeffs(b, j, nb, 1, stream_);
     block_launch_gram(b, j, gs_mode_, eta_, stream_);
     timed_begin(kUpdate);
@@ -351,7 +356,10 @@ void BlockRunner::expmv_device(const double* d_V, double* d_Y) {
     if (t_now < t_) {
       MEK_HIP(hipMemcpyAsync(d_P_, d_W_, full, hipMemcpyDeviceToDevice, stream_));
       MEK_HIP(hipMemsetAsync(d_flags_, 0, sizeof(int) * 3, stream_));
-      block_launch_dots(b, d_P_, 1, d_P_, 0, 0, stream_);
+      timed_begin(kDots);
+      block_launch_dots_partial(b, d_P_, 1, d_P_, 0, 0, stream_);
+      timed_end(kDots, blk);
+      block_launch_dots_fin(b, 1, 0, 0, stream_);
       block_launch_chol(b, -1, stream_);
       MEK_HIP_LAUNCHED();
     }

@@ -145,11 +145,21 @@ __global__ __launch_bounds__(kBlock) void k_bdots_fin(BlockBufs b, int cond) {
   if (lane == 0) b.C[static_cast<long long>(i) * kRhs * kRhs + e] = s;
 }
 
-void block_launch_dots(const BlockBufs& b, const double* X1, int nb, const double* X2, int gram, int cond,
-                       hipStream_t s) {
+void block_launch_dots_partial(const BlockBufs& b, const double* X1, int nb, const double* X2, int gram,
+                               int cond, hipStream_t s) {
   const int ntot = nb + (gram ? 1 : 0);
   hipLaunchKernelGGL(k_bdots, dim3(ntot * b.nchunks), dim3(kBlock), 0, s, b, X1, nb, X2, ntot, cond);
+}
+
+void block_launch_dots_fin(const BlockBufs& b, int nb, int gram, int cond, hipStream_t s) {
+  const int ntot = nb + (gram ? 1 : 0);
   hipLaunchKernelGGL(k_bdots_fin, dim3(ntot * (kRhs * kRhs / (kBlock / kWave))), dim3(kBlock), 0, s, b, cond);
+}
+
+void block_launch_dots(const BlockBufs& b, const double* X1, int nb, const double* X2, int gram, int cond,
+                       hipStream_t s) {
+  block_launch_dots_partial(b, X1, nb, X2, gram, cond, s);
+  block_launch_dots_fin(b, nb, gram, cond, s);
 }
 
 // -------------------------------------------------------- block update

@@ -47,6 +47,11 @@ void block_launch_spmm(const BlockBufs& b, const double* X, double* Out, hipStre
 // nothing unless flags[2] asks for the second pass.
 void block_launch_dots(const BlockBufs& b, const double* X1, int nb, const double* X2, int gram, int cond,
                        hipStream_t s);
+// The two halves of block_launch_dots: the split-K partials (k_bdots) and
+// their fixed-order reduction into b.C.
+void block_launch_dots_partial(const BlockBufs& b, const double* X1, int nb, const double* X2, int gram,
+                               int cond, hipStream_t s);
+void block_launch_dots_fin(const BlockBufs& b, int nb, int gram, int cond, hipStream_t s);
 // Out = X - sum_{i<nb} P_i G_i (X may be null: Out = -sum ...); cond as above.
 void block_launch_update(const BlockBufs& b, int nb, const double* X, double* Out, int cond, hipStream_t s);
 // One workgroup after the first pass of column j: the new block's Gram

@@ -33,6 +33,7 @@ CAL_BYTES = 1 << 30  # calibration buffer size (bytes read = bytes written)
 # kernel symbol prefix -> kind reported by the runner's event timers
 KINDS = [
     ("k_spmm", "block_spmm"),
+    ("k_bdots_fin", "block_other"),
     ("k_bdots", "block_dots"),
     ("k_bupdate", "block_update"),
     ("k_dgemm", "block_expm_gemm"),
@@ -178,7 +179,7 @@ def read_stats(d):
 
 
 def read_counter(d, counter):
-    """Per-dispatch counter values: list of (kernel name, value)."""
+    """Per-dispatch counter values in dispatch order: list of (kernel name, value)."""
     per = defaultdict(float)
     names = {}
     for fn in _find(d, "*counter_collection.csv"):
@@ -186,10 +187,27 @@ def read_counter(d, counter):
             for r in csv.DictReader(f):
                 if r["Counter_Name"] != counter:
                     continue
-                key = (fn, r["Dispatch_Id"])
+                key = (fn, int(r["Dispatch_Id"]))
                 per[key] += float(r["Counter_Value"])
                 names[key] = r["Kernel_Name"]
-    return [(names[k], v) for k, v in per.items()]
+    return [(names[k], per[k]) for k in sorted(per)]
+
+
+# A dispatch that moves less than this is a launch that returned at once (a
+# conditional second pass that was not needed, work after a breakdown): it
+# is left out of the per-launch averages of its kind.
+WORKING_BYTES = 1 << 20
+WORKING_NS = 20_000
+
+
+def read_trace(d):
+    """Per-dispatch durations (ns) from the kernel trace: list of (name, ns)."""
+    out = []
+    for fn in _find(d, "*kernel_trace.csv"):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                out.append((r["Kernel_Name"], int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+    return out
 
 
 def summarize(args):
@@ -210,28 +228,33 @@ def summarize(args):
     w_scale = CAL_BYTES / (sum(cal_w) / len(cal_w) * 1024.0)
 
     def per_kind(vals, scale):
-        acc = defaultdict(lambda: [0.0, 0])
+        out = defaultdict(list)
         for n, v in vals:
-            if is_cal(n):
-                continue
-            k = kind_of(n)
-            acc[k][0] += v * 1024.0 * scale
-            acc[k][1] += 1
-        return acc
+            if not is_cal(n):
+                out[kind_of(n)].append(v * 1024.0 * scale)
+        return out
 
+    # the FETCH and WRITE runs execute the same launch sequence, so the i-th
+    # dispatch of a kind is the same launch in both
     fk = per_kind(fetch, f_scale)
     wk = per_kind(write, w_scale)
-    traffic = {}
-    for k in sorted(set(fk) | set(wk)):
-        nf = fk[k][1] if k in fk else 0
-        nw = wk[k][1] if k in wk else 0
-        if nf == 0 or nw == 0:
-            continue
-        traffic[k] = fk[k][0] / nf + wk[k][0] / nw
+    traffic, working, idle = {}, {}, {}
+    for k in sorted(set(fk) & set(wk)):
+        if len(fk[k]) != len(wk[k]):
+            raise SystemExit(f"{k}: {len(fk[k])} FETCH dispatches but {len(wk[k])} WRITE dispatches")
+        both = [f + w for f, w in zip(fk[k], wk[k])]
+        act = [b for b in both if b >= WORKING_BYTES]
+        idle[k] = len(both) - len(act)
+        if act:
+            traffic[k] = sum(act) / len(act)
+            working[k] = len(act)
     out = {"workload": args.workload, "calibration": {
         "kernel": "torch.neg, 1 GiB FP64 read + 1 GiB written",
         "fetch_scale": f_scale, "write_scale": w_scale},
-        "bytes_per_launch": traffic}
+        "bytes_per_launch": traffic,
+        "note": f"averaged over the launches that moved at least {WORKING_BYTES} bytes; "
+                "launches that returned at once are counted under idle_launches",
+        "working_launches": working, "idle_launches": idle}
     for k, v in traffic.items():
         out[k] = v
     if have_counters and args.out_json:
@@ -263,15 +286,29 @@ def summarize(args):
         for tot, name, calls, avg in lines[:25]:
             short = name.split("(")[0].replace("void ", "")
             f.write(f"| `{short}` | {calls} | {tot / 1e6:.3f} | {avg / 1e3:.1f} |\n")
+        trace = read_trace(args.stats)
+        if trace:
+            wk_t = defaultdict(list)
+            for name, ns in trace:
+                if ns >= WORKING_NS:
+                    wk_t[kind_of(name)].append(ns)
+            f.write(f"\n## Working launches (kernel trace, duration >= {WORKING_NS / 1e3:.0f} us)\n\n"
+                    "Conditional launches that return at once (a second Gram-Schmidt pass that "
+                    "was not needed) are left out here.\n\n")
+            f.write("| kind | launches | total ms | avg us |\n|---|---|---|---|\n")
+            for k, v in sorted(wk_t.items(), key=lambda kv: -sum(kv[1])):
+                f.write(f"| {k} | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e3:.1f} |\n")
         if not have_counters:
             return
         f.write("\n## HBM traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated)\n\n")
         f.write(f"Calibration: torch.neg on 1 GiB FP64 (1 GiB read, 1 GiB written); "
                 f"FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} "
                 f"(bytes = KiB x 1024 x scale).\n\n")
-        f.write("| kind | launches | MB per launch (read + write) |\n|---|---|---|\n")
+        f.write(f"Averages over the launches that moved at least {WORKING_BYTES >> 20} MiB.\n\n")
+        f.write("| kind | working launches | idle launches | MB per launch (read + write) |\n"
+                "|---|---|---|---|\n")
         for k, v in sorted(traffic.items(), key=lambda kv: -kv[1]):
-            f.write(f"| {k} | {fk[k][1]} | {v / 1e6:.1f} |\n")
+            f.write(f"| {k} | {working[k]} | {idle[k]} | {v / 1e6:.1f} |\n")
     print(json.dumps(out))
 
 
