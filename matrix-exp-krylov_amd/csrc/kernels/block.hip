@@ -9,6 +9,8 @@
 //
 // MFMA: v_mfma_f64_16x16x4f64.  Operand A[m][k] sits in lane m + 16 k, B[k][n]
 // in lane n + 16 k; result element q of lane l is D[l / 16 + 4 q][l % 16].
+#include <algorithm>
+
 #include "block.hpp"
 
 namespace mek {
@@ -75,22 +77,25 @@ void block_launch_spmm(const BlockBufs& b, const double* X, double* Out, hipStre
 
 // ---------------------------------------------------------- block dots
 
-// Partial C_i = X1_i^T X2 over one chunk of kBlkRows rows, for block i.
-// Logical workgroup L = g * nb + i (chunk g, block i); consecutive logical
-// ids run on one XCD, so the nb workgroups of a chunk share X2 through L2.
-// Each wave takes every 4th k-step of 4 rows; the four waves' 32 x 32 tiles
-// are summed in LDS in wave order.
+// Partial C_i = X1_i^T X2 over one chunk of kBlkRows rows for the blocks
+// i0 <= i < i0 + cnt: block i < nb of X1, or (i == nb) X2 itself for its
+// Gram matrix.  Partials are laid out [block i][element e][chunk g], so the
+// finaliser reads each element's chunks contiguously.
+
+// Split-K form for a few blocks: logical workgroup L = g * cnt + (i - i0)
+// (chunk g); consecutive logical ids run on one XCD, so the workgroups of a
+// chunk share X2 through L2.  Each wave takes every 4th k-step of 4 rows; the
+// four waves' 32 x 32 tiles are summed in LDS in wave order.
 __global__ __launch_bounds__(kBlock) void k_bdots(BlockBufs b, const double* __restrict__ X1, int nb,
-                                                  const double* __restrict__ X2, int ntot, int cond) {
+                                                  const double* __restrict__ X2, int i0, int cnt, int cond) {
   if (b.flags[0] || (cond && !b.flags[2])) return;
   __shared__ double red[kWavesPerBlock][kRhs * kRhs];
   const int L = xcd_block_id();
-  const int i = L % ntot;
-  const long long g = L / ntot;
+  const int i = i0 + L % cnt;
+  const long long g = L / cnt;
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   const int lr = lane / 16, lc = lane % 16;
-  // block i < nb of X1, or (i == nb) X2 itself for its Gram matrix
   const double* __restrict__ A = (i < nb) ? X1 + static_cast<long long>(i) * b.n_pad * kRhs : X2;
   const long long row0 = g * kBlkRows;
   dbl4 c00 = zero4(), c01 = zero4(), c10 = zero4(), c11 = zero4();
@@ -123,11 +128,104 @@ __global__ __launch_bounds__(kBlock) void k_bdots(BlockBufs b, const double* __r
     red[wave][(16 + m) * kRhs + 16 + lc] = c11[q];
   }
   __syncthreads();
-  // partials laid out [block i][element e][chunk g]: the finaliser reads each
-  // element's chunks contiguously
   double* out = b.partials + static_cast<long long>(i) * (kRhs * kRhs) * b.nchunks + g;
   for (int e = threadIdx.x; e < kRhs * kRhs; e += kBlock)
     out[static_cast<long long>(e) * b.nchunks] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+}
+
+// Wave-per-block form for kDotsWavesMin .. kDotsWavesMax blocks: one
+// workgroup per chunk, wave w takes block i0 + w over all the chunk's rows
+// (no split-K, no imbalance), and each 64-row slab of X2 is loaded from HBM
+// once, into LDS (double buffered), and read from there by every wave.
+// Relying on L2 instead (the split-K form) reads X2 about 1.2 times per
+// block step; here it is read once.
+constexpr int kDotsWavesMin = 8;
+constexpr int kDotsWavesMax = 16;
+constexpr int kZRows = 64;       // rows of an X2 slab
+constexpr int kZLd = kRhs + 4;   // padded LDS row of the slab
+
+__global__ __launch_bounds__(kWave * kDotsWavesMax) void k_bdots_w(BlockBufs b, const double* __restrict__ X1,
+                                                                    int nb, const double* __restrict__ X2,
+                                                                    int i0, int cond) {
+  if (b.flags[0] || (cond && !b.flags[2])) return;
+  __shared__ double zs[2][kZRows * kZLd];
+  const long long g = xcd_block_id();
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;  // >= 64 kDotsWavesMin
+  const int lane = tid % kWave;
+  const int wave = tid / kWave;
+  const int lr = lane / 16, lc = lane % 16;
+  const long long row0 = g * kBlkRows;
+  const int i = i0 + wave;
+  const double* __restrict__ A = (i < nb) ? X1 + static_cast<long long>(i) * b.n_pad * kRhs : X2;
+  dbl4 c00 = zero4(), c01 = zero4(), c10 = zero4(), c11 = zero4();
+  // a slab is 1024 pairs of doubles; each thread moves at most two
+  constexpr int kPairs = kZRows * kRhs / 2;
+  static_assert(kPairs <= 2 * kWave * kDotsWavesMin, "two pairs per thread");
+  double2 zr[2];
+  auto load_slab = [&](int s) {
+    const double* src = X2 + (row0 + static_cast<long long>(s) * kZRows) * kRhs;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int q = tid + nt * p;
+      if (q < kPairs) zr[p] = *reinterpret_cast<const double2*>(src + 2 * q);
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int q = tid + nt * p;
+      if (q < kPairs) {
+        const int r = (2 * q) / kRhs, c = (2 * q) % kRhs;
+        zs[buf][r * kZLd + c] = zr[p].x;
+        zs[buf][r * kZLd + c + 1] = zr[p].y;
+      }
+    }
+  };
+  constexpr int kSlabs = kBlkRows / kZRows;
+  load_slab(0);
+  store_slab(0);
+  __syncthreads();
+  for (int s = 0; s < kSlabs; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < kSlabs) load_slab(s + 1);  // in flight during this slab
+#pragma unroll
+    for (int kq = 0; kq < kZRows / 16; ++kq) {
+      double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long long r = row0 + static_cast<long long>(s) * kZRows + 16 * kq + 4 * u + lr;
+        a0[u] = A[r * kRhs + lc];
+        a1[u] = A[r * kRhs + 16 + lc];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int rz = 16 * kq + 4 * u + lr;
+        b0[u] = zs[buf][rz * kZLd + lc];
+        b1[u] = zs[buf][rz * kZLd + 16 + lc];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        c00 = mfma(a0[u], b0[u], c00);
+        c01 = mfma(a0[u], b1[u], c01);
+        c10 = mfma(a1[u], b0[u], c10);
+        c11 = mfma(a1[u], b1[u], c11);
+      }
+    }
+    // every wave is done with buffer buf ^ 1 (read during slab s - 1) since
+    // the barrier that ended that slab
+    if (s + 1 < kSlabs) store_slab(buf ^ 1);
+    __syncthreads();
+  }
+  double* out = b.partials + static_cast<long long>(i) * (kRhs * kRhs) * b.nchunks + g;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int m = lr + 4 * q;
+    out[static_cast<long long>(m * kRhs + lc) * b.nchunks] = c00[q];
+    out[static_cast<long long>(m * kRhs + 16 + lc) * b.nchunks] = c01[q];
+    out[static_cast<long long>((16 + m) * kRhs + lc) * b.nchunks] = c10[q];
+    out[static_cast<long long>((16 + m) * kRhs + 16 + lc) * b.nchunks] = c11[q];
+  }
 }
 
 // C[i][e] = the sum of the chunk partials: one wave per element, lane l
@@ -145,10 +243,22 @@ __global__ __launch_bounds__(kBlock) void k_bdots_fin(BlockBufs b, int cond) {
   if (lane == 0) b.C[static_cast<long long>(i) * kRhs * kRhs + e] = s;
 }
 
+// Groups of kDotsWavesMin .. kDotsWavesMax blocks take the wave-per-block
+// form (one launch each, X2 read once per launch); fewer remaining blocks take
+// the split-K form.
 void block_launch_dots_partial(const BlockBufs& b, const double* X1, int nb, const double* X2, int gram,
                                int cond, hipStream_t s) {
   const int ntot = nb + (gram ? 1 : 0);
-  hipLaunchKernelGGL(k_bdots, dim3(ntot * b.nchunks), dim3(kBlock), 0, s, b, X1, nb, X2, ntot, cond);
+  int i0 = 0;
+  while (ntot - i0 >= kDotsWavesMin) {
+    const int w = std::min(ntot - i0, kDotsWavesMax);
+    hipLaunchKernelGGL(k_bdots_w, dim3(b.nchunks), dim3(kWave * w), 0, s, b, X1, nb, X2, i0, cond);
+    i0 += w;
+  }
+  if (i0 < ntot) {
+    const int cnt = ntot - i0;
+    hipLaunchKernelGGL(k_bdots, dim3(cnt * b.nchunks), dim3(kBlock), 0, s, b, X1, nb, X2, i0, cnt, cond);
+  }
 }
 
 void block_launch_dots_fin(const BlockBufs& b, int nb, int gram, int cond, hipStream_t s) {

@@ -71,6 +71,75 @@ def test_eigenvector_happy_breakdown(gpu):
     assert R.steps()[0]["m_used"] == 1  # breakdown at the first column
 
 
+def _dense_csr(A):
+    rows, cols = np.nonzero(A)
+    rp = np.zeros(A.shape[0] + 1, dtype=np.int32)
+    np.add.at(rp, rows + 1, 1)
+    return np.cumsum(rp).astype(np.int32), cols.astype(np.int32), A[rows, cols].astype(np.float64)
+
+
+def _small_negative(n, seed):
+    """Dense n x n matrix, off-diagonal U[-1, 1], Gershgorin-negative diagonal."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1.0, 1.0, (n, n))
+    np.fill_diagonal(A, 0.0)
+    np.fill_diagonal(A, -1.0 - np.abs(A).sum(axis=1))
+    return A
+
+
+def test_zero_start_vector(gpu, oracle):
+    """v = 0: the start scale is guarded (main_cpu.cpp:480, scale 1 when
+    ||v|| <= 1e-14), so y = 0 exactly, as in the oracle."""
+    rp, ci, va = gen.random_negative(500, 1e-2, seed=8)
+    v = np.zeros(500)
+    R = _runner(rp, ci, va, m=10, t=1.0, tol=1e-8, max_restarts=10)
+    y = R.expmv(v)
+    yref, st, _log = oracle.expmv(rp, ci, va, v, 1.0, 1e-8, 10, 10)
+    assert np.array_equal(y, np.zeros(500)) and np.array_equal(yref, np.zeros(500))
+    assert R.stats()["converged"] == st["converged"]
+
+
+def test_largest_m_and_rejected_m(gpu, oracle):
+    """m = n - 1, the largest the reference accepts (main.cu:1322-1326): the
+    m + 1 basis vectors span the whole space.  y against the oracle and a
+    dense exponential; m = n is rejected with the reference's message."""
+    import scipy.linalg
+
+    n, t, tol = 6, 0.7, 1e-8
+    A = _small_negative(n, seed=5)
+    rp, ci, va = _dense_csr(A)
+    v = np.random.default_rng(6).standard_normal(n)
+    R = _runner(rp, ci, va, m=n - 1, t=t, tol=tol, max_restarts=200)
+    y = R.expmv(v)
+    yref, st, _log = oracle.expmv(rp, ci, va, v, t, tol, n - 1, 200)
+    assert st["converged"] == 1 and R.stats()["converged"] == 1
+    assert rel_err(y, yref) <= TOL
+    assert rel_err(y, scipy.linalg.expm(t * A) @ v) <= TOL
+    assert all(s["m_used"] == n - 1 for s in R.steps())
+    with pytest.raises(mek.MekError, match="must be less than global_rows"):
+        _runner(rp, ci, va, m=n, t=t, tol=tol, max_restarts=10)
+
+
+def test_two_rows_m1_aborts_like_the_oracle(gpu, oracle):
+    """n = 2, m = 1: the reference's residual estimate |H(1,0)| |exp(dt h00)|
+    beta does not shrink with dt, so every attempt is rejected and the
+    controller gives up after 20 rejections (main_cpu.cpp:602-700); the GPU
+    controller takes the same 20 steps."""
+    A = np.array([[-2.0, 0.5], [1.0, -3.0]])
+    rp, ci, va = _dense_csr(A)
+    v = np.array([0.3, -1.1])
+    R = _runner(rp, ci, va, m=1, t=1.5, tol=1e-8, max_restarts=200)
+    R.expmv(v)
+    _y, st, log = oracle.expmv(rp, ci, va, v, 1.5, 1e-8, 1, 200)
+    assert st["aborted"] == 1 and R.stats()["aborted"] == 1
+    assert R.stats()["converged"] == 0
+    steps = R.steps()
+    assert len(steps) == len(log) == 20
+    for a, b in zip(steps, log):
+        assert a["rejects"] == b["rejects"]
+        assert a["dt"] == pytest.approx(b["dt"], rel=1e-15)
+
+
 @pytest.mark.parametrize("control", ["reference", "expokit"])
 def test_laplacian_dst(gpu, control):
     N, t = 128, 10.0
