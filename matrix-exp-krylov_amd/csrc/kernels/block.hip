@@ -25,6 +25,15 @@ __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) {
 
 __device__ __forceinline__ dbl4 zero4() { return dbl4{0.0, 0.0, 0.0, 0.0}; }
 
+// Orders a wave's LDS stores before its later LDS loads (and the loads before
+// the next stores) when the LDS region is private to the wave: no workgroup
+// barrier is needed then.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ SpMM
@@ -279,7 +288,9 @@ void block_launch_dots(const BlockBufs& b, const double* X1, int nb, const doubl
 // block the wave's 64 x 32 slab of P_i (16 KB, contiguous) is loaded with
 // fully coalesced 16-byte loads into LDS (rows padded to 33 doubles against
 // bank conflicts), and the MFMA A operands are read from there; G_i is loaded
-// once as eight 4 x 32 B-operand slabs.  In place (Out == X) is allowed:
+// once as eight 4 x 32 B-operand slabs.  Each wave's slab is private, so the
+// LDS stores and loads are ordered within the wave and the waves of a
+// workgroup never wait for each other.  In place (Out == X) is allowed:
 // every element is read and written by one lane.
 constexpr int kUpdPad = kRhs + 1;
 
@@ -316,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(BlockBufs b, int nb, const d
         my[row * kUpdPad + col] = v[u].x;
         my[row * kUpdPad + col + 1] = v[u].y;
       }
-      __syncthreads();
+      wave_lds_sync();
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const double* prow = my + (16 * t + lc) * kUpdPad + lr;
@@ -327,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(BlockBufs b, int nb, const d
           acc[t][1] = mfma(a, g1[ks], acc[t][1]);
         }
       }
-      __syncthreads();
+      wave_lds_sync();
     }
     // acc[t][nh] element q: row 16 t + lr + 4 q, column 16 nh + lc
 #pragma unroll
