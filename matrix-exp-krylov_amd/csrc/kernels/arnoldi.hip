@@ -133,7 +133,9 @@ __device__ __forceinline__ double sweep_combination(const double* __restrict__ V
 // operations as a sequential CSR loop.  All nonzero loads of a block are
 // issued together (kNnzCap / kBlock per thread), and the next block's bounds
 // are loaded while the current one is processed, so a block costs about two
-// dependent memory latencies (nonzeros, then x).
+// dependent memory latencies (nonzeros, then x).  w is written with
+// non-temporal stores: its stream then interferes less with the reads
+// (measured 7 % on a 5-point stencil).
 template <typename VT, int MAXJ>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double* x_in,
                                                      const double* xs_ptr, double* wout,
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
         }
         double a = 0.0;
         for (int e = e0; e < e1; ++e) a += prod[e];
-        w[r] = a;
+        __builtin_nontemporal_store(a, w + r);
         acc[MAXJ] += a * a;
 #pragma unroll
         for (int i = 0; i < MAXJ; ++i)
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
       if (tid == 0) {
         double t = prod[0];
         for (int q = 1; q < kWavesPerBlock; ++q) t += prod[q];
-        w[r0] = t;
+        __builtin_nontemporal_store(t, w + r0);
         acc[MAXJ] += t * t;
 #pragma unroll
         for (int i = 0; i < MAXJ; ++i)

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, in
           }
           double a = 0.0;
           for (int e = e0; e < e1; ++e) a += prod[e];
-          b.w[r] = a;
+          __builtin_nontemporal_store(a, b.w + r);
         }
         __syncthreads();
       } else {

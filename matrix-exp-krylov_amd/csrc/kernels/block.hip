@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(BlockBufs b, const double* __re
         a = a + p;
       }
     }
-    Out[r * kRhs + c] = a;
+    __builtin_nontemporal_store(a, Out + r * kRhs + c);
   }
 }
 
