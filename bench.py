@@ -88,19 +88,34 @@ def cpu_baseline(rp, ci, va, v, columns_per_solve, m, n, seconds_budget):
             break
         total += nxt
         ncols += 1
-    t0 = time.time()
-    col_ms = pyoracle.arnoldi_columns(rp, ci, va, v, ncols)
-    wall = time.time() - t0
+    # one thread pinned to one core, as the reference's single-threaded CPU
+    # binary runs (the calling thread's affinity; restored afterwards)
+    prev = os.sched_getaffinity(0)
+    core = min(prev)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.time()
+        col_ms = pyoracle.arnoldi_columns(rp, ci, va, v, ncols)
+        wall = time.time() - t0
+    finally:
+        os.sched_setaffinity(0, prev)
     j = np.arange(1, col_ms.size + 1, dtype=float)
     b, a = np.polyfit(j, col_ms, 1)  # ms per column = a + b * (basis vectors)
     per_step_ms = sum(a + b * (k + 1) for k in range(m))
     lift_ms = (m + 1) * b / 4.0  # one axpy per basis vector vs. two dots + two axpys
     solve_ms = columns_per_solve / m * (per_step_ms + lift_ms)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": 1000.0 / solve_ms,
         "unit": "solves/s",
         "cores": 1,
         "kind": "port",
+        "host": f"1 thread pinned to core {core} of {os.cpu_count()} ({model or 'unknown CPU'})",
         "sample": (f"oracle Arnoldi loop (SpMV + MGS2 + normalise, main_cpu.cpp:486-546 restated in C) "
                    f"for the first {col_ms.size} columns of the same matrix and start vector "
                    f"({wall:.1f} s wall), fitted as {a:.1f} + {b:.1f}*(j+1) ms per column and "
@@ -476,6 +491,8 @@ def main():
         "roofline": roofline,
         "spmv": spmv_roof,
         "kernels_ms_per_solve": {k: round(kern[k]["ms"] / args.steps, 3) for k in kern if kern[k]["launches"]},
+        # algorithmic bytes of every timed kernel of a solve / the solve's wall time
+        "solve_effective_gbs": round(sum(kern[k]["bytes"] for k in kern) / args.steps / (ms_per_step * 1e-3) / 1e9, 1),
         "launch_shapes": R.tuning() if args.autotune else None,
         "cpu_baseline": cpu,
     }

@@ -64,6 +64,20 @@ def test_expm_scipy_fixtures(oracle):
             assert rel_err(bad, E) > 1e-6
 
 
+def test_expm_against_torch_matrix_exp(oracle):
+    """A second independent library (torch.linalg.matrix_exp, FP64 on the
+    CPU) on upper Hessenberg H of the sizes and norms the solver produces."""
+    import torch
+
+    rng = np.random.default_rng(17)
+    for k, scale in ((3, 1e-3), (8, 0.5), (20, 5.0), (40, 40.0), (61, 300.0)):
+        H = np.triu(rng.standard_normal((k, k)), -1)
+        H *= scale / np.abs(H).sum(0).max()
+        ref = torch.linalg.matrix_exp(torch.from_numpy(H)).numpy()
+        got, _ = oracle.expm(H)
+        assert rel_err(got, ref) < 1e-12 * max(1.0, scale), (k, scale)
+
+
 def test_expm_closed_forms(oracle):
     a, t = -0.7, 3.0
     E, _ = oracle.expm(np.array([[a * t, t], [0.0, a * t]]))
