@@ -51,34 +51,64 @@ def _mods():
     return out
 
 
+def _gen():
+    """The matrix generator alone (no GPU library), for CPU worker processes."""
+    import importlib.util
+    import sys
+    if "mek_generate" not in sys.modules:
+        spec = importlib.util.spec_from_file_location("mek_generate", os.path.join(HERE, "generate.py"))
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules["mek_generate"] = mod
+        spec.loader.exec_module(mod)
+    return sys.modules["mek_generate"]
+
+
 def norm1(rp, ci, va, n):
     return float(np.bincount(ci, weights=np.abs(va), minlength=n).max())
 
 
-def training_set(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
-    """List of (row_ptr, col_idx, values, t, kind) with ||tA||_1 log-uniform in
-    [1, 1e4]; values are rounded to FP32 so FP32 storage is exact."""
-    _, gen = _mods()
+def problem_specs(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
+    """Light descriptions (index, kind, size, Pe, ||tA||_1) of a training set,
+    drawn in a fixed order from one generator; ``make_problem`` builds each.
+    Sets of large matrices are built a part at a time from these."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        kind = rng.choice(["lap2d", "cd3d", "rows"], p=[0.3, 0.3, 0.4])
+        kind = str(rng.choice(["lap2d", "cd3d", "rows"], p=[0.3, 0.3, 0.4]))
+        pe = 0.0
         if kind == "lap2d":
-            N = int(round(math.sqrt(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))))))
-            rp, ci, va = gen.laplacian_2d(max(N, 8))
+            size = int(round(math.sqrt(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))))))
         elif kind == "cd3d":
-            N = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))) ** (1 / 3)))
+            size = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))) ** (1 / 3)))
             pe = float(math.exp(rng.uniform(math.log(0.5), math.log(100.0))))
-            rp, ci, va = gen.convdiff_3d(max(N, 5), pe=pe)
         else:
-            n = int(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))))
-            rp, ci, va = gen.random_rows(n, 5, 30, seed=seed * 100003 + i)
-        va = va.astype(np.float32).astype(np.float64)  # FP32-exact values: lossless mixed storage
-        n = rp.size - 1
+            size = int(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi))))
         tau = math.exp(rng.uniform(0.0, math.log(1e4)))
-        t = tau / norm1(rp, ci, va, n)
-        out.append((rp, ci, va, t, str(kind)))
+        out.append((i, kind, size, pe, tau))
     return out
+
+
+def make_problem(spec, seed: int):
+    """(row_ptr, col_idx, values, t, kind) of one spec; values are rounded to
+    FP32 so FP32 storage is exact, and t makes ||tA||_1 = tau."""
+    gen = _gen()
+    i, kind, size, pe, tau = spec
+    if kind == "lap2d":
+        rp, ci, va = gen.laplacian_2d(max(size, 8))
+    elif kind == "cd3d":
+        rp, ci, va = gen.convdiff_3d(max(size, 5), pe=pe)
+    else:
+        rp, ci, va = gen.random_rows(size, 5, 30, seed=seed * 100003 + i)
+    va = va.astype(np.float32).astype(np.float64)  # FP32-exact values: lossless mixed storage
+    n = rp.size - 1
+    return (rp, ci, va, tau / norm1(rp, ci, va, n), kind)
+
+
+def training_set(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
+    """List of (row_ptr, col_idx, values, t, kind) with ||tA||_1 log-uniform in
+    [1, 1e4] (2-D Laplacians, 3-D upwind convection-diffusion with Pe in
+    [0.5, 100], random rows with 5-30 nonzeros)."""
+    return [make_problem(sp, seed) for sp in problem_specs(count, seed, n_lo, n_hi)]
 
 
 def modelled_bytes(m: int, s: int, n: int, nnz: int, value_bytes: int = 4, sweeps: int = 2) -> float:
