@@ -8,10 +8,13 @@ Pipeline (every GPU step runs through libmek.so):
 2. ``label``: one m=61 Arnoldi decomposition of each problem on the GPU; for
    every m on a grid the largest first sub-step dt that Expokit's step
    control accepts (local error of the augmented exponential, err_loc <=
-   1.2 dt tol; bisection), hence substeps s_m = ceil(t/dt).  The label is the
-   m (and its s_m) that minimises the modelled HBM bytes of the whole
-   mixed-precision solve, s_m * ((m+1) B_spmv + 2 Gram-Schmidt sweeps per
-   column + lift); the SpMV-minimal choice is recorded beside it.
+   1.2 dt tol; bisection), hence substeps s_m = ceil(t/dt).  Every (m, s_m)
+   is solved by the batched mixed-precision solver, and the label is the m
+   (and its s_m) whose solve takes the least modelled time: the HBM bytes of
+   its m + 1 SpMVs per sub-step at the SpMV's bandwidth (gather-bound for
+   random rows, HBM-bound for stencils), of its 2 Gram-Schmidt sweeps per
+   column and its lift at the sweeps' bandwidth.  The SpMV-minimal choice and
+   every candidate's column count are recorded beside it.
 3. ``train``: features on the GPU (``mek.features``), standardised; the MLP
    (16 -> 64 -> 64 -> 2) is trained on the GPU by the library's own kernels.
 4. ``Selector.predict``: (m, substeps) per problem; the batched solver runs
@@ -119,6 +122,32 @@ def modelled_bytes(m: int, s: int, n: int, nnz: int, value_bytes: int = 4, sweep
     bspmv = (value_bytes + 4.0) * nnz + 4.0 * (n + 1) + 16.0 * n
     gs = sweeps * 8.0 * n * m * (m + 1) / 2 + (2 * sweeps + 1) * 8.0 * n * m
     return s * ((m + 1) * bspmv + gs + 8.0 * n * (m + 2))
+
+
+# Bandwidths of the batched mixed-precision solver's kernels on one MI355X
+# (rocprofv3 kernel trace of the 256-problem batch): the SpMV of rows with
+# random column indices is bound by its x gathers, a banded stencil's SpMV and
+# the Gram-Schmidt sweeps by HBM.
+BW_SPMV_GATHER = 2.0e12
+BW_SPMV_BANDED = 4.0e12
+BW_SWEEP = 4.3e12
+
+
+def modelled_seconds(m: int, s: int, n: int, nnz: int, banded: bool, value_bytes: int = 4,
+                     sweeps: int = 2) -> float:
+    """The bytes of modelled_bytes, each kernel's at its own bandwidth: the
+    SpMV's at BW_SPMV_BANDED or BW_SPMV_GATHER, the sweeps' and the lift's at
+    BW_SWEEP."""
+    bspmv = (value_bytes + 4.0) * nnz + 4.0 * (n + 1) + 16.0 * n
+    gs = sweeps * 8.0 * n * m * (m + 1) / 2 + (2 * sweeps + 1) * 8.0 * n * m
+    bw = BW_SPMV_BANDED if banded else BW_SPMV_GATHER
+    return s * ((m + 1) * bspmv / bw + (gs + 8.0 * n * (m + 2)) / BW_SWEEP)
+
+
+def label_cost(m: int, cols: int, n: int, nnz: int, kind: str) -> float:
+    """Modelled seconds of a solve that built `cols` Krylov columns with an
+    m-column basis (cols / m sub-steps)."""
+    return modelled_seconds(m, 1, n, nnz, kind != "rows") * cols / m
 
 
 DELTA = 1.2  # Expokit's acceptance safety factor: err_loc <= DELTA * dt * tol
@@ -229,9 +258,10 @@ def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
     first sub-step count s_m comes from one m=61 Arnoldi decomposition
     (substeps_per_m); then every (m, s_m) is actually solved, a chunk of
     problems at a time, by the batched mixed-precision solver under Expokit
-    step control (tol/t), and the label is the m whose solve moved the fewest
-    modelled HBM bytes, with its s_m.  The fewest-SpMV choice is recorded
-    beside it."""
+    step control (tol/t), and the label is the m whose solve takes the least
+    modelled time (label_cost: each kernel's bytes at its bandwidth), with
+    its s_m.  The fewest-SpMV choice is recorded beside it, and so is every
+    candidate (m, s_m, columns built)."""
     mek, gen = _mods()
     X, T, meta = [], [], []
     for c0 in range(0, len(problems), chunk):
@@ -249,7 +279,6 @@ def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
             vs.append(v)
             cand.append(substeps_per_m(H, k, t, tol))
         ts = [p[3] for p in part]
-        cost = [dict() for _ in part]
         cols = [dict() for _ in part]
         for m in M_GRID:
             idx = [i for i in range(len(part)) if m in cand[i]]
@@ -261,23 +290,21 @@ def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
             B.upload([vs[i] for i in idx])
             B.solve()
             for q, i in enumerate(idx):
-                st = B.stats(q)
-                rp = part[i][0]
-                n, nnz = rp.size - 1, int(rp[-1])
-                cols[i][m] = st["spmv_count"]
-                cost[i][m] = modelled_bytes(m, 1, n, nnz) * st["spmv_count"] / m
+                cols[i][m] = B.stats(q)["spmv_count"]
             B.close()
         for i, (rp, ci, va, t, kind) in enumerate(part):
+            n, nnz = rp.size - 1, int(rp[-1])
             # breakdown-sized candidates (m not on the grid) are solved in one step
-            for m, sm in cand[i].items():
-                if m not in cost[i]:
-                    n, nnz = rp.size - 1, int(rp[-1])
-                    cols[i][m] = m
-                    cost[i][m] = modelled_bytes(m, 1, n, nnz)
-            m_b = min(cost[i], key=lambda mm: cost[i][mm])
+            for m in cand[i]:
+                cols[i].setdefault(m, m)
+            cost = {m: label_cost(m, c, n, nnz, kind) for m, c in cols[i].items()}
+            m_b = min(cost, key=lambda mm: cost[mm])
             m_s = min(cols[i], key=lambda mm: cols[i][mm])
             T.append(encode(m_b, cand[i][m_b]))
-            meta.append((kind, rp.size - 1, t, m_b, cand[i][m_b], m_s, cand[i][m_s]))
+            # every candidate (m, first sub-steps, columns built) travels with
+            # the label, so other cost models can relabel without solving again
+            meta.append((kind, n, t, m_b, cand[i][m_b], m_s, cand[i][m_s],
+                         [(mm, cand[i][mm], cols[i][mm]) for mm in sorted(cols[i])]))
         for A in csrs:
             A.close()
         if log:

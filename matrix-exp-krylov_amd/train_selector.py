@@ -145,14 +145,24 @@ def _build(spec, seed):
     return sys.modules["mek_selector"].make_problem(spec, seed)
 
 
+MAX_CAND = 12  # grid m values plus one breakdown-sized candidate
+
+
 def meta_arrays(meta):
-    """The label metadata as plain arrays (kind, n, t, byte-minimal m and
-    sub-steps, SpMV-minimal m and sub-steps)."""
+    """The label metadata as plain arrays: kind, n, t, the label's m and
+    first sub-steps (m_b, s_b), the fewest-SpMV m and sub-steps (m_s, s_s),
+    and every candidate solved, padded with -1 (cand_m, cand_s, cand_cols:
+    m, first sub-steps, Krylov columns built)."""
     cols = list(zip(*meta))
+    cand = np.full((len(meta), 3, MAX_CAND), -1, np.int64)
+    for p, cl in enumerate(cols[7]):
+        for q, (m, s, c) in enumerate(cl[:MAX_CAND]):
+            cand[p, :, q] = (m, s, c)
     return {"kind": np.array(cols[0], dtype="U8"), "n": np.array(cols[1], np.int64),
             "t": np.array(cols[2], np.float64), "m_b": np.array(cols[3], np.int32),
             "s_b": np.array(cols[4], np.int64), "m_s": np.array(cols[5], np.int32),
-            "s_s": np.array(cols[6], np.int64)}
+            "s_s": np.array(cols[6], np.int64), "cand_m": cand[:, 0], "cand_s": cand[:, 1],
+            "cand_cols": cand[:, 2]}
 
 
 def main():
@@ -206,7 +216,7 @@ def main():
         report["train_mse"] = loss
         report["train_seconds"] = time.time() - t1
         selector.save(args.out)
-        report["label_m_hist"] = np.bincount([m for *_x, m, s, ms, ss in meta], minlength=61)[10::5].tolist()
+        report["label_m_hist"] = np.bincount([mt[3] for mt in meta], minlength=61)[10::5].tolist()
         np.savez(os.path.join(os.path.dirname(os.path.abspath(args.out)), "selector_dataset.npz"), X=X, T=T)
     else:
         selector = sel.Selector.load(args.out)
@@ -215,7 +225,7 @@ def main():
     if not args.no_test_labels:
         Xt, Tt, meta_t = sel.build_dataset(test_p, args.tol, log=log)
         pred = selector.predict_features(Xt)
-        report["heldout_label_bytes"] = float(sum(sel.modelled_bytes(m, s, n, 0) for (_k, n, _t, m, s, _ms, _ss) in meta_t))
+        report["heldout_label_bytes"] = float(sum(sel.modelled_bytes(mt[3], mt[4], mt[1], 0) for mt in meta_t))
         report["heldout_pred_m_abs_err_mean"] = float(np.mean([abs(p[0] - mt[3]) for p, mt in zip(pred, meta_t)]))
     report["eval"] = evaluate(selector, test_p, args.tol)
     report["total_seconds"] = time.time() - t0

@@ -215,6 +215,37 @@ def test_cli_builtin_matrix(gpu, tmp_path, oracle):
     assert ("converged=yes" in p.stdout) == bool(st["converged"])
 
 
+def test_cli_bincsr_and_mtx_files(gpu, tmp_path, oracle):
+    """The reference's CPU configuration as a file: n = 2000 random
+    Gershgorin-negative matrix (seed 42), v = 0.1, t = 1 clamped by the CLI to
+    10 / max|a_ij| (main.cu:2228-2242), m = 30, tol 1e-6, 10 restarts; the
+    same matrix as .bincsr and as MatrixMarket gives the same y, which meets
+    the oracle at the clamped t."""
+    rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
+    n = rp.size - 1
+    gen.write_bincsr(tmp_path / "A.bincsr", rp, ci, va)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    with open(tmp_path / "A.mtx", "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n% written by the test\n")
+        f.write(f"{n} {n} {ci.size}\n")
+        for r, c, v in zip(rows, ci, va):
+            f.write(f"{r + 1} {c + 1} {float(v)!r}\n")  # shortest round-trip text of the double
+    ys = {}
+    for name, banner in (("A.bincsr", "Detected Binary CSR v1 input"), ("A.mtx", "Detected MatrixMarket input")):
+        out = tmp_path / (name + ".y")
+        env = dict(os.environ, OUTPUT_Y_FILE=str(out))
+        p = subprocess.run([mek.CLI_PATH, str(tmp_path / name), "--t", "1.0"], env=env, capture_output=True,
+                           text=True, timeout=120)
+        assert p.returncode == 0, p.stderr
+        assert banner in p.stdout and f"Loaded matrix: {n}x{n} with {ci.size} non-zeros" in p.stdout
+        assert "Clamping t to" in p.stdout
+        ys[name] = np.array([float(x) for x in out.read_text().splitlines()])
+    np.testing.assert_array_equal(ys["A.bincsr"], ys["A.mtx"])
+    t = min(1.0, 10.0 / float(np.abs(va).max()))
+    yref, st, _ = oracle.expmv(rp, ci, va, np.full(n, 0.1), t, 1e-6, 30, 10)
+    assert rel_err(ys["A.bincsr"], yref) <= TOL
+
+
 @pytest.mark.parametrize("control", ["expokit", "reference"])
 def test_lanczos_laplacian_dst(gpu, control):
     """Symmetric Lanczos path on the 2-D Laplacian against the DST-I closed
