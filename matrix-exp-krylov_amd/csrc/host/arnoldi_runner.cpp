@@ -214,12 +214,12 @@ void ArnoldiRunner::run_columns(int m_cols) {
     const int nq = j + 1;
     if (nq <= mek::kMaxFusedDots && options.fuse_spmv_dot) {
       begin_timed(mek::kKSpmv);
-      mek::launch_spmv_dot(b, j, nq, nq, s);
+      mek::launch_spmv_dots(b, j, s);
       end_timed(mek::kKSpmv, bspmv + 8.0 * n * nq);
       mek::launch_fin_h1(b, nq, b.spmv_grid, s);
     } else {
       begin_timed(mek::kKSpmv);
-      mek::launch_spmv_dot(b, j, 0, spare_q, s);
+      mek::launch_spmv_col(b, j, spare_q, s);
       end_timed(mek::kKSpmv, bspmv);
       for (int c0 = 0; c0 < nq; c0 += mek::kMaxFusedDots) {
         const int cnt = std::min(mek::kMaxFusedDots, nq - c0);

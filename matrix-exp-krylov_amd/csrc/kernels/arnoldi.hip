@@ -26,6 +26,7 @@
 // block order.  After a breakdown every later kernel of the attempt returns at
 // once, so the host can enqueue all m columns without reading back.
 #include "common.hpp"
+#include "launch.hpp"
 #include "sweeps.hpp"
 
 namespace mek {
@@ -249,6 +250,133 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
   __syncthreads();
   double nv[1] = {acc[MAXJ]};
   block_partials<1>(nv, 1, red, b.partials + static_cast<long long>(normq) * grid, grid, lb);
+}
+
+// K1 and K1b in one pass for column j (nq = j + 1 <= 64): w = A q_j as in
+// k_spmv_dot, and, while each row block's w is still in LDS, the partials of
+// q_i . w for i < nq over that block's rows, the basis columns split over the
+// waves as in the sweeps (wave w owns columns w, w + 4, ...).  The dot sweep
+// then needs no separate pass over w, and the basis reads stream while the
+// SpMV waits on its gathers.  Partials: quantity i < nq and ||w||^2 at nq,
+// spmv_grid blocks.
+template <typename VT, int MAXK>
+__global__ __launch_bounds__(kBlock) void k_spmv_dots_cs(ArnoldiBufs b, int j) {
+  if (b.iscal[kBreakdown]) return;
+  constexpr int U = kNnzCap / kBlock;
+  __shared__ double prod[kNnzCap];
+  __shared__ double wt[kSpmvRowCap];  // w of the current block's rows
+  __shared__ double red[4 * MAXK + 1];
+  __shared__ double redn[kWavesPerBlock];
+  const int lb = xcd_block_id();
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave;
+  const int grid = gridDim.x;
+  const int nq = j + 1;
+  const double* __restrict__ x = b.V + static_cast<long long>(j) * b.ld;
+  const double xs = b.colscale[j];
+  const double* __restrict__ V = b.V;
+  const int* __restrict__ rp = b.row_ptr;
+  const int* __restrict__ ci = b.col_idx;
+  const int* __restrict__ rbk = b.row_blocks;
+  double* __restrict__ w = b.w;
+  long long col[MAXK];
+  own_columns<MAXK>(col, b.ld, 0, nq);
+  double acc[MAXK];
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
+  double nrm = 0.0;
+
+  const long long nrb = b.n_row_blocks;
+  const int rb0 = static_cast<int>(nrb * lb / grid);
+  const int rb1 = static_cast<int>(nrb * (lb + 1) / grid);
+  int r0 = 0, r1 = 0, k0 = 0, k1 = 0, r2 = 0;
+  if (rb0 < rb1) {
+    r0 = rbk[rb0];
+    r1 = rbk[rb0 + 1];
+    k0 = rp[r0];
+    k1 = rp[r1];
+    r2 = (rb0 + 1 < rb1) ? rbk[rb0 + 2] : r1;
+  }
+  for (int rb = rb0; rb < rb1; ++rb) {
+    const int k2 = (rb + 1 < rb1) ? rp[r2] : k1;
+    const int r3 = (rb + 2 < rb1) ? rbk[rb + 3] : r2;
+    if (k1 - k0 <= kNnzCap) {
+      int idx[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + tid + u * kBlock;
+        idx[u] = 0;
+        v[u] = 0.0;
+        if (k < k1) {
+          idx[u] = load_idx_nt(ci, k);
+          v[u] = load_val_nt<VT>(b.vals, k);
+        }
+      }
+      const int rr = r0 + tid;
+      int e0 = 0, e1 = 0;
+      if (rr < r1) {
+        e0 = rp[rr] - k0;
+        e1 = rp[rr + 1] - k0;
+      }
+      double pr[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) pr[u] = v[u] * (xs * x[idx[u]]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + tid + u * kBlock;
+        if (k < k1) prod[k - k0] = pr[u];
+      }
+      __syncthreads();
+      for (int r = rr; r < r1; r += kBlock) {
+        if (r != rr) {
+          e0 = rp[r] - k0;
+          e1 = rp[r + 1] - k0;
+        }
+        double a = 0.0;
+        for (int e = e0; e < e1; ++e) a += prod[e];
+        __builtin_nontemporal_store(a, w + r);
+        wt[r - r0] = a;
+        nrm += a * a;
+      }
+    } else {
+      // one long row: the block strides through it and reduces
+      double a = 0.0;
+      for (int k = k0 + tid; k < k1; k += kBlock)
+        a += load_val_nt<VT>(b.vals, k) * (xs * x[load_idx_nt(ci, k)]);
+      const double s = wave_sum(a);
+      if (lane == 0) prod[tid / kWave] = s;
+      __syncthreads();
+      if (tid == 0) {
+        double t = prod[0];
+        for (int q = 1; q < kWavesPerBlock; ++q) t += prod[q];
+        __builtin_nontemporal_store(t, w + r0);
+        wt[0] = t;
+        nrm += t * t;
+      }
+    }
+    __syncthreads();  // the block's w is in wt
+    for (int rc = r0; rc < r1; rc += kWave) {
+      const int r = rc + lane;
+      if (r < r1) {
+        const double wr = wt[r - r0];
+        double vq[MAXK];
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k) vq[k] = __builtin_nontemporal_load(V + col[k] + r);
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k) acc[k] += vq[k] * wr;
+      }
+    }
+    __syncthreads();  // prod and wt are free for the next block
+    r0 = r1;
+    k0 = k1;
+    r1 = r2;
+    k1 = k2;
+    r2 = r3;
+  }
+  write_column_partials<MAXK>(acc, 0.0, nq, false, b.colscale, red, b.partials, grid, lb);
+  double nv[1] = {nrm};
+  block_partials<1>(nv, 1, redn, b.partials + static_cast<long long>(nq) * grid, grid, lb);
 }
 
 // ------------------------------------------- K2 and K1b: column-split sweeps (nq <= 64)
@@ -566,17 +694,25 @@ static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const doubl
                        ndots, normq, chk, i0);
 }
 
-// Arnoldi column j: w = A q_j with dots against q_0..q_{ndots-1}; the norm
-// partial goes to quantity index normq.  Partials use b.spmv_grid blocks.
-void launch_spmv_dot(const ArnoldiBufs& b, int j, int ndots, int normq, hipStream_t s) {
+// Arnoldi column j: w = A q_j with the ||w||^2 partial at quantity normq
+// (spmv_grid blocks).  The form with the first Gram-Schmidt pass fused in is
+// launch_spmv_dots.
+void launch_spmv_col(const ArnoldiBufs& b, int j, int normq, hipStream_t s) {
   const double* x = b.V + static_cast<long long>(j) * b.ld;
   const double* xs = b.colscale + j;
-  if (ndots <= 0) launch_spmv_dot_t<1>(b, x, xs, b.w, 0, normq, 1, s);
-  else if (ndots <= 8) launch_spmv_dot_t<8>(b, x, xs, b.w, ndots, normq, 1, s);
-  else if (ndots <= 16) launch_spmv_dot_t<16>(b, x, xs, b.w, ndots, normq, 1, s);
-  else if (ndots <= 32) launch_spmv_dot_t<32>(b, x, xs, b.w, ndots, normq, 1, s);
-  else if (ndots <= 48) launch_spmv_dot_t<48>(b, x, xs, b.w, ndots, normq, 1, s);
-  else launch_spmv_dot_t<64>(b, x, xs, b.w, ndots < 64 ? ndots : 64, normq, 1, s);
+  launch_spmv_dot_t<1>(b, x, xs, b.w, 0, normq, 1, s);
+}
+
+// Column j with the first Gram-Schmidt pass fused into the SpMV (nq <= 64):
+// partials of q_i . w (i < nq) and ||w||^2 (quantity nq), spmv_grid blocks.
+void launch_spmv_dots(const ArnoldiBufs& b, int j, hipStream_t s) {
+#define MEK_L(K)                                                                                       \
+  if (b.vals_f32)                                                                                      \
+    hipLaunchKernelGGL((k_spmv_dots_cs<float, K>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, j);       \
+  else                                                                                                 \
+    hipLaunchKernelGGL((k_spmv_dots_cs<double, K>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, j)
+  MEK_SWEEP_BUCKETS(j + 1, MEK_L);
+#undef MEK_L
 }
 
 // Lanczos column j: w = A q_j with the partial q_j . w at quantity 0 and

@@ -13,7 +13,12 @@ constexpr int kSpmvRowCap = 2048;
 constexpr int kMaxKrylov = 500;    // largest supported m
 constexpr int kMaxGrid = 4096;     // blocks of the streaming and SpMV kernels
 
-void launch_spmv_dot(const ArnoldiBufs& b, int j, int ndots, int normq, hipStream_t s);
+// Column j: w = A q_j with the ||w||^2 partial at quantity normq.
+void launch_spmv_col(const ArnoldiBufs& b, int j, int normq, hipStream_t s);
+// Column j (j + 1 <= kSweepMaxCols) with the first Gram-Schmidt pass fused
+// into the SpMV: partials of q_i . w (i <= j) and ||w||^2 (quantity j + 1),
+// spmv_grid blocks (finalise with launch_fin_h1(b, j + 1, b.spmv_grid)).
+void launch_spmv_dots(const ArnoldiBufs& b, int j, hipStream_t s);
 void launch_update_dot(const ArnoldiBufs& b, int j, hipStream_t s);
 void launch_update_norm(const ArnoldiBufs& b, int j, int gs_mode, double eta, hipStream_t s);
 void launch_dots(const ArnoldiBufs& b, const double* x, int i0, int nq, int with_norm, int qoff,

@@ -215,6 +215,49 @@ def test_cli_builtin_matrix(gpu, tmp_path, oracle):
     assert ("converged=yes" in p.stdout) == bool(st["converged"])
 
 
+@pytest.mark.parametrize("f32", [False, True])
+def test_fused_spmv_dots(gpu, f32):
+    """The first Gram-Schmidt pass fused into the SpMV (fuse_spmv_dot) gives
+    the same solve as the separate sweep: the 2-D Laplacian against its DST-I
+    closed form and against the unfused path, for basis sizes across every
+    column bucket (m = 60)."""
+    N, t = 96, 10.0
+    rp, ci, va = gen.laplacian_2d(N)
+    v = gen.normal_vector(N * N, seed=7)
+    ys = {}
+    for fused in (False, True):
+        R = _runner(rp, ci, va, m=60, t=t, tol=1e-10, max_restarts=0, step_control="expokit",
+                    gs_mode=2, values_f32=f32, fuse_spmv_dot=fused)
+        ys[fused] = R.expmv(v)
+        R.close()
+    assert rel_err(ys[True], laplacian_2d_expmv(N, t, v)) <= TOL
+    assert rel_err(ys[True], ys[False]) <= 1e-12
+
+
+def test_fused_spmv_dots_random_rows(gpu, oracle):
+    """Fused path on a nonsymmetric matrix with ragged rows (1 to 40
+    nonzeros, one row longer than an SpMV block) against the oracle."""
+    rp, ci, va = gen.random_rows(3000, 1, 40, seed=9)
+    # one long row (3000 nonzeros > 2048) takes the whole-workgroup path
+    n = rp.size - 1
+    dense_row = np.arange(n, dtype=np.int32)
+    rows = [np.asarray(ci[rp[i]:rp[i + 1]]) for i in range(n)]
+    vals = [np.asarray(va[rp[i]:rp[i + 1]]) for i in range(n)]
+    rows[17] = dense_row
+    vals[17] = np.where(dense_row == 17, -float(n), 0.5 / n)
+    rp = np.concatenate([[0], np.cumsum([r.size for r in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32)
+    va = np.concatenate(vals)
+    v = gen.normal_vector(n, seed=10)
+    R = _runner(rp, ci, va, m=30, t=0.5, tol=1e-10, max_restarts=0, step_control="expokit", gs_mode=2,
+                fuse_spmv_dot=True)
+    y = R.expmv(v)
+    R.close()
+    yref, st, _log = oracle.expmv(rp, ci, va, v, 0.5, 1e-12, 30, 200)
+    assert st["converged"] == 1
+    assert rel_err(y, yref) <= TOL
+
+
 def test_cli_bincsr_and_mtx_files(gpu, tmp_path, oracle):
     """The reference's CPU configuration as a file: n = 2000 random
     Gershgorin-negative matrix (seed 42), v = 0.1, t = 1 clamped by the CLI to
