@@ -56,7 +56,7 @@ struct DeviceContext {
   // the values in FP32 (the caller decides whether that is lossless).
   void alloc_from(const CSRHost& part, int global_cols, int m_alloc_max, bool store_f32 = false);
   // reference: create_descriptors (main.cu:900-1088).  On one GPU the matrix
-  // has no off-diagonal block; this builds the adaptive SpMV row blocks.
+  // has no off-diagonal block; the call uploads the adaptive SpMV row blocks.
   void create_descriptors(const CSRHost& part, int global_cols, const CSRHost::GhostMap& ghost_map,
                           int global_row_offset, int m);
   void destroy();
