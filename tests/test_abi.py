@@ -41,5 +41,50 @@ def test_errors_cross_the_boundary_as_status_codes(tmp_path):
     assert mek.lib().mek_version().decode().startswith("mek")
 
 
+C_CALLER = r"""
+#include <stdio.h>
+#include <string.h>
+#include "mek.h"
+
+int main(int argc, char** argv) {
+  mek_csr* A = NULL;
+  mek_solver_options o;
+  if (argc < 2) return 2;
+  printf("%s\n", mek_version());
+  if (mek_csr_load_bincsr(argv[1], &A) == MEK_OK) return 3;        /* the file does not exist */
+  printf("error: %s\n", mek_last_error());
+  if (mek_csr_load_bincsr(argv[2], &A) != MEK_OK) return 4;        /* a valid file */
+  mek_solver_options_default(&o);
+  printf("loaded; step_control=%d\n", o.step_control);
+  mek_csr_destroy(A);
+  return 0;
+}
+"""
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """A C11 program includes include/mek.h, links libmek.so and calls the
+    host-only entry points (no GPU needed), as a harness in another language
+    would through its FFI."""
+    src = tmp_path / "caller.c"
+    src.write_text(C_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(mek.LIB_PATH)
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic", f"-I{os.path.dirname(HEADER)}",
+                    str(src), "-o", str(exe), f"-L{libdir}", "-lmek", f"-Wl,-rpath,{libdir}"], check=True)
+    from conftest import gen
+    rp, ci, va = gen.laplacian_2d(5)
+    gen.write_bincsr(tmp_path / "A.bincsr", rp, ci, va)
+    p = subprocess.run([str(exe), str(tmp_path / "missing.bincsr"), str(tmp_path / "A.bincsr")],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, (p.returncode, p.stdout, p.stderr)
+    lines = p.stdout.splitlines()
+    assert lines[0].startswith("mek")
+    assert "Could not open binary CSR file" in lines[1]
+    # the loader prints the reference's header line, then the program its own
+    assert lines[2].startswith("Binary CSR header: version=1")
+    assert lines[-1].startswith("loaded; step_control=")
+
+
 def test_cli_binary_is_built():
     assert os.access(mek.CLI_PATH, os.X_OK)
