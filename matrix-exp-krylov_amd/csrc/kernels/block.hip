@@ -38,8 +38,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // ------------------------------------------------------------------ SpMM
 
-// Out[r, c] = sum_k A[r, k] X[k, c] for 32 columns c: a half-wave per row,
-// one lane per right-hand side; each row is summed in storage order with
+// Out[r, c] = sum_k A[r, k] X[k, c] for 32 columns c: a quarter-wave per
+// row, two right-hand sides per lane, so every gathered row of X is one
+// 256-byte segment read with 16-byte loads and one wave instruction carries
+// four rows (the gather instruction rate bounds this kernel: one column per
+// lane took 1.6x as long).  Each column is summed in storage order with
 // separately rounded products (the same operations as a sequential CSR loop
 // per column).  Workgroups own contiguous row ranges; rows in [n, n_pad) are
 // written as zeros.
@@ -47,36 +50,45 @@ __global__ __launch_bounds__(kBlock) void k_spmm(BlockBufs b, const double* __re
                                                  double* __restrict__ Out) {
 #pragma clang fp contract(off)
   if (b.flags[0]) return;
-  const int lane = threadIdx.x % kWave;
-  const int c = lane % kRhs;
+  constexpr int kLanesPerRow = kRhs / 2;
+  constexpr int kRowsPerPass = kBlock / kLanesPerRow;
+  const int c2 = 2 * (threadIdx.x % kLanesPerRow);  // this lane's columns c2, c2 + 1
   const long long rows_per_blk =
-      ((b.n_pad + gridDim.x - 1) / gridDim.x + 7) / 8 * 8;  // multiple of 8 rows
+      ((b.n_pad + gridDim.x - 1) / gridDim.x + kRowsPerPass - 1) / kRowsPerPass * kRowsPerPass;
   const long long r0 = static_cast<long long>(xcd_block_id()) * rows_per_blk;
   const long long r1 = r0 + rows_per_blk < b.n_pad ? r0 + rows_per_blk : b.n_pad;
-  for (long long r = r0 + threadIdx.x / kRhs; r < r1; r += kBlock / kRhs) {
-    double a = 0.0;
+  for (long long r = r0 + threadIdx.x / kLanesPerRow; r < r1; r += kRowsPerPass) {
+    double a0 = 0.0, a1 = 0.0;
     if (r < b.n) {
       const int k0 = b.row_ptr[r], k1 = b.row_ptr[r + 1];
       int k = k0;
       for (; k + 4 <= k1; k += 4) {  // four gathers in flight, summed in order
-        double v[4], x[4];
+        double v[4];
+        double2 x[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           v[u] = b.vals[k + u];
-          x[u] = X[static_cast<long long>(b.col_idx[k + u]) * kRhs + c];
+          x[u] = *reinterpret_cast<const double2*>(X + static_cast<long long>(b.col_idx[k + u]) * kRhs + c2);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const double p = v[u] * x[u];
-          a = a + p;
+          const double p0 = v[u] * x[u].x;
+          const double p1 = v[u] * x[u].y;
+          a0 = a0 + p0;
+          a1 = a1 + p1;
         }
       }
       for (; k < k1; ++k) {
-        const double p = b.vals[k] * X[static_cast<long long>(b.col_idx[k]) * kRhs + c];
-        a = a + p;
+        const double vv = b.vals[k];
+        const double2 xx = *reinterpret_cast<const double2*>(X + static_cast<long long>(b.col_idx[k]) * kRhs + c2);
+        const double p0 = vv * xx.x;
+        const double p1 = vv * xx.y;
+        a0 = a0 + p0;
+        a1 = a1 + p1;
       }
     }
-    __builtin_nontemporal_store(a, Out + r * kRhs + c);
+    __builtin_nontemporal_store(a0, Out + r * kRhs + c2);
+    __builtin_nontemporal_store(a1, Out + r * kRhs + c2 + 1);
   }
 }
 
