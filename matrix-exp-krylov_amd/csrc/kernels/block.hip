@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kWave * kDotsWavesMax) void k_bdots_w(BlockBufs b, 
                                                                     int nb, const double* __restrict__ X2,
                                                                     int i0, int cond) {
   if (b.flags[0] || (cond && !b.flags[2])) return;
-  __shared__ double zs[2][kZRows * kZLd];
+  __shared__ __align__(16) double zs[2][kZRows * kZLd];
   const long long g = xcd_block_id();
   const int tid = threadIdx.x;
   const int nt = blockDim.x;  // >= 64 kDotsWavesMin
@@ -179,7 +179,9 @@ __global__ __launch_bounds__(kWave * kDotsWavesMax) void k_bdots_w(BlockBufs b, 
   const long long row0 = g * kBlkRows;
   const int i = i0 + wave;
   const double* __restrict__ A = (i < nb) ? X1 + static_cast<long long>(i) * b.n_pad * kRhs : X2;
-  dbl4 c00 = zero4(), c01 = zero4(), c10 = zero4(), c11 = zero4();
+  // the four MFMA tiles take the even and odd columns of X1_i and of X2, so
+  // one 16-byte load feeds two of them
+  dbl4 cee = zero4(), ceo = zero4(), coe = zero4(), coo = zero4();
   // a slab is 1024 pairs of doubles; each thread moves at most two
   constexpr int kPairs = kZRows * kRhs / 2;
   static_assert(kPairs <= 2 * kWave * kDotsWavesMin, "two pairs per thread");
@@ -212,25 +214,23 @@ __global__ __launch_bounds__(kWave * kDotsWavesMax) void k_bdots_w(BlockBufs b, 
     if (s + 1 < kSlabs) load_slab(s + 1);  // in flight during this slab
 #pragma unroll
     for (int kq = 0; kq < kZRows / 16; ++kq) {
-      double a0[4], a1[4], b0[4], b1[4];
+      double2 av[4], bv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const long long r = row0 + static_cast<long long>(s) * kZRows + 16 * kq + 4 * u + lr;
-        a0[u] = A[r * kRhs + lc];
-        a1[u] = A[r * kRhs + 16 + lc];
+        av[u] = *reinterpret_cast<const double2*>(A + r * kRhs + 2 * lc);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int rz = 16 * kq + 4 * u + lr;
-        b0[u] = zs[buf][rz * kZLd + lc];
-        b1[u] = zs[buf][rz * kZLd + 16 + lc];
+        bv[u] = *reinterpret_cast<const double2*>(&zs[buf][rz * kZLd + 2 * lc]);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        c00 = mfma(a0[u], b0[u], c00);
-        c01 = mfma(a0[u], b1[u], c01);
-        c10 = mfma(a1[u], b0[u], c10);
-        c11 = mfma(a1[u], b1[u], c11);
+        cee = mfma(av[u].x, bv[u].x, cee);
+        ceo = mfma(av[u].x, bv[u].y, ceo);
+        coe = mfma(av[u].y, bv[u].x, coe);
+        coo = mfma(av[u].y, bv[u].y, coo);
       }
     }
     // every wave is done with buffer buf ^ 1 (read during slab s - 1) since
@@ -238,14 +238,16 @@ __global__ __launch_bounds__(kWave * kDotsWavesMax) void k_bdots_w(BlockBufs b, 
     if (s + 1 < kSlabs) store_slab(buf ^ 1);
     __syncthreads();
   }
+  // tile element q of lane l is (m, n) = (l / 16 + 4 q, l % 16): column 2m
+  // (+1) of X1_i against column 2n (+1) of X2
   double* out = b.partials + static_cast<long long>(i) * (kRhs * kRhs) * b.nchunks + g;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int m = lr + 4 * q;
-    out[static_cast<long long>(m * kRhs + lc) * b.nchunks] = c00[q];
-    out[static_cast<long long>(m * kRhs + 16 + lc) * b.nchunks] = c01[q];
-    out[static_cast<long long>((16 + m) * kRhs + lc) * b.nchunks] = c10[q];
-    out[static_cast<long long>((16 + m) * kRhs + 16 + lc) * b.nchunks] = c11[q];
+    const int m2 = 2 * (lr + 4 * q), n2 = 2 * lc;
+    out[static_cast<long long>(m2 * kRhs + n2) * b.nchunks] = cee[q];
+    out[static_cast<long long>(m2 * kRhs + n2 + 1) * b.nchunks] = ceo[q];
+    out[static_cast<long long>((m2 + 1) * kRhs + n2) * b.nchunks] = coe[q];
+    out[static_cast<long long>((m2 + 1) * kRhs + n2 + 1) * b.nchunks] = coo[q];
   }
 }
 
