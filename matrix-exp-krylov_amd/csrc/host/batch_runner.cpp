@@ -37,157 +37,164 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     : P_(static_cast<int>(probs_in.size())), device_(device_id), use_graph_(use_graph),
       gs_mode_(gs_mode), eta_(dgks_eta), autotune_(autotune) {
   if (P_ < 1) throw Error("ERROR: a batch needs at least one problem");
-  MEK_HIP(hipSetDevice(device_));
-  // internal order: decreasing m (stable), so Arnoldi column j concerns a
-  // prefix of the problems; callers keep their own indices
-  std::vector<int> order(P_);
-  for (int p = 0; p < P_; ++p) order[p] = p;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int c) { return probs_in[a].m > probs_in[c].m; });
-  ext2int_.assign(P_, 0);
-  for (int i = 0; i < P_; ++i) ext2int_[order[i]] = i;
-  std::vector<BatchProblem> probs(P_);
-  for (int i = 0; i < P_; ++i) probs[i] = probs_in[order[i]];
-  // ---- host assembly: block-diagonal CSR, row blocks, streaming blocks
-  row0_.assign(P_ + 1, 0);
-  long long nnz_tot = 0;
-  for (int p = 0; p < P_; ++p) {
-    const CSRHost& A = *probs[p].A;
-    if (A.rows != A.cols) throw Error("ERROR: Matrix must be square (batch problem " + std::to_string(p) + ")");
-    if (probs[p].m < 1 || probs[p].m > kBatchMaxM || probs[p].m >= A.rows)
-      throw Error("ERROR: batch problem " + std::to_string(p) + ": m=" + std::to_string(probs[p].m) +
-                  " outside [1, min(" + std::to_string(kBatchMaxM) + ", rows-1)]");
-    row0_[p + 1] = row0_[p] + A.rows;
-    nnz_tot += A.nnz;
-    M_ = std::max(M_, probs[p].m);
-  }
-  n_tot_ = row0_[P_];
-  if (n_tot_ > 0x7fffffffLL || nnz_tot > 0x7fffffffLL) throw Error("ERROR: batch exceeds int32 rows/nnz");
-  std::vector<int> row_ptr(static_cast<size_t>(n_tot_) + 1), col_idx(static_cast<size_t>(nnz_tot));
-  std::vector<double> vals(static_cast<size_t>(nnz_tot));
-  std::vector<int> rb_start, rb_prob, sb_row0, sb_prob, p_sb0(P_ + 1), p_rb0(P_ + 1);
-  long long k = 0;
-  for (int p = 0; p < P_; ++p) {
-    const CSRHost& A = *probs[p].A;
-    const int r0 = row0_[p];
-    for (int r = 0; r < A.rows; ++r) row_ptr[r0 + r] = static_cast<int>(k + A.row_ptr[r]);
-    for (long long e = 0; e < A.nnz; ++e) {
-      col_idx[k + e] = A.col_idx[e] + r0;
-      vals[k + e] = A.values[e];
+  // a failure part way releases what was already allocated (the destructor
+  // does not run for an object whose constructor threw)
+  try {
+    MEK_HIP(hipSetDevice(device_));
+    // internal order: decreasing m (stable), so Arnoldi column j concerns a
+    // prefix of the problems; callers keep their own indices
+    std::vector<int> order(P_);
+    for (int p = 0; p < P_; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return probs_in[a].m > probs_in[c].m; });
+    ext2int_.assign(P_, 0);
+    for (int i = 0; i < P_; ++i) ext2int_[order[i]] = i;
+    std::vector<BatchProblem> probs(P_);
+    for (int i = 0; i < P_; ++i) probs[i] = probs_in[order[i]];
+    // ---- host assembly: block-diagonal CSR, row blocks, streaming blocks
+    row0_.assign(P_ + 1, 0);
+    long long nnz_tot = 0;
+    for (int p = 0; p < P_; ++p) {
+      const CSRHost& A = *probs[p].A;
+      if (A.rows != A.cols) throw Error("ERROR: Matrix must be square (batch problem " + std::to_string(p) + ")");
+      if (probs[p].m < 1 || probs[p].m > kBatchMaxM || probs[p].m >= A.rows)
+        throw Error("ERROR: batch problem " + std::to_string(p) + ": m=" + std::to_string(probs[p].m) +
+                    " outside [1, min(" + std::to_string(kBatchMaxM) + ", rows-1)]");
+      row0_[p + 1] = row0_[p] + A.rows;
+      nnz_tot += A.nnz;
+      M_ = std::max(M_, probs[p].m);
     }
-    k += A.nnz;
-    const std::vector<int> rb = build_row_blocks(A.row_ptr.data(), A.rows, kBatchNnzCap, kSpmvRowCap);
-    p_rb0[p] = static_cast<int>(rb_prob.size());
-    for (size_t i = 0; i + 1 < rb.size(); ++i) {
-      rb_start.push_back(r0 + rb[i]);
-      rb_prob.push_back(p);
+    n_tot_ = row0_[P_];
+    if (n_tot_ > 0x7fffffffLL || nnz_tot > 0x7fffffffLL) throw Error("ERROR: batch exceeds int32 rows/nnz");
+    std::vector<int> row_ptr(static_cast<size_t>(n_tot_) + 1), col_idx(static_cast<size_t>(nnz_tot));
+    std::vector<double> vals(static_cast<size_t>(nnz_tot));
+    std::vector<int> rb_start, rb_prob, sb_row0, sb_prob, p_sb0(P_ + 1), p_rb0(P_ + 1);
+    long long k = 0;
+    for (int p = 0; p < P_; ++p) {
+      const CSRHost& A = *probs[p].A;
+      const int r0 = row0_[p];
+      for (int r = 0; r < A.rows; ++r) row_ptr[r0 + r] = static_cast<int>(k + A.row_ptr[r]);
+      for (long long e = 0; e < A.nnz; ++e) {
+        col_idx[k + e] = A.col_idx[e] + r0;
+        vals[k + e] = A.values[e];
+      }
+      k += A.nnz;
+      const std::vector<int> rb = build_row_blocks(A.row_ptr.data(), A.rows, kBatchNnzCap, kSpmvRowCap);
+      p_rb0[p] = static_cast<int>(rb_prob.size());
+      for (size_t i = 0; i + 1 < rb.size(); ++i) {
+        rb_start.push_back(r0 + rb[i]);
+        rb_prob.push_back(p);
+      }
+      p_sb0[p] = static_cast<int>(sb_prob.size());
+      for (int r = 0; r < A.rows; r += kBatchRowsPerBlock) {
+        sb_row0.push_back(r0 + r);
+        sb_prob.push_back(p);
+      }
     }
-    p_sb0[p] = static_cast<int>(sb_prob.size());
-    for (int r = 0; r < A.rows; r += kBatchRowsPerBlock) {
-      sb_row0.push_back(r0 + r);
-      sb_prob.push_back(p);
+    row_ptr[n_tot_] = static_cast<int>(k);
+    rb_start.push_back(static_cast<int>(n_tot_));
+    sb_row0.push_back(static_cast<int>(n_tot_));
+    p_sb0[P_] = static_cast<int>(sb_prob.size());
+    p_rb0[P_] = static_cast<int>(rb_prob.size());
+    const int nsb = static_cast<int>(sb_prob.size());
+    const int nrb = static_cast<int>(rb_prob.size());
+    // per-column launch plan over the active prefix; SpMV grids are multiples
+    // of 8 (one share per XCD)
+    plan_.np.assign(M_, 0);
+    plan_.nsb.assign(M_, 0);
+    plan_.nrb.assign(M_, 0);
+    plan_.spmv_grid.assign(M_, 8);
+    for (int j = 0; j < M_; ++j) {
+      int np = 0;
+      while (np < P_ && probs[np].m > j) ++np;
+      plan_.np[j] = np;
+      plan_.nsb[j] = p_sb0[np];
+      plan_.nrb[j] = p_rb0[np];
+      const int g = std::min(plan_.nrb[j], 2048);
+      plan_.spmv_grid[j] = std::max(8, (g + 7) / 8 * 8);
     }
-  }
-  row_ptr[n_tot_] = static_cast<int>(k);
-  rb_start.push_back(static_cast<int>(n_tot_));
-  sb_row0.push_back(static_cast<int>(n_tot_));
-  p_sb0[P_] = static_cast<int>(sb_prob.size());
-  p_rb0[P_] = static_cast<int>(rb_prob.size());
-  const int nsb = static_cast<int>(sb_prob.size());
-  const int nrb = static_cast<int>(rb_prob.size());
-  // per-column launch plan over the active prefix; SpMV grids are multiples
-  // of 8 (one share per XCD)
-  plan_.np.assign(M_, 0);
-  plan_.nsb.assign(M_, 0);
-  plan_.nrb.assign(M_, 0);
-  plan_.spmv_grid.assign(M_, 8);
-  for (int j = 0; j < M_; ++j) {
-    int np = 0;
-    while (np < P_ && probs[np].m > j) ++np;
-    plan_.np[j] = np;
-    plan_.nsb[j] = p_sb0[np];
-    plan_.nrb[j] = p_rb0[np];
-    const int g = std::min(plan_.nrb[j], 2048);
-    plan_.spmv_grid[j] = std::max(8, (g + 7) / 8 * 8);
-  }
 
-  pint_init_.assign(static_cast<size_t>(P_) * kPInt, 0);
-  int any_expokit = 0;
-  pscal_init_.assign(static_cast<size_t>(P_) * kPScal, 0.0);
-  for (int p = 0; p < P_; ++p) {
-    pint_init_[p * kPInt + kBM] = probs[p].m;
-    pint_init_[p * kPInt + kBMaxRestarts] = probs[p].max_restarts;
-    pint_init_[p * kPInt + kBSubsteps] = probs[p].substeps;
-    pint_init_[p * kPInt + kBCtrl] = probs[p].control;
-    pscal_init_[p * kPScal + kBT] = probs[p].t;
-    pscal_init_[p * kPScal + kBTol] = probs[p].tol;
-    pscal_init_[p * kPScal + kBAnorm] = probs[p].A->norm_inf();
-    if (probs[p].control >= 1) any_expokit = 1;
-  }
+    pint_init_.assign(static_cast<size_t>(P_) * kPInt, 0);
+    int any_expokit = 0;
+    pscal_init_.assign(static_cast<size_t>(P_) * kPScal, 0.0);
+    for (int p = 0; p < P_; ++p) {
+      pint_init_[p * kPInt + kBM] = probs[p].m;
+      pint_init_[p * kPInt + kBMaxRestarts] = probs[p].max_restarts;
+      pint_init_[p * kPInt + kBSubsteps] = probs[p].substeps;
+      pint_init_[p * kPInt + kBCtrl] = probs[p].control;
+      pscal_init_[p * kPScal + kBT] = probs[p].t;
+      pscal_init_[p * kPScal + kBTol] = probs[p].tol;
+      pscal_init_[p * kPScal + kBAnorm] = probs[p].A->norm_inf();
+      if (probs[p].control >= 1) any_expokit = 1;
+    }
 
-  // ---- device
-  MEK_HIP(hipStreamCreate(&stream_));
-  d_row_ptr_ = upload(row_ptr);
-  d_col_idx_ = upload(col_idx);
-  if (values_f32) {
-    std::vector<float> f(vals.begin(), vals.end());
-    d_vals_ = upload(f);
-  } else {
-    d_vals_ = upload(vals);
-  }
-  d_rb_start_ = upload(rb_start);
-  d_rb_prob_ = upload(rb_prob);
-  d_sb_row0_ = upload(sb_row0);
-  d_sb_prob_ = upload(sb_prob);
-  d_p_sb0_ = upload(p_sb0);
-  const long long ld = (n_tot_ + 63) / 64 * 64;
-  const size_t ncols = static_cast<size_t>(M_) + 2;
-  MEK_HIP(hipMalloc(&d_V_, sizeof(double) * ld * ncols));
-  MEK_HIP(hipMalloc(&d_w_, sizeof(double) * n_tot_));
-  MEK_HIP(hipMalloc(&d_y_, sizeof(double) * n_tot_));
-  MEK_HIP(hipMalloc(&d_v0_, sizeof(double) * n_tot_));
-  const size_t per = 3 * ncols + (ncols) * (ncols) + (M_ + 4);
-  MEK_HIP(hipMalloc(&d_small_, sizeof(double) * per * P_));
-  MEK_HIP(hipMemset(d_small_, 0, sizeof(double) * per * P_));
-  MEK_HIP(hipMalloc(&d_pscal_, sizeof(double) * kPScal * P_));
-  MEK_HIP(hipMalloc(&d_pint_, sizeof(int) * kPInt * P_));
-  MEK_HIP(hipMalloc(&d_partials_, sizeof(double) * (ncols + 1) * nsb));
-  const int ord = any_expokit ? M_ + 2 : M_;  // largest exponential order
-  if (ord > kExpmMaxKLds) MEK_HIP(hipMalloc(&d_expm_ws_, sizeof(double) * 6 * ord * ord * P_));
-  MEK_HIP(hipMalloc(&d_counters_, sizeof(int) * 4));
-  MEK_HIP(hipMemset(d_counters_, 0, sizeof(int) * 4));
-  MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), sizeof(int) * 4));
+    // ---- device
+    MEK_HIP(hipStreamCreate(&stream_));
+    d_row_ptr_ = upload(row_ptr);
+    d_col_idx_ = upload(col_idx);
+    if (values_f32) {
+      std::vector<float> f(vals.begin(), vals.end());
+      d_vals_ = upload(f);
+    } else {
+      d_vals_ = upload(vals);
+    }
+    d_rb_start_ = upload(rb_start);
+    d_rb_prob_ = upload(rb_prob);
+    d_sb_row0_ = upload(sb_row0);
+    d_sb_prob_ = upload(sb_prob);
+    d_p_sb0_ = upload(p_sb0);
+    const long long ld = (n_tot_ + 63) / 64 * 64;
+    const size_t ncols = static_cast<size_t>(M_) + 2;
+    MEK_HIP(hipMalloc(&d_V_, sizeof(double) * ld * ncols));
+    MEK_HIP(hipMalloc(&d_w_, sizeof(double) * n_tot_));
+    MEK_HIP(hipMalloc(&d_y_, sizeof(double) * n_tot_));
+    MEK_HIP(hipMalloc(&d_v0_, sizeof(double) * n_tot_));
+    const size_t per = 3 * ncols + (ncols) * (ncols) + (M_ + 4);
+    MEK_HIP(hipMalloc(&d_small_, sizeof(double) * per * P_));
+    MEK_HIP(hipMemset(d_small_, 0, sizeof(double) * per * P_));
+    MEK_HIP(hipMalloc(&d_pscal_, sizeof(double) * kPScal * P_));
+    MEK_HIP(hipMalloc(&d_pint_, sizeof(int) * kPInt * P_));
+    MEK_HIP(hipMalloc(&d_partials_, sizeof(double) * (ncols + 1) * nsb));
+    const int ord = any_expokit ? M_ + 2 : M_;  // largest exponential order
+    if (ord > kExpmMaxKLds) MEK_HIP(hipMalloc(&d_expm_ws_, sizeof(double) * 6 * ord * ord * P_));
+    MEK_HIP(hipMalloc(&d_counters_, sizeof(int) * 4));
+    MEK_HIP(hipMemset(d_counters_, 0, sizeof(int) * 4));
+    MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_counters_), sizeof(int) * 4));
 
-  auto* b = new BatchBufs{};
-  b->P = P_;
-  b->M = M_;
-  b->row_ptr = d_row_ptr_;
-  b->col_idx = d_col_idx_;
-  b->vals = d_vals_;
-  b->vals_f32 = values_f32 ? 1 : 0;
-  b->rb_start = d_rb_start_;
-  b->rb_prob = d_rb_prob_;
-  b->nrb = nrb;
-  b->sb_prob = d_sb_prob_;
-  b->sb_row0 = d_sb_row0_;
-  b->nsb = nsb;
-  b->p_sb0 = d_p_sb0_;
-  b->V = d_V_;
-  b->ld = ld;
-  b->w = d_w_;
-  b->y = d_y_;
-  b->colscale = d_small_;
-  b->h1 = b->colscale + ncols * P_;
-  b->h2 = b->h1 + ncols * P_;
-  b->H = b->h2 + ncols * P_;
-  b->wH = b->H + ncols * ncols * P_;
-  b->pscal = d_pscal_;
-  b->pint = d_pint_;
-  b->partials = d_partials_;
-  b->expm_ws = d_expm_ws_;
-  b->counters = d_counters_;
-  b->any_expokit = any_expokit;
-  bufs_ = b;
+    auto* b = new BatchBufs{};
+    b->P = P_;
+    b->M = M_;
+    b->row_ptr = d_row_ptr_;
+    b->col_idx = d_col_idx_;
+    b->vals = d_vals_;
+    b->vals_f32 = values_f32 ? 1 : 0;
+    b->rb_start = d_rb_start_;
+    b->rb_prob = d_rb_prob_;
+    b->nrb = nrb;
+    b->sb_prob = d_sb_prob_;
+    b->sb_row0 = d_sb_row0_;
+    b->nsb = nsb;
+    b->p_sb0 = d_p_sb0_;
+    b->V = d_V_;
+    b->ld = ld;
+    b->w = d_w_;
+    b->y = d_y_;
+    b->colscale = d_small_;
+    b->h1 = b->colscale + ncols * P_;
+    b->h2 = b->h1 + ncols * P_;
+    b->H = b->h2 + ncols * P_;
+    b->wH = b->H + ncols * ncols * P_;
+    b->pscal = d_pscal_;
+    b->pint = d_pint_;
+    b->partials = d_partials_;
+    b->expm_ws = d_expm_ws_;
+    b->counters = d_counters_;
+    b->any_expokit = any_expokit;
+    bufs_ = b;
+  } catch (...) {
+    destroy();
+    throw;
+  }
 }
 
 BatchRunner::~BatchRunner() { destroy(); }
@@ -320,8 +327,9 @@ void BatchRunner::features(float* out) {
   MEK_HIP(e);
 }
 
+// Releases whatever is allocated (also after a constructor that failed part
+// way); safe to call more than once.
 void BatchRunner::destroy() {
-  if (!bufs_) return;
   (void)hipSetDevice(device_);
   for (hipGraphExec_t g : graph_exec_)
     if (g) (void)hipGraphExecDestroy(g);
@@ -329,11 +337,28 @@ void BatchRunner::destroy() {
     if (g) (void)hipGraphDestroy(g);
   graph_exec_.clear();
   graph_.clear();
-  void* ptrs[] = {d_row_ptr_, d_col_idx_, d_rb_start_, d_rb_prob_, d_sb_prob_, d_sb_row0_, d_p_sb0_,
-                  d_pint_, d_counters_, d_vals_, d_V_, d_w_, d_y_, d_v0_, d_small_, d_partials_,
-                  d_expm_ws_, d_pscal_};
-  for (void* p : ptrs)
+  auto release = [](auto*& p) {
     if (p) (void)hipFree(p);
+    p = nullptr;
+  };
+  release(d_row_ptr_);
+  release(d_col_idx_);
+  release(d_rb_start_);
+  release(d_rb_prob_);
+  release(d_sb_prob_);
+  release(d_sb_row0_);
+  release(d_p_sb0_);
+  release(d_pint_);
+  release(d_counters_);
+  release(d_vals_);
+  release(d_V_);
+  release(d_w_);
+  release(d_y_);
+  release(d_v0_);
+  release(d_small_);
+  release(d_partials_);
+  release(d_expm_ws_);
+  release(d_pscal_);
   if (h_counters_) (void)hipHostFree(h_counters_);
   h_counters_ = nullptr;
   if (stream_) (void)hipStreamDestroy(stream_);

@@ -46,60 +46,67 @@ BlockRunner::BlockRunner(const CSRHost& A, int m, double t, double tol, int subs
   if (static_cast<long long>(kRhs) * (m + 1) >= A.rows)
     throw Error("ERROR: 32 (m + 1) must be below the matrix size for a block Krylov basis");
   if (!(t > 0.0)) throw Error("ERROR: t must be positive");
-  MEK_HIP(hipSetDevice(device_));
-  MEK_HIP(hipStreamCreate(&stream_));
-  anorm_ = std::max(A.norm_inf(), 1e-300);
-  n_pad_ = (static_cast<long long>(n_) + kBlkRows - 1) / kBlkRows * kBlkRows;
-  const int nchunks = static_cast<int>(n_pad_ / kBlkRows);
-  const size_t blk = static_cast<size_t>(n_pad_) * kRhs;
-  MEK_HIP(hipMalloc(&d_row_ptr_, sizeof(int) * (n_ + 1)));
-  MEK_HIP(hipMalloc(&d_col_idx_, sizeof(int) * std::max<long long>(nnz_, 1)));
-  MEK_HIP(hipMalloc(&d_vals_, sizeof(double) * std::max<long long>(nnz_, 1)));
-  MEK_HIP(hipMemcpy(d_row_ptr_, A.row_ptr.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice));
-  if (nnz_ > 0) {
-    MEK_HIP(hipMemcpy(d_col_idx_, A.col_idx.data(), sizeof(int) * nnz_, hipMemcpyHostToDevice));
-    MEK_HIP(hipMemcpy(d_vals_, A.values.data(), sizeof(double) * nnz_, hipMemcpyHostToDevice));
-  }
-  MEK_HIP(hipMalloc(&d_P_, sizeof(double) * blk * (m + 1)));
-  MEK_HIP(hipMemset(d_P_, 0, sizeof(double) * blk * (m + 1)));
-  MEK_HIP(hipMalloc(&d_W_, sizeof(double) * blk));
-  MEK_HIP(hipMemset(d_W_, 0, sizeof(double) * blk));
-  const int ldh = kRhs * (m + 1);
-  const size_t b32 = kRhs * kRhs;
-  const size_t small = 3 * b32 * (m + 1) + static_cast<size_t>(ldh) * ldh + 2 * b32 + 16;
-  MEK_HIP(hipMalloc(&d_small_, sizeof(double) * small));
-  MEK_HIP(hipMemset(d_small_, 0, sizeof(double) * small));
-  MEK_HIP(hipMalloc(&d_partials_, sizeof(double) * b32 * nchunks * (m + 1)));
-  MEK_HIP(hipMalloc(&d_flags_, sizeof(int) * 4));
-  MEK_HIP(hipMemset(d_flags_, 0, sizeof(int) * 4));
-  npad_max_ = (kRhs * m + kGemmTile - 1) / kGemmTile * kGemmTile;
-  MEK_HIP(hipMalloc(&d_mat_, sizeof(double) * 7 * static_cast<size_t>(npad_max_) * npad_max_));
-  MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), sizeof(double) * 16));
-  MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_flags_), sizeof(int) * 4));
+  // a failure part way releases what was already allocated (the destructor
+  // does not run for an object whose constructor threw)
+  try {
+    MEK_HIP(hipSetDevice(device_));
+    MEK_HIP(hipStreamCreate(&stream_));
+    anorm_ = std::max(A.norm_inf(), 1e-300);
+    n_pad_ = (static_cast<long long>(n_) + kBlkRows - 1) / kBlkRows * kBlkRows;
+    const int nchunks = static_cast<int>(n_pad_ / kBlkRows);
+    const size_t blk = static_cast<size_t>(n_pad_) * kRhs;
+    MEK_HIP(hipMalloc(&d_row_ptr_, sizeof(int) * (n_ + 1)));
+    MEK_HIP(hipMalloc(&d_col_idx_, sizeof(int) * std::max<long long>(nnz_, 1)));
+    MEK_HIP(hipMalloc(&d_vals_, sizeof(double) * std::max<long long>(nnz_, 1)));
+    MEK_HIP(hipMemcpy(d_row_ptr_, A.row_ptr.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice));
+    if (nnz_ > 0) {
+      MEK_HIP(hipMemcpy(d_col_idx_, A.col_idx.data(), sizeof(int) * nnz_, hipMemcpyHostToDevice));
+      MEK_HIP(hipMemcpy(d_vals_, A.values.data(), sizeof(double) * nnz_, hipMemcpyHostToDevice));
+    }
+    MEK_HIP(hipMalloc(&d_P_, sizeof(double) * blk * (m + 1)));
+    MEK_HIP(hipMemset(d_P_, 0, sizeof(double) * blk * (m + 1)));
+    MEK_HIP(hipMalloc(&d_W_, sizeof(double) * blk));
+    MEK_HIP(hipMemset(d_W_, 0, sizeof(double) * blk));
+    const int ldh = kRhs * (m + 1);
+    const size_t b32 = kRhs * kRhs;
+    const size_t small = 3 * b32 * (m + 1) + static_cast<size_t>(ldh) * ldh + 2 * b32 + 16;
+    MEK_HIP(hipMalloc(&d_small_, sizeof(double) * small));
+    MEK_HIP(hipMemset(d_small_, 0, sizeof(double) * small));
+    MEK_HIP(hipMalloc(&d_partials_, sizeof(double) * b32 * nchunks * (m + 1)));
+    MEK_HIP(hipMalloc(&d_flags_, sizeof(int) * 4));
+    MEK_HIP(hipMemset(d_flags_, 0, sizeof(int) * 4));
+    npad_max_ = (kRhs * m + kGemmTile - 1) / kGemmTile * kGemmTile;
+    MEK_HIP(hipMalloc(&d_mat_, sizeof(double) * 7 * static_cast<size_t>(npad_max_) * npad_max_));
+    MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), sizeof(double) * 16));
+    MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_flags_), sizeof(int) * 4));
 
-  auto* b = new BlockBufs{};
-  b->n = n_;
-  b->n_pad = n_pad_;
-  b->row_ptr = d_row_ptr_;
-  b->col_idx = d_col_idx_;
-  b->vals = d_vals_;
-  b->P = d_P_;
-  b->W = d_W_;
-  b->Y = d_W_;
-  b->S = d_small_;
-  b->G = b->S + b32 * (m + 1);
-  b->C = b->G + b32 * (m + 1);
-  b->Hb = b->C + b32 * (m + 1);
-  b->ldh = ldh;
-  b->R0 = b->Hb + static_cast<size_t>(ldh) * ldh;
-  b->Gm = b->R0 + b32;
-  b->scal = b->Gm + b32;
-  b->partials = d_partials_;
-  b->nchunks = nchunks;
-  b->flags = d_flags_;
-  b->spmm_grid = 4096;
-  b->update_grid = 4096;
-  bufs_ = b;
+    auto* b = new BlockBufs{};
+    b->n = n_;
+    b->n_pad = n_pad_;
+    b->row_ptr = d_row_ptr_;
+    b->col_idx = d_col_idx_;
+    b->vals = d_vals_;
+    b->P = d_P_;
+    b->W = d_W_;
+    b->Y = d_W_;
+    b->S = d_small_;
+    b->G = b->S + b32 * (m + 1);
+    b->C = b->G + b32 * (m + 1);
+    b->Hb = b->C + b32 * (m + 1);
+    b->ldh = ldh;
+    b->R0 = b->Hb + static_cast<size_t>(ldh) * ldh;
+    b->Gm = b->R0 + b32;
+    b->scal = b->Gm + b32;
+    b->partials = d_partials_;
+    b->nchunks = nchunks;
+    b->flags = d_flags_;
+    b->spmm_grid = 4096;
+    b->update_grid = 4096;
+    bufs_ = b;
+  } catch (...) {
+    destroy();
+    throw;
+  }
 }
 
 // Time the SpMM and update-sweep grids on this matrix (the start block is in
@@ -375,17 +382,28 @@ void BlockRunner::expmv_device(const double* d_V, double* d_Y) {
   stats_.solve_ms = now_ms() - t0;
 }
 
+// Releases whatever is allocated (also after a constructor that failed part
+// way); safe to call more than once.
 void BlockRunner::destroy() {
-  if (!bufs_) return;
   (void)hipSetDevice(device_);
   for (Ev& e : ev_) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);
   }
   ev_.clear();
-  void* ptrs[] = {d_row_ptr_, d_col_idx_, d_vals_, d_P_, d_W_, d_small_, d_partials_, d_flags_, d_mat_};
-  for (void* p : ptrs)
+  auto release = [](auto*& p) {
     if (p) (void)hipFree(p);
+    p = nullptr;
+  };
+  release(d_row_ptr_);
+  release(d_col_idx_);
+  release(d_vals_);
+  release(d_P_);
+  release(d_W_);
+  release(d_small_);
+  release(d_partials_);
+  release(d_flags_);
+  release(d_mat_);
   if (h_pinned_) (void)hipHostFree(h_pinned_);
   if (h_flags_) (void)hipHostFree(h_flags_);
   h_pinned_ = nullptr;
