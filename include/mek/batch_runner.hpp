@@ -48,9 +48,11 @@ class BatchRunner {
 
   int num_problems() const { return P_; }
   long long total_rows() const { return n_tot_; }
-  // rows of problem p (the caller's index) in the batch's global row space
+  // rows of problem p (the caller's index) in the batch's global row space;
+  // each problem takes an even number of rows there (one empty padding row
+  // after an odd count), so that the sweeps can take two rows per lane
   int row_offset(int p) const { return row0_[ext2int_[p]]; }
-  int rows(int p) const { return row0_[ext2int_[p] + 1] - row0_[ext2int_[p]]; }
+  int rows(int p) const { return nrows_[ext2int_[p]]; }
 
   // Start vectors (host, one array per problem) into the device copy of v.
   void upload_v(const std::vector<const double*>& v);
@@ -72,7 +74,8 @@ class BatchRunner {
   int P_ = 0;
   int M_ = 0;
   long long n_tot_ = 0;
-  std::vector<int> row0_;
+  std::vector<int> row0_;   // padded offsets (even), P + 1 entries
+  std::vector<int> nrows_;  // rows of each problem
   int device_ = 0;
   bool use_graph_ = true;
   int gs_mode_ = 1;

@@ -53,6 +53,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     for (int i = 0; i < P_; ++i) probs[i] = probs_in[order[i]];
     // ---- host assembly: block-diagonal CSR, row blocks, streaming blocks
     row0_.assign(P_ + 1, 0);
+    nrows_.assign(P_, 0);
     long long nnz_tot = 0;
     for (int p = 0; p < P_; ++p) {
       const CSRHost& A = *probs[p].A;
@@ -60,7 +61,10 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
       if (probs[p].m < 1 || probs[p].m > kBatchMaxM || probs[p].m >= A.rows)
         throw Error("ERROR: batch problem " + std::to_string(p) + ": m=" + std::to_string(probs[p].m) +
                     " outside [1, min(" + std::to_string(kBatchMaxM) + ", rows-1)]");
-      row0_[p + 1] = row0_[p] + A.rows;
+      // an odd row count gets one empty padding row: every problem starts on
+      // an even row, as the two-rows-per-lane sweeps need
+      nrows_[p] = A.rows;
+      row0_[p + 1] = row0_[p] + A.rows + (A.rows & 1);
       nnz_tot += A.nnz;
       M_ = std::max(M_, probs[p].m);
     }
@@ -79,6 +83,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
         vals[k + e] = A.values[e];
       }
       k += A.nnz;
+      if (A.rows & 1) row_ptr[r0 + A.rows] = static_cast<int>(k);  // the empty padding row
       const std::vector<int> rb = build_row_blocks(A.row_ptr.data(), A.rows, kBatchNnzCap, kSpmvRowCap);
       p_rb0[p] = static_cast<int>(rb_prob.size());
       for (size_t i = 0; i + 1 < rb.size(); ++i) {
@@ -86,7 +91,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
         rb_prob.push_back(p);
       }
       p_sb0[p] = static_cast<int>(sb_prob.size());
-      for (int r = 0; r < A.rows; r += kBatchRowsPerBlock) {
+      for (int r = 0; r < row0_[p + 1] - r0; r += kBatchRowsPerBlock) {
         sb_row0.push_back(r0 + r);
         sb_prob.push_back(p);
       }
@@ -149,6 +154,12 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     MEK_HIP(hipMalloc(&d_w_, sizeof(double) * n_tot_));
     MEK_HIP(hipMalloc(&d_y_, sizeof(double) * n_tot_));
     MEK_HIP(hipMalloc(&d_v0_, sizeof(double) * n_tot_));
+    // padding rows stay zero in every vector: no SpMV row block covers them,
+    // and the sweeps then compute zero there from zero inputs
+    MEK_HIP(hipMemset(d_V_, 0, sizeof(double) * ld * ncols));
+    MEK_HIP(hipMemset(d_w_, 0, sizeof(double) * n_tot_));
+    MEK_HIP(hipMemset(d_y_, 0, sizeof(double) * n_tot_));
+    MEK_HIP(hipMemset(d_v0_, 0, sizeof(double) * n_tot_));
     const size_t per = 3 * ncols + (ncols) * (ncols) + (M_ + 4);
     MEK_HIP(hipMalloc(&d_small_, sizeof(double) * per * P_));
     MEK_HIP(hipMemset(d_small_, 0, sizeof(double) * per * P_));
@@ -299,6 +310,9 @@ void BatchRunner::features(float* out) {
     tol[p] = pscal_init_[p * kPScal + kBTol];
   }
   int* d_p0 = upload(row0_);
+  std::vector<int> row1(P_);
+  for (int p = 0; p < P_; ++p) row1[p] = row0_[p] + nrows_[p];  // without the padding row
+  int* d_p1 = upload(row1);
   double* d_t = upload(t);
   double* d_tol = upload(tol);
   double *d_col = nullptr, *d_stats = nullptr;
@@ -309,8 +323,8 @@ void BatchRunner::features(float* out) {
   if (e == hipSuccess) e = hipMemsetAsync(d_col, 0, sizeof(double) * n_tot_, stream_);
   if (e == hipSuccess) {
     const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
-    feature_launch(d_row_ptr_, d_col_idx_, d_vals_, b.vals_f32, d_p0, P_, d_col, d_stats, d_t, d_tol, d_f,
-                   stream_);
+    feature_launch(d_row_ptr_, d_col_idx_, d_vals_, b.vals_f32, d_p0, d_p1, P_, d_col, d_stats, d_t, d_tol,
+                   d_f, stream_);
     e = hipGetLastError();
   }
   std::vector<float> f_int(static_cast<size_t>(kNumFeatures) * P_);
@@ -321,7 +335,7 @@ void BatchRunner::features(float* out) {
     for (int p = 0; p < P_; ++p)
       std::memcpy(out + static_cast<size_t>(p) * kNumFeatures,
                   f_int.data() + static_cast<size_t>(ext2int_[p]) * kNumFeatures, sizeof(float) * kNumFeatures);
-  for (void* p : {static_cast<void*>(d_p0), static_cast<void*>(d_t), static_cast<void*>(d_tol),
+  for (void* p : {static_cast<void*>(d_p0), static_cast<void*>(d_p1), static_cast<void*>(d_t), static_cast<void*>(d_tol),
                   static_cast<void*>(d_col), static_cast<void*>(d_stats), static_cast<void*>(d_f)})
     if (p) (void)hipFree(p);
   MEK_HIP(e);

@@ -192,7 +192,7 @@ void features_csr(const CSRHost& A, double t, double tol, float* out, int device
     MEK_HIP(hipMemset(d_col, 0, sizeof(double) * std::max(n, 1)));
     MEK_HIP(hipMemcpy(d_p0, p0, sizeof(p0), hipMemcpyHostToDevice));
     MEK_HIP(hipMemcpy(d_tt, tt, sizeof(tt), hipMemcpyHostToDevice));
-    feature_launch(d_rp, d_ci, d_v, 0, d_p0, 1, d_col, d_stats, d_tt, d_tt + 1, d_f, nullptr);
+    feature_launch(d_rp, d_ci, d_v, 0, d_p0, d_p0 + 1, 1, d_col, d_stats, d_tt, d_tt + 1, d_f, nullptr);
     MEK_HIP_LAUNCHED();
     MEK_HIP(hipMemcpy(out, d_f, sizeof(float) * kNumFeatures, hipMemcpyDeviceToHost));
   } catch (...) {

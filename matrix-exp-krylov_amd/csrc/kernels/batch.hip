@@ -4,7 +4,8 @@
 // The matrices are stored as one block-diagonal CSR (columns offset to global
 // rows), so one SpMV launch multiplies every active problem at once.  The
 // basis V is one global column-major array; column j of problem p is rows
-// [row0_p, row0_p + n_p) of column j.  Streaming kernels run over "streaming
+// [row0_p, row0_p + n_p) of column j; every row0_p is even (an odd n_p is
+// followed by one empty row, zero in every vector).  Streaming kernels run over "streaming
 // blocks", each a row range of exactly one problem, so every block's partial
 // sums belong to one problem; per-problem finalisers (one workgroup each)
 // reduce them in block order.  The reference controller
@@ -313,9 +314,10 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, in
   }
 }
 
-// Streaming blocks run the column-split sweeps over their own row range.
+// Streaming blocks run the column-split sweeps over their own row range, two
+// rows per lane (the range starts and ends on an even row).
 __device__ __forceinline__ TileRange sb_tiles(const BatchBufs& b, int sb) {
-  return block_tiles(b.sb_row0[sb], b.sb_row0[sb + 1]);
+  return block_tiles<2>(b.sb_row0[sb], b.sb_row0[sb + 1]);
 }
 
 // partials of q_i . w (i <= j) and ||w||^2 (quantity j+1) per streaming block
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void kb_dots1(BatchBufs b, int j) {
   const int sb = blockIdx.x;
   const int p = b.sb_prob[sb];
   if (!col_active(b, p, j)) return;
-  sweep_dots<MAXK>(b.V, b.ld, 0, j + 1, pcs(b, p), b.w, sb_tiles(b, sb), true, b.partials, b.nsb, sb);
+  sweep_dots<MAXK, 2>(b.V, b.ld, 0, j + 1, pcs(b, p), b.w, sb_tiles(b, sb), true, b.partials, b.nsb, sb);
 }
 
 // w' = w - sum h1_i q_i into V[:, j+1]; partials of q_i . w' and ||w'||^2
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void kb_update_dot(BatchBufs b, int j) {
   const int sb = blockIdx.x;
   const int p = b.sb_prob[sb];
   if (!col_active(b, p, j)) return;
-  sweep_update_dot<MAXK>(b.V, b.ld, j + 1, b.h1 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
+  sweep_update_dot<MAXK, 2>(b.V, b.ld, j + 1, b.h1 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
                          b.w, b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb), b.partials,
                          b.nsb, sb);
 }
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void kb_update_norm(BatchBufs b, int j) {
   const int sb = blockIdx.x;
   const int p = b.sb_prob[sb];
   if (!col_active(b, p, j) || !pi(b, p)[kBReorth]) return;
-  sweep_update_norm<MAXK>(b.V, b.ld, j + 1, b.h2 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
+  sweep_update_norm<MAXK, 2>(b.V, b.ld, j + 1, b.h2 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
                           b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb), b.partials, sb);
 }
 
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void kb_lift(BatchBufs b) {
     if (threadIdx.x == 0) b.partials[sb] = 0.0;
     return;
   }
-  sweep_lift<MAXK>(b.V, b.ld, nu, used, coef, b.y, sb_tiles(b, sb), b.partials, sb);
+  sweep_lift<MAXK, 2>(b.V, b.ld, nu, used, coef, b.y, sb_tiles(b, sb), b.partials, sb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_fin_lift(BatchBufs b) {

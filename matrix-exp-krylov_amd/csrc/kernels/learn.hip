@@ -111,16 +111,16 @@ __global__ void k_gather(const float* X, const float* T, const int* idx, float* 
 
 // ------------------------------------------------------------ features
 
-// Per-problem row statistics over rows [p_row0[p], p_row0[p+1]) of a
+// Per-problem row statistics over rows [p_row0[p], p_row1[p]) of a
 // (block-diagonal) CSR whose columns are global; one workgroup per problem.
 // Column absolute sums go to colsum (zeroed by the caller) for ||A||_1.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_feat_rows(const int* row_ptr, const int* col_idx,
                                                       const void* vals, const int* p_row0,
-                                                      double* colsum, double* stats) {
+                                                      const int* p_row1, double* colsum, double* stats) {
   __shared__ double red[kWavesPerBlock][8];
   const int p = blockIdx.x;
-  const int r0 = p_row0[p], r1 = p_row0[p + 1];
+  const int r0 = p_row0[p], r1 = p_row1[p];
   double s_nnz = 0, s_nnz2 = 0, mx_nnz = 0, mx_rowabs = 0, mx_right = -1e308, mn_left = 1e308,
          mx_diag = 0, s_dom = 0;
   const VT* v = static_cast<const VT*>(vals);
@@ -164,12 +164,13 @@ __global__ __launch_bounds__(kBlock) void k_feat_rows(const int* row_ptr, const 
 
 // The 16 features of problem p (see learn.hpp) from the row statistics and
 // the max column sum over the problem's columns.
-__global__ __launch_bounds__(kBlock) void k_feat_final(const int* p_row0, const double* colsum,
+__global__ __launch_bounds__(kBlock) void k_feat_final(const int* p_row0, const int* p_row1,
+                                                       const double* colsum,
                                                        const double* stats, const double* t,
                                                        const double* tol, float* feat) {
   __shared__ double red[kWavesPerBlock];
   const int p = blockIdx.x;
-  const int r0 = p_row0[p], r1 = p_row0[p + 1];
+  const int r0 = p_row0[p], r1 = p_row1[p];
   double mx = 0.0;
   for (int c = r0 + threadIdx.x; c < r1; c += kBlock) mx = fmax(mx, colsum[c]);
   mx = wave_max(mx);
@@ -235,15 +236,15 @@ void mlp_gather(const float* X, const float* T, const int* idx, float* Xb, float
   hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, X, T, idx, Xb, Tb, B, dx, dt);
 }
 void feature_launch(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32,
-                    const int* p_row0, int P, double* colsum, double* stats, const double* t,
-                    const double* tol, float* feat, hipStream_t s) {
+                    const int* p_row0, const int* p_row1, int P, double* colsum, double* stats,
+                    const double* t, const double* tol, float* feat, hipStream_t s) {
   if (vals_f32)
     hipLaunchKernelGGL(k_feat_rows<float>, dim3(P), dim3(kBlock), 0, s, row_ptr, col_idx, vals, p_row0,
-                       colsum, stats);
+                       p_row1, colsum, stats);
   else
     hipLaunchKernelGGL(k_feat_rows<double>, dim3(P), dim3(kBlock), 0, s, row_ptr, col_idx, vals, p_row0,
-                       colsum, stats);
-  hipLaunchKernelGGL(k_feat_final, dim3(P), dim3(kBlock), 0, s, p_row0, colsum, stats, t, tol, feat);
+                       p_row1, colsum, stats);
+  hipLaunchKernelGGL(k_feat_final, dim3(P), dim3(kBlock), 0, s, p_row0, p_row1, colsum, stats, t, tol, feat);
 }
 
 }  // namespace mek

@@ -23,10 +23,10 @@ void mlp_adam(float* p, const float* g, float* m, float* v, int n, float lr, flo
               int step, hipStream_t s);
 void mlp_gather(const float* X, const float* T, const int* idx, float* Xb, float* Tb, int B, int dx, int dt,
                 hipStream_t s);
-// 16 features per problem for rows [p_row0[p], p_row0[p+1]) of a
+// 16 features per problem for rows [p_row0[p], p_row1[p]) of a
 // block-diagonal CSR; colsum (n_total doubles) must be zero on entry.
 void feature_launch(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32,
-                    const int* p_row0, int P, double* colsum, double* stats, const double* t,
-                    const double* tol, float* feat, hipStream_t s);
+                    const int* p_row0, const int* p_row1, int P, double* colsum, double* stats,
+                    const double* t, const double* tol, float* feat, hipStream_t s);
 
 }  // namespace mek

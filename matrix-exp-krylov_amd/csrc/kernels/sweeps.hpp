@@ -42,8 +42,9 @@ __device__ __forceinline__ TileRange grid_tiles(long long n) {
                    static_cast<long long>(gridDim.x), n};
 }
 
+template <int R = 1>
 __device__ __forceinline__ TileRange block_tiles(long long row0, long long row1) {
-  return TileRange{row0, 0, (row1 - row0 + kWave - 1) / kWave, 1, row1};
+  return TileRange{row0, 0, (row1 - row0 + kWave * R - 1) / (kWave * R), 1, row1};
 }
 
 // R rows of one column starting at p (non-temporal; p 16-byte aligned for R = 2)
