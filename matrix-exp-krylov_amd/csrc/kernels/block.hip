@@ -302,32 +302,40 @@ void block_launch_dots(const BlockBufs& b, const double* X1, int nb, const doubl
 // block the wave's 64 x 32 slab of P_i (16 KB, contiguous) is loaded with
 // fully coalesced 16-byte loads into LDS (rows padded to 33 doubles against
 // bank conflicts), and the MFMA A operands are read from there; G_i is loaded
-// once as eight 4 x 32 B-operand slabs.  Each wave's slab is private, so the
-// LDS stores and loads are ordered within the wave and the waves of a
-// workgroup never wait for each other.  In place (Out == X) is allowed:
-// every element is read and written by one lane.
+// once as eight 4 x 32 B-operand slabs.  The loads of block i + 1's slab are
+// issued as soon as block i's slab is in LDS, so they are in flight during
+// block i's MFMAs (the MFMA time is about half the load time at this
+// arithmetic intensity, 8 flop per byte).  Two waves per SIMD: the registers
+// (slab, accumulators, B operands) fit 256 VGPRs, and LDS allows two
+// workgroups per CU anyway.  Each wave's slab is private, so the LDS stores
+// and loads are ordered within the wave and the waves of a workgroup never
+// wait for each other.  In place (Out == X) is allowed: every element is read
+// and written by one lane.
 constexpr int kUpdPad = kRhs + 1;
 
-__global__ __launch_bounds__(kBlock) void k_bupdate(BlockBufs b, int nb, const double* X, double* Out,
-                                                    int cond) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_bupdate(
+    BlockBufs b, int nb, const double* X, double* Out, int cond) {
   if (b.flags[0] || (cond && !b.flags[2])) return;
   __shared__ double tile[kWavesPerBlock][64 * kUpdPad];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   const int lr = lane / 16, lc = lane % 16;
   const long long ntile256 = b.n_pad / 256;
+  const long long bstride = b.n_pad * kRhs;  // doubles of one basis block
   double* my = tile[wave];
   for (long long T = blockIdx.x; T < ntile256; T += gridDim.x) {
     const long long rbase = T * 256 + 64LL * wave;
     dbl4 acc[4][2];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = zero4();
+    const double* __restrict__ src = b.P + rbase * kRhs + 2 * lane;
+    double2 v[16];  // the slab of block i, lane-interleaved
+    if (nb > 0) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = *reinterpret_cast<const double2*>(src + u * 128);
+    }
     for (int i = 0; i < nb; ++i) {
       const double* __restrict__ Gi = b.G + static_cast<long long>(i) * kRhs * kRhs;
-      const double* __restrict__ src = b.P + static_cast<long long>(i) * b.n_pad * kRhs + rbase * kRhs;
-      double2 v[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) v[u] = *reinterpret_cast<const double2*>(src + u * 128 + 2 * lane);
       double g0[8], g1[8];
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks) {
@@ -340,6 +348,11 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(BlockBufs b, int nb, const d
         const int row = e / kRhs, col = e % kRhs;
         my[row * kUpdPad + col] = v[u].x;
         my[row * kUpdPad + col + 1] = v[u].y;
+      }
+      if (i + 1 < nb) {
+        const double* __restrict__ nsrc = src + (i + 1) * bstride;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = *reinterpret_cast<const double2*>(nsrc + u * 128);
       }
       wave_lds_sync();
 #pragma unroll
