@@ -154,8 +154,8 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     MEK_HIP(hipMalloc(&d_w_, sizeof(double) * n_tot_));
     MEK_HIP(hipMalloc(&d_y_, sizeof(double) * n_tot_));
     MEK_HIP(hipMalloc(&d_v0_, sizeof(double) * n_tot_));
-    // padding rows stay zero in every vector: no SpMV row block covers them,
-    // and the sweeps then compute zero there from zero inputs
+    // padding rows stay zero in every vector: the SpMV writes zero there (an
+    // empty row) or leaves them alone, and the sweeps compute zero from zeros
     MEK_HIP(hipMemset(d_V_, 0, sizeof(double) * ld * ncols));
     MEK_HIP(hipMemset(d_w_, 0, sizeof(double) * n_tot_));
     MEK_HIP(hipMemset(d_y_, 0, sizeof(double) * n_tot_));
