@@ -118,3 +118,44 @@ def test_batch_autotune_same_result(gpu):
         B.close()
     for a, b in zip(ys[False], ys[True]):
         np.testing.assert_array_equal(a, b)
+
+
+def _rows_edited(n, seed, dense=(), empty=()):
+    """random_rows with some rows replaced by a dense row (n nonzeros, small
+    values) or by an empty row."""
+    rp, ci, va = gen.random_rows(n, 5, 30, seed=seed)
+    rows = [list(zip(ci[rp[r]:rp[r + 1]], va[rp[r]:rp[r + 1]])) for r in range(n)]
+    rng = np.random.default_rng(seed)
+    for r in dense:
+        rows[r] = [(c, float(x)) for c, x in zip(range(n), rng.uniform(-0.01, 0.01, n))]
+    for r in empty:
+        rows[r] = []
+    new_rp = np.zeros(n + 1, np.int32)
+    new_rp[1:] = np.cumsum([len(r) for r in rows])
+    new_ci = np.array([c for r in rows for c, _ in r], np.int32)
+    new_va = np.array([x for r in rows for _, x in r], np.float64)
+    return new_rp, new_ci, new_va
+
+
+def test_batch_odd_rows_long_and_empty_rows(gpu, oracle):
+    """Odd row counts (each problem is followed by one empty padding row in
+    the batch's row space), a dense last row (a long-row SpMV block that ends
+    on the padding row), a dense row inside, and an empty last row."""
+    mats = [_rows_edited(2501, 1, dense=[2500]),
+            _rows_edited(1999, 2, empty=[1998]),
+            _rows_edited(3001, 3, dense=[7], empty=[11]),
+            _rows_edited(1200, 4)]
+    ts = [0.2, 0.5, 0.3, 1.0]
+    ms = [30, 20, 25, 12]
+    vs = [gen.normal_vector(r[0].size - 1, seed=10 + i) for i, r in enumerate(mats)]
+    tol = 1e-9
+    B = mek.Batch(mats, ts, ms, tol, max_restarts=10)
+    B.upload(vs)
+    B.solve()
+    ys = B.download()
+    for p, ((rp, ci, va), t, m, v, y) in enumerate(zip(mats, ts, ms, vs, ys)):
+        assert y.shape == v.shape, p
+        yref, st, _ = oracle.expmv(rp, ci, va, v, t, tol, m, 10)
+        assert B.stats(p)["converged"] == st["converged"] == 1, p
+        assert rel_err(y, yref) <= 1e-8, (p, rel_err(y, yref))
+    B.close()
