@@ -21,9 +21,13 @@
  *                     start (happy breakdown), 2-D Laplacian via DST-I;
  *                     scipy.sparse.linalg.expm_multiply fixtures on random
  *                     Gershgorin-negative matrices
- *   controller trace  (dt sequence, accept/reject) -- parity unpinned: only
- *                     the reference source text fixes it and the reference
- *                     cannot run here.
+ *   controller trace  (dt sequence, accept/reject) -- closed-form traces
+ *                     with m = 1 on 2 x 2 diagonal matrices (err from h11 and
+ *                     h21 in closed form; the step rules restated a second
+ *                     time from main_cpu.cpp:602-680) cover the grow,
+ *                     shrink-and-cap and abort branches; for m > 1 only the
+ *                     reference source text fixes the trace and the reference
+ *                     cannot run here, so there it stays unpinned.
  */
 #ifndef MEK_ORACLE_H
 #define MEK_ORACLE_H

@@ -47,3 +47,63 @@ def rel_err(y, ref) -> float:
     if not np.isfinite(s) or s == 0.0:
         s = 1.0
     return float(np.linalg.norm((y - ref) / s) / np.linalg.norm(ref / s))
+
+
+def diag2_m1_trace(lam1: float, lam2: float, a: float, b: float, t: float, tol: float,
+                   max_restarts: int):
+    """Step trace and result of the reference controller with m = 1 on
+    A = diag(lam1, lam2), v = (a, b), from closed forms only.
+
+    With one Krylov column, q1 = v / beta never changes direction: the lift is
+    y = beta e^{dt h11} q1, so a restart only rescales beta.  With p = q1^2,
+    h11 = lam1 p1 + lam2 p2 and h21 = |lam1 - lam2| sqrt(p1 p2), and the
+    residual estimate |h21 [exp(dt H)e1]_1| beta of main.cu:2004-2020 is
+    h21 e^{dt h11} beta.  The step rules (first step t / max_restarts, accept
+    when err <= tol, grow by 1.5 * 0.8 when err < 0.1 tol, shrink by
+    0.5 * 0.8 on reject, abort after 20 rejects, at most max_restarts accepted
+    steps) are those of main.cu:1411-1518 / main_cpu.cpp:602-680.
+
+    Returns (records, y, stats): records hold t, dt, err, accepts, rejects and
+    the decision margin |err - threshold| / threshold of each attempt.
+    """
+    beta = float(np.hypot(a, b))
+    q = np.array([a, b]) / beta
+    p1, p2 = q[0] ** 2, q[1] ** 2
+    h11 = lam1 * p1 + lam2 * p2
+    h21 = abs(lam1 - lam2) * np.sqrt(p1 * p2)
+    dt_max, dt_min = t, max(1e-12, t * 1e-6)
+    dt = min(max(t / (max_restarts if max_restarts > 0 else 1), dt_min), dt_max)
+    max_steps = max_restarts if max_restarts > 0 else 1000
+    t_done, accepted, converged, aborted = 0.0, 0, 0, 0
+    records, y = [], beta * q
+    while t_done < t and accepted < max_steps:
+        dt = min(dt, t - t_done)
+        rejects = 0
+        while True:
+            growth = np.exp(dt * h11)
+            err = abs(h21 * growth) * beta
+            margin = abs(err - tol) / tol
+            if err <= tol:
+                margin = min(margin, abs(err - 0.1 * tol) / (0.1 * tol))
+            records.append({"t": t_done, "dt": dt, "err": err, "accepts": accepted,
+                            "rejects": rejects, "margin": margin})
+            if err <= tol:
+                t_done += dt
+                accepted += 1
+                beta *= growth
+                y = beta * q
+                if t_done >= t:
+                    converged = 1
+                    break
+                if err < 0.1 * tol:
+                    dt = min(dt * 1.5 * 0.8, dt_max)
+                break
+            rejects += 1
+            if rejects >= 20:
+                y = beta * growth * q  # the rejected lift is returned
+                aborted, t_done = 1, t
+                break
+            dt = max(dt * 0.5 * 0.8, dt_min)
+    stats = {"converged": converged, "restarts_done": accepted, "attempts": len(records),
+             "aborted": aborted}
+    return records, y, stats

@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from closed_forms import kron3_expmv, laplacian_2d_eigvec, laplacian_2d_expmv, rel_err
+from closed_forms import diag2_m1_trace, kron3_expmv, laplacian_2d_eigvec, laplacian_2d_expmv, rel_err
 from conftest import gen, mek
 
 pytestmark = pytest.mark.gpu
@@ -138,6 +138,36 @@ def test_two_rows_m1_aborts_like_the_oracle(gpu, oracle):
     for a, b in zip(steps, log):
         assert a["rejects"] == b["rejects"]
         assert a["dt"] == pytest.approx(b["dt"], rel=1e-15)
+
+
+@pytest.mark.parametrize("case", ["grow", "shrink_then_cap", "abort"])
+def test_controller_trace_closed_form(gpu, case):
+    """m = 1 on a 2 x 2 diagonal matrix: the GPU controller's attempts (t, dt,
+    err, accepts, rejects), stats and y against closed forms (q1 keeps its
+    direction, so err = h21 e^{dt h11} beta with beta growing by e^{dt h11}
+    per accepted step)."""
+    lam1, lam2, a, b, t, tol, R = {
+        "grow": (-1.0, -3.0, 1.0, 1.0, 4.0, 1.0, 20),
+        "shrink_then_cap": (0.5, -0.1, 1.0, 1.0, 3.0, 0.6, 10),
+        "abort": (-1.0, -3.0, 1.0, 1.0, 4.0, 0.5, 20),
+    }[case]
+    rp, ci, va = np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([lam1, lam2])
+    recs, yref, sref = diag2_m1_trace(lam1, lam2, a, b, t, tol, R)
+    print(f"[trace] {case} attempts={len(recs)}", flush=True)
+    run = _runner(rp, ci, va, m=1, t=t, tol=tol, max_restarts=R)
+    y = run.expmv(np.array([a, b]))
+    steps = run.steps()
+    assert len(steps) == len(recs)
+    for got, want in zip(steps, recs):
+        assert got["m_used"] == 1
+        assert (got["accepts"], got["rejects"]) == (want["accepts"], want["rejects"])
+        for f in ("t", "dt", "err"):
+            assert got[f] == pytest.approx(want[f], rel=1e-13, abs=1e-300), (f, got, want)
+    st = run.stats()
+    assert (st["converged"], st["restarts_done"], st["aborted"]) == \
+        (sref["converged"], sref["restarts_done"], sref["aborted"])
+    np.testing.assert_allclose(y, yref, rtol=1e-13)
+    run.close()
 
 
 @pytest.mark.parametrize("control", ["reference", "expokit"])

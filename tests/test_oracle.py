@@ -180,3 +180,34 @@ def test_nan_start_vector_is_accepted(oracle):
     y, st, log = oracle.expmv(rp, ci, va, v, 1.0, 1e-6, 5, 10)
     assert log[0]["err"] == 0.0 and log[0]["m_used"] == 1
     assert st["converged"] == 1 and np.all(y == 0.0)
+
+
+@pytest.mark.parametrize("case", ["grow", "shrink_then_cap", "abort"])
+def test_controller_trace_closed_form(oracle, case):
+    """m = 1 on a 2 x 2 diagonal matrix: every attempt's t, dt, err, accept
+    and reject count, the final y and the stats follow from closed forms."""
+    from closed_forms import diag2_m1_trace
+    lam1, lam2, a, b, t, tol, R = {
+        "grow": (-1.0, -3.0, 1.0, 1.0, 4.0, 1.0, 20),            # h11 < 0: err falls, dt grows
+        "shrink_then_cap": (0.5, -0.1, 1.0, 1.0, 3.0, 0.6, 10),  # h11 > 0: err rises, rejects shrink dt
+        "abort": (-1.0, -3.0, 1.0, 1.0, 4.0, 0.5, 20),           # err only rises as dt shrinks
+    }[case]
+    rp, ci, va = np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([lam1, lam2])
+    v = np.array([a, b])
+    recs, yref, sref = diag2_m1_trace(lam1, lam2, a, b, t, tol, R)
+    assert min(r["margin"] for r in recs) > 1e-9  # no decision within rounding of its threshold
+    y, st, log = oracle.expmv(rp, ci, va, v, t, tol, 1, R)
+    assert len(log) == len(recs)
+    for got, want in zip(log, recs):
+        assert got["m_used"] == 1
+        assert (got["accepts"], got["rejects"]) == (want["accepts"], want["rejects"])
+        for f in ("t", "dt", "err"):
+            assert got[f] == pytest.approx(want[f], rel=1e-13, abs=1e-300), (f, got, want)
+    for f in ("converged", "restarts_done", "attempts", "aborted"):
+        assert st[f] == sref[f], f
+    np.testing.assert_allclose(y, yref, rtol=1e-13)
+    if case == "grow":
+        dts = [r["dt"] for r in recs]
+        assert any(b2 > b1 * 1.19 for b1, b2 in zip(dts, dts[1:]))  # the growth branch ran
+    if case == "shrink_then_cap":
+        assert sref["converged"] == 0 and any(r["rejects"] > 0 for r in recs)
