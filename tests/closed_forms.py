@@ -107,3 +107,22 @@ def diag2_m1_trace(lam1: float, lam2: float, a: float, b: float, t: float, tol: 
     stats = {"converged": converged, "restarts_done": accepted, "attempts": len(records),
              "aborted": aborted}
     return records, y, stats
+
+
+def breakdown_trace(t: float, max_restarts: int):
+    """Step trace of the reference controller when every attempt ends in a
+    happy breakdown: err is 0 (main.cu:1467-1469), so every attempt is
+    accepted and dt grows by 1.5 * 0.8 up to t, clipped to the time left
+    (main.cu:1425-1518).  Returns a list of (t_done, dt) pairs."""
+    dt_min = max(1e-12, t * 1e-6)
+    dt = min(max(t / (max_restarts if max_restarts > 0 else 1), dt_min), t)
+    max_steps = max_restarts if max_restarts > 0 else 1000
+    t_done, out = 0.0, []
+    while t_done < t and len(out) < max_steps:
+        dt = min(dt, t - t_done)
+        out.append((t_done, dt))
+        t_done += dt
+        if t_done >= t:
+            break
+        dt = min(dt * 1.5 * 0.8, t)
+    return out

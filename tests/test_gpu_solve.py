@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from closed_forms import diag2_m1_trace, kron3_expmv, laplacian_2d_eigvec, laplacian_2d_expmv, rel_err
+from closed_forms import breakdown_trace, diag2_m1_trace, kron3_expmv, laplacian_2d_eigvec, laplacian_2d_expmv, rel_err
 from conftest import gen, mek
 
 pytestmark = pytest.mark.gpu
@@ -167,6 +167,25 @@ def test_controller_trace_closed_form(gpu, case):
     assert (st["converged"], st["restarts_done"], st["aborted"]) == \
         (sref["converged"], sref["restarts_done"], sref["aborted"])
     np.testing.assert_allclose(y, yref, rtol=1e-13)
+    run.close()
+
+
+def test_controller_trace_happy_breakdown(gpu):
+    """v in a 2-D invariant subspace of diag(-1, -2, -3, -4) with m = 3: every
+    attempt breaks down after 2 columns with err = 0, so the GPU trace is the
+    pure growth sequence and y is exact."""
+    rp, ci = np.arange(5, dtype=np.int32), np.arange(4, dtype=np.int32)
+    va = np.array([-1.0, -2.0, -3.0, -4.0])
+    v = np.array([0.7, -1.3, 0.0, 0.0])
+    run = _runner(rp, ci, va, m=3, t=2.0, tol=1e-10, max_restarts=10)
+    y = run.expmv(v)
+    want = breakdown_trace(2.0, 10)
+    steps = run.steps()
+    assert len(steps) == len(want) == run.stats()["restarts_done"] and run.stats()["converged"] == 1
+    for got, (t0, dt) in zip(steps, want):
+        assert got["m_used"] == 2 and got["err"] == 0.0 and got["rejects"] == 0
+        assert got["t"] == pytest.approx(t0, rel=1e-14) and got["dt"] == pytest.approx(dt, rel=1e-14)
+    np.testing.assert_allclose(y, np.exp(2.0 * va) * v, rtol=1e-13, atol=1e-300)
     run.close()
 
 

@@ -211,3 +211,20 @@ def test_controller_trace_closed_form(oracle, case):
         assert any(b2 > b1 * 1.19 for b1, b2 in zip(dts, dts[1:]))  # the growth branch ran
     if case == "shrink_then_cap":
         assert sref["converged"] == 0 and any(r["rejects"] > 0 for r in recs)
+
+
+def test_controller_trace_happy_breakdown(oracle):
+    """v in a 2-D invariant subspace of diag(-1, -2, -3, -4) with m = 3: every
+    attempt breaks down after 2 columns with err = 0, so the trace is the pure
+    growth sequence and y is exact."""
+    from closed_forms import breakdown_trace
+    rp, ci = np.arange(5, dtype=np.int32), np.arange(4, dtype=np.int32)
+    va = np.array([-1.0, -2.0, -3.0, -4.0])
+    v = np.array([0.7, -1.3, 0.0, 0.0])
+    y, st, log = oracle.expmv(rp, ci, va, v, 2.0, 1e-10, 3, 10)
+    want = breakdown_trace(2.0, 10)
+    assert len(log) == len(want) == st["restarts_done"] and st["converged"] == 1
+    for got, (t0, dt) in zip(log, want):
+        assert got["m_used"] == 2 and got["err"] == 0.0 and got["rejects"] == 0
+        assert got["t"] == pytest.approx(t0, rel=1e-14) and got["dt"] == pytest.approx(dt, rel=1e-14)
+    np.testing.assert_allclose(y, np.exp(2.0 * va) * v, rtol=1e-13, atol=1e-300)
