@@ -25,9 +25,10 @@
  *                     with m = 1 on 2 x 2 diagonal matrices (err from h11 and
  *                     h21 in closed form; the step rules restated a second
  *                     time from main_cpu.cpp:602-680) cover the grow,
- *                     shrink-and-cap and abort branches; for m > 1 only the
- *                     reference source text fixes the trace and the reference
- *                     cannot run here, so there it stays unpinned.
+ *                     shrink-and-cap and abort branches; a happy-breakdown
+ *                     trace (m = 3) the pure growth branch; for m > 1 without
+ *                     breakdown, a dense numpy/scipy restatement sharing no
+ *                     code with this file (tests/independent.py).
  */
 #ifndef MEK_ORACLE_H
 #define MEK_ORACLE_H

@@ -189,6 +189,34 @@ def test_controller_trace_happy_breakdown(gpu):
     run.close()
 
 
+@pytest.mark.parametrize("seed,m,tol,t,R", [(2, 6, 1e-9, 2.0, 0), (4, 8, 1e-11, 3.0, 20)])
+def test_controller_trace_independent_restatement(gpu, seed, m, tol, t, R):
+    """m > 1 without breakdown: the GPU's step trace against a dense numpy /
+    scipy restatement of the reference controller (rejections, growth and,
+    for R = 20, the step cap)."""
+    from independent import controller_trace
+    n = 80
+    rp, ci, va = gen.random_negative(n, 8e-2, seed=seed)
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, ci[rp[r]:rp[r + 1]]] += va[rp[r]:rp[r + 1]]
+    v = gen.normal_vector(n, seed=seed)
+    recs, yref, sref = controller_trace(A, v, t, tol, m, R)
+    run = _runner(rp, ci, va, m=m, t=t, tol=tol, max_restarts=R)
+    y = run.expmv(v)
+    steps = run.steps()
+    assert len(steps) == len(recs)
+    for got, want in zip(steps, recs):
+        assert (got["m_used"], got["accepts"], got["rejects"]) == \
+            (want["m_used"], want["accepts"], want["rejects"])
+        assert got["dt"] == pytest.approx(want["dt"], rel=1e-12)
+        assert got["err"] == pytest.approx(want["err"], rel=1e-7, abs=1e-300)
+    st = run.stats()
+    assert (st["converged"], st["restarts_done"]) == (sref["converged"], sref["restarts_done"])
+    assert rel_err(y, yref) < 1e-10
+    run.close()
+
+
 @pytest.mark.parametrize("control", ["reference", "expokit"])
 def test_laplacian_dst(gpu, control):
     N, t = 128, 10.0

@@ -228,3 +228,30 @@ def test_controller_trace_happy_breakdown(oracle):
         assert got["m_used"] == 2 and got["err"] == 0.0 and got["rejects"] == 0
         assert got["t"] == pytest.approx(t0, rel=1e-14) and got["dt"] == pytest.approx(dt, rel=1e-14)
     np.testing.assert_allclose(y, np.exp(2.0 * va) * v, rtol=1e-13, atol=1e-300)
+
+
+@pytest.mark.parametrize("seed,m,tol,t,R", [(1, 4, 1e-6, 1.0, 10), (2, 6, 1e-9, 2.0, 0),
+                                            (3, 3, 1e-7, 1.0, 5), (4, 8, 1e-11, 3.0, 20)])
+def test_controller_trace_independent_restatement(oracle, seed, m, tol, t, R):
+    """m > 1 without breakdown: the oracle's whole step trace against a dense
+    numpy / scipy restatement (CGS2 basis, scipy expm); same accept / reject
+    sequence, dt to rounding, err to 1e-7 relative."""
+    from independent import controller_trace
+    rp, ci, va = gen.random_negative(80, 8e-2, seed=seed)
+    A = np.zeros((80, 80))
+    for r in range(80):
+        A[r, ci[rp[r]:rp[r + 1]]] += va[rp[r]:rp[r + 1]]
+    v = gen.normal_vector(80, seed=seed)
+    recs, yref, sref = controller_trace(A, v, t, tol, m, R)
+    assert min(r["margin"] for r in recs) > 1e-5  # decisions clear of rounding
+    y, st, log = oracle.expmv(rp, ci, va, v, t, tol, m, R)
+    assert len(log) == len(recs)
+    for got, want in zip(log, recs):
+        assert (got["m_used"], got["accepts"], got["rejects"]) == \
+            (want["m_used"], want["accepts"], want["rejects"])
+        assert got["t"] == pytest.approx(want["t"], rel=1e-12, abs=1e-300)
+        assert got["dt"] == pytest.approx(want["dt"], rel=1e-12)
+        assert got["err"] == pytest.approx(want["err"], rel=1e-7, abs=1e-300)
+    for f in ("converged", "restarts_done", "attempts", "aborted"):
+        assert st[f] == sref[f], f
+    assert rel_err(y, yref) < 1e-10
