@@ -98,8 +98,13 @@ class BatchRunner {
   void* d_vals_ = nullptr;
   double *d_V_ = nullptr, *d_w_ = nullptr, *d_y_ = nullptr, *d_v0_ = nullptr, *d_small_ = nullptr;
   double *d_partials_ = nullptr, *d_expm_ws_ = nullptr, *d_pscal_ = nullptr;
+  // feature extraction (allocated by the first features() call)
+  int *d_row0_ = nullptr, *d_feat_int_ = nullptr;
+  double* d_feat_dbl_ = nullptr;
+  float* d_feat_out_ = nullptr;
   std::vector<int> pint_init_;
   std::vector<double> pscal_init_;
+  std::vector<double> norm1_;  // ||A_p||_1 (host), a selector feature
   void* bufs_ = nullptr;  // BatchBufs (opaque here)
 };
 

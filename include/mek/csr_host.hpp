@@ -76,6 +76,7 @@ struct CSRHost {
   // Additive helpers.
   double max_abs_value() const;       // max |a_ij| (0 for an empty matrix)
   double norm_inf() const;            // max_i sum_j |a_ij|
+  double norm_1() const;              // max_j sum_i |a_ij| (sums in storage order)
 };
 
 // reference: validate_ghost_maps (main.cu:695-744).  Prints the mismatch and

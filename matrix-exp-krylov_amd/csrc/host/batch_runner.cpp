@@ -120,6 +120,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     }
 
     pint_init_.assign(static_cast<size_t>(P_) * kPInt, 0);
+    norm1_.assign(P_, 0.0);
     int any_expokit = 0;
     pscal_init_.assign(static_cast<size_t>(P_) * kPScal, 0.0);
     for (int p = 0; p < P_; ++p) {
@@ -130,6 +131,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
       pscal_init_[p * kPScal + kBT] = probs[p].t;
       pscal_init_[p * kPScal + kBTol] = probs[p].tol;
       pscal_init_[p * kPScal + kBAnorm] = probs[p].A->norm_inf();
+      norm1_[p] = probs[p].A->norm_1();  // a selector feature
       if (probs[p].control >= 1) any_expokit = 1;
     }
 
@@ -148,6 +150,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     d_sb_row0_ = upload(sb_row0);
     d_sb_prob_ = upload(sb_prob);
     d_p_sb0_ = upload(p_sb0);
+    d_row0_ = upload(std::vector<int>(row0_.begin(), row0_.end() - 1));
     const long long ld = (n_tot_ + 63) / 64 * 64;
     const size_t ncols = static_cast<size_t>(M_) + 2;
     MEK_HIP(hipMalloc(&d_V_, sizeof(double) * ld * ncols));
@@ -304,41 +307,40 @@ BatchProblemStats BatchRunner::stats(int p_ext) const {
 
 void BatchRunner::features(float* out) {
   MEK_HIP(hipSetDevice(device_));
-  std::vector<double> t(P_), tol(P_);
-  for (int p = 0; p < P_; ++p) {
-    t[p] = pscal_init_[p * kPScal + kBT];
-    tol[p] = pscal_init_[p * kPScal + kBTol];
+  const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
+  if (!d_feat_dbl_) {
+    // kept for later calls: 8 statistics per streaming block, t, tol and
+    // ||A_p||_1, true row ends, and the features
+    MEK_HIP(hipMalloc(&d_feat_dbl_, sizeof(double) * (8 * static_cast<size_t>(b.nsb) + 3 * P_)));
+    MEK_HIP(hipMalloc(&d_feat_int_, sizeof(int) * P_));
+    MEK_HIP(hipMalloc(&d_feat_out_, sizeof(float) * kNumFeatures * P_));
   }
-  int* d_p0 = upload(row0_);
+  std::vector<double> tt(3 * static_cast<size_t>(P_));
   std::vector<int> row1(P_);
-  for (int p = 0; p < P_; ++p) row1[p] = row0_[p] + nrows_[p];  // without the padding row
-  int* d_p1 = upload(row1);
-  double* d_t = upload(t);
-  double* d_tol = upload(tol);
-  double *d_col = nullptr, *d_stats = nullptr;
-  float* d_f = nullptr;
-  hipError_t e = hipMalloc(&d_col, sizeof(double) * n_tot_);
-  if (e == hipSuccess) e = hipMalloc(&d_stats, sizeof(double) * 8 * P_);
-  if (e == hipSuccess) e = hipMalloc(&d_f, sizeof(float) * kNumFeatures * P_);
-  if (e == hipSuccess) e = hipMemsetAsync(d_col, 0, sizeof(double) * n_tot_, stream_);
-  if (e == hipSuccess) {
-    const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
-    feature_launch(d_row_ptr_, d_col_idx_, d_vals_, b.vals_f32, d_p0, d_p1, P_, d_col, d_stats, d_t, d_tol,
-                   d_f, stream_);
-    e = hipGetLastError();
+  for (int p = 0; p < P_; ++p) {
+    tt[p] = pscal_init_[p * kPScal + kBT];
+    tt[P_ + p] = pscal_init_[p * kPScal + kBTol];
+    tt[2 * P_ + p] = norm1_[p];
+    row1[p] = row0_[p] + nrows_[p];  // without the padding row
   }
+  double* d_stats = d_feat_dbl_;
+  double* d_tt = d_feat_dbl_ + 8 * static_cast<size_t>(b.nsb);
+  MEK_HIP(hipMemcpyAsync(d_tt, tt.data(), sizeof(double) * tt.size(), hipMemcpyHostToDevice, stream_));
+  MEK_HIP(hipMemcpyAsync(d_feat_int_, row1.data(), sizeof(int) * P_, hipMemcpyHostToDevice, stream_));
+  // the streaming blocks are the feature slabs: kBatchRowsPerBlock rows from
+  // each problem's first row, as a problem alone gets in features_csr
+  static_assert(kBatchRowsPerBlock == kFeatSlabRows, "streaming blocks double as feature slabs");
+  FeatureLayout L{b.sb_row0, b.sb_prob, b.p_sb0, d_row0_, d_feat_int_, b.nsb, P_};
+  feature_launch(d_row_ptr_, d_col_idx_, d_vals_, b.vals_f32, L, d_tt + 2 * P_, d_stats, d_tt, d_tt + P_,
+                 d_feat_out_, stream_);
+  MEK_HIP_LAUNCHED();
   std::vector<float> f_int(static_cast<size_t>(kNumFeatures) * P_);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(f_int.data(), d_f, sizeof(float) * f_int.size(), hipMemcpyDeviceToHost, stream_);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-  if (e == hipSuccess)
-    for (int p = 0; p < P_; ++p)
-      std::memcpy(out + static_cast<size_t>(p) * kNumFeatures,
-                  f_int.data() + static_cast<size_t>(ext2int_[p]) * kNumFeatures, sizeof(float) * kNumFeatures);
-  for (void* p : {static_cast<void*>(d_p0), static_cast<void*>(d_p1), static_cast<void*>(d_t), static_cast<void*>(d_tol),
-                  static_cast<void*>(d_col), static_cast<void*>(d_stats), static_cast<void*>(d_f)})
-    if (p) (void)hipFree(p);
-  MEK_HIP(e);
+  MEK_HIP(hipMemcpyAsync(f_int.data(), d_feat_out_, sizeof(float) * f_int.size(), hipMemcpyDeviceToHost,
+                         stream_));
+  MEK_HIP(hipStreamSynchronize(stream_));
+  for (int p = 0; p < P_; ++p)
+    std::memcpy(out + static_cast<size_t>(p) * kNumFeatures,
+                f_int.data() + static_cast<size_t>(ext2int_[p]) * kNumFeatures, sizeof(float) * kNumFeatures);
 }
 
 // Releases whatever is allocated (also after a constructor that failed part
@@ -373,6 +375,10 @@ void BatchRunner::destroy() {
   release(d_partials_);
   release(d_expm_ws_);
   release(d_pscal_);
+  release(d_row0_);
+  release(d_feat_dbl_);
+  release(d_feat_int_);
+  release(d_feat_out_);
   if (h_counters_) (void)hipHostFree(h_counters_);
   h_counters_ = nullptr;
   if (stream_) (void)hipStreamDestroy(stream_);

@@ -429,6 +429,14 @@ double CSRHost::max_abs_value() const {
   return m;
 }
 
+double CSRHost::norm_1() const {
+  std::vector<double> s(static_cast<size_t>(std::max(cols, 0)), 0.0);
+  for (long long k = 0; k < nnz; ++k) s[col_idx[k]] += std::abs(values[k]);
+  double m = 0.0;
+  for (double v : s) m = std::max(m, v);
+  return m;
+}
+
 double CSRHost::norm_inf() const {
   double m = 0.0;
   for (int r = 0; r < rows; ++r) {

@@ -164,14 +164,20 @@ float Mlp::train(const float* X, const float* T, int N, int batch, int epochs, f
 void features_csr(const CSRHost& A, double t, double tol, float* out, int device) {
   MEK_HIP(hipSetDevice(device));
   const int n = A.rows;
-  int *d_rp = nullptr, *d_ci = nullptr, *d_p0 = nullptr;
-  double *d_v = nullptr, *d_col = nullptr, *d_stats = nullptr, *d_tt = nullptr;
+  // slabs of kFeatSlabRows rows from row 0 (a batch cuts each problem the same way)
+  std::vector<int> lay;  // slab_row0 (nslab + 1) | slab_prob (nslab) | p_slab0 (2) | p_row0 | p_row1
+  for (int r = 0; r < std::max(n, 1); r += kFeatSlabRows) lay.push_back(r);
+  const int nslab = static_cast<int>(lay.size());
+  lay.push_back(n);
+  lay.insert(lay.end(), nslab, 0);
+  lay.insert(lay.end(), {0, nslab, 0, n});
+  const double tt[3] = {t, tol, A.norm_1()};
+  int *d_rp = nullptr, *d_ci = nullptr, *d_lay = nullptr;
+  double *d_v = nullptr, *d_stats = nullptr, *d_tt = nullptr;
   float* d_f = nullptr;
-  const int p0[2] = {0, n};
-  const double tt[2] = {t, tol};
   auto cleanup = [&] {
-    for (void* p : {static_cast<void*>(d_rp), static_cast<void*>(d_ci), static_cast<void*>(d_p0),
-                    static_cast<void*>(d_v), static_cast<void*>(d_col), static_cast<void*>(d_stats),
+    for (void* p : {static_cast<void*>(d_rp), static_cast<void*>(d_ci), static_cast<void*>(d_lay),
+                    static_cast<void*>(d_v), static_cast<void*>(d_stats),
                     static_cast<void*>(d_tt), static_cast<void*>(d_f)})
       if (p) (void)hipFree(p);
   };
@@ -179,20 +185,20 @@ void features_csr(const CSRHost& A, double t, double tol, float* out, int device
     MEK_HIP(hipMalloc(&d_rp, sizeof(int) * (n + 1)));
     MEK_HIP(hipMalloc(&d_ci, sizeof(int) * std::max<long long>(A.nnz, 1)));
     MEK_HIP(hipMalloc(&d_v, sizeof(double) * std::max<long long>(A.nnz, 1)));
-    MEK_HIP(hipMalloc(&d_col, sizeof(double) * std::max(n, 1)));
-    MEK_HIP(hipMalloc(&d_p0, sizeof(int) * 2));
-    MEK_HIP(hipMalloc(&d_stats, sizeof(double) * 8));
-    MEK_HIP(hipMalloc(&d_tt, sizeof(double) * 2));
+    MEK_HIP(hipMalloc(&d_lay, sizeof(int) * lay.size()));
+    MEK_HIP(hipMalloc(&d_stats, sizeof(double) * 8 * std::max(nslab, 1)));
+    MEK_HIP(hipMalloc(&d_tt, sizeof(double) * 3));
     MEK_HIP(hipMalloc(&d_f, sizeof(float) * kNumFeatures));
     MEK_HIP(hipMemcpy(d_rp, A.row_ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     if (A.nnz) {
       MEK_HIP(hipMemcpy(d_ci, A.col_idx.data(), sizeof(int) * A.nnz, hipMemcpyHostToDevice));
       MEK_HIP(hipMemcpy(d_v, A.values.data(), sizeof(double) * A.nnz, hipMemcpyHostToDevice));
     }
-    MEK_HIP(hipMemset(d_col, 0, sizeof(double) * std::max(n, 1)));
-    MEK_HIP(hipMemcpy(d_p0, p0, sizeof(p0), hipMemcpyHostToDevice));
+    MEK_HIP(hipMemcpy(d_lay, lay.data(), sizeof(int) * lay.size(), hipMemcpyHostToDevice));
     MEK_HIP(hipMemcpy(d_tt, tt, sizeof(tt), hipMemcpyHostToDevice));
-    feature_launch(d_rp, d_ci, d_v, 0, d_p0, d_p0 + 1, 1, d_col, d_stats, d_tt, d_tt + 1, d_f, nullptr);
+    const int* sl = d_lay + nslab + 1 + nslab;  // p_slab0, then p_row0, p_row1
+    FeatureLayout L{d_lay, d_lay + nslab + 1, sl, sl + 2, sl + 3, nslab, 1};
+    feature_launch(d_rp, d_ci, d_v, 0, L, d_tt + 2, d_stats, d_tt, d_tt + 1, d_f, nullptr);
     MEK_HIP_LAUNCHED();
     MEK_HIP(hipMemcpy(out, d_f, sizeof(float) * kNumFeatures, hipMemcpyDeviceToHost));
   } catch (...) {

@@ -111,16 +111,14 @@ __global__ void k_gather(const float* X, const float* T, const int* idx, float* 
 
 // ------------------------------------------------------------ features
 
-// Per-problem row statistics over rows [p_row0[p], p_row1[p]) of a
-// (block-diagonal) CSR whose columns are global; one workgroup per problem.
-// Column absolute sums go to colsum (zeroed by the caller) for ||A||_1.
+// The eight row statistics of one row slab of a (block-diagonal) CSR whose
+// columns are global; one workgroup per slab, one row per thread at a time.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_feat_rows(const int* row_ptr, const int* col_idx,
-                                                      const void* vals, const int* p_row0,
-                                                      const int* p_row1, double* colsum, double* stats) {
+                                                      const void* vals, FeatureLayout L, double* stats) {
   __shared__ double red[kWavesPerBlock][8];
-  const int p = blockIdx.x;
-  const int r0 = p_row0[p], r1 = p_row1[p];
+  const int sl = blockIdx.x;
+  const int r0 = L.slab_row0[sl], r1 = min(L.slab_row0[sl + 1], L.p_row1[L.slab_prob[sl]]);
   double s_nnz = 0, s_nnz2 = 0, mx_nnz = 0, mx_rowabs = 0, mx_right = -1e308, mn_left = 1e308,
          mx_diag = 0, s_dom = 0;
   const VT* v = static_cast<const VT*>(vals);
@@ -132,7 +130,6 @@ __global__ __launch_bounds__(kBlock) void k_feat_rows(const int* row_ptr, const 
       const int c = col_idx[k];
       if (c == r) diag += a;
       else off += fabs(a);
-      atomicAdd(&colsum[c], fabs(a));
     }
     const double nz = k1 - k0;
     s_nnz += nz;
@@ -158,28 +155,27 @@ __global__ __launch_bounds__(kBlock) void k_feat_rows(const int* row_ptr, const 
     const bool is_max = (q == 2 || q == 3 || q == 4 || q == 5 || q == 6);
     double s = red[0][q];
     for (int w = 1; w < kWavesPerBlock; ++w) s = is_max ? fmax(s, red[w][q]) : s + red[w][q];
-    stats[p * 8 + q] = s;
+    stats[sl * 8 + q] = s;
   }
 }
 
-// The 16 features of problem p (see learn.hpp) from the row statistics and
-// the max column sum over the problem's columns.
-__global__ __launch_bounds__(kBlock) void k_feat_final(const int* p_row0, const int* p_row1,
-                                                       const double* colsum,
-                                                       const double* stats, const double* t,
-                                                       const double* tol, float* feat) {
-  __shared__ double red[kWavesPerBlock];
+// The 16 features of problem p (see learn.hpp) from its slabs' row
+// statistics, combined in slab order, and ||A_p||_1.
+__global__ void k_feat_final(FeatureLayout L, const double* norm1, const double* stats, const double* t,
+                             const double* tol, float* feat) {
+  __shared__ double s[8];
   const int p = blockIdx.x;
-  const int r0 = p_row0[p], r1 = p_row1[p];
-  double mx = 0.0;
-  for (int c = r0 + threadIdx.x; c < r1; c += kBlock) mx = fmax(mx, colsum[c]);
-  mx = wave_max(mx);
-  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = mx;
+  const int r0 = L.p_row0[p], r1 = L.p_row1[p];
+  if (threadIdx.x < 8) {
+    const int q = threadIdx.x;
+    const bool is_max = (q == 2 || q == 3 || q == 4 || q == 5 || q == 6);
+    const int s0 = L.p_slab0[p], s1 = L.p_slab0[p + 1];
+    double v = stats[s0 * 8 + q];
+    for (int sl = s0 + 1; sl < s1; ++sl) v = is_max ? fmax(v, stats[sl * 8 + q]) : v + stats[sl * 8 + q];
+    s[q] = v;
+  }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  double norm1 = red[0];
-  for (int w = 1; w < kWavesPerBlock; ++w) norm1 = fmax(norm1, red[w]);
-  const double* s = stats + p * 8;
   const double n = fmax(1.0, static_cast<double>(r1 - r0));
   const double mean_nnz = s[0] / n;
   const double var_nnz = fmax(0.0, s[1] / n - mean_nnz * mean_nnz);
@@ -193,10 +189,10 @@ __global__ __launch_bounds__(kBlock) void k_feat_final(const int* p_row0, const 
   f[3] = static_cast<float>(s[2] / fmax(mean_nnz, 1e-300));
   f[4] = static_cast<float>(sqrt(var_nnz) / fmax(mean_nnz, 1e-300));
   f[5] = static_cast<float>(lg(ninf));
-  f[6] = static_cast<float>(lg(norm1));
+  f[6] = static_cast<float>(lg(norm1[p]));
   f[7] = static_cast<float>(lg(tt));
   f[8] = static_cast<float>(lg(tt * ninf));
-  f[9] = static_cast<float>(lg(tt * norm1));
+  f[9] = static_cast<float>(lg(tt * norm1[p]));
   f[10] = static_cast<float>(tt * s[4]);                    // t * rightmost Gershgorin bound
   f[11] = static_cast<float>(lg(tt * s[5]));                // log t |leftmost Gershgorin bound|
   f[12] = static_cast<float>(lg(tt * s[6]));                // log t max |a_ii|
@@ -236,15 +232,15 @@ void mlp_gather(const float* X, const float* T, const int* idx, float* Xb, float
   hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, X, T, idx, Xb, Tb, B, dx, dt);
 }
 void feature_launch(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32,
-                    const int* p_row0, const int* p_row1, int P, double* colsum, double* stats,
-                    const double* t, const double* tol, float* feat, hipStream_t s) {
+                    const FeatureLayout& L, const double* norm1, double* stats, const double* t,
+                    const double* tol, float* feat, hipStream_t s) {
   if (vals_f32)
-    hipLaunchKernelGGL(k_feat_rows<float>, dim3(P), dim3(kBlock), 0, s, row_ptr, col_idx, vals, p_row0,
-                       p_row1, colsum, stats);
+    hipLaunchKernelGGL(k_feat_rows<float>, dim3(L.nslab), dim3(kBlock), 0, s, row_ptr, col_idx, vals, L,
+                       stats);
   else
-    hipLaunchKernelGGL(k_feat_rows<double>, dim3(P), dim3(kBlock), 0, s, row_ptr, col_idx, vals, p_row0,
-                       p_row1, colsum, stats);
-  hipLaunchKernelGGL(k_feat_final, dim3(P), dim3(kBlock), 0, s, p_row0, p_row1, colsum, stats, t, tol, feat);
+    hipLaunchKernelGGL(k_feat_rows<double>, dim3(L.nslab), dim3(kBlock), 0, s, row_ptr, col_idx, vals, L,
+                       stats);
+  hipLaunchKernelGGL(k_feat_final, dim3(L.P), dim3(kWave), 0, s, L, norm1, stats, t, tol, feat);
 }
 
 }  // namespace mek
