@@ -308,13 +308,11 @@ BatchProblemStats BatchRunner::stats(int p_ext) const {
 void BatchRunner::features(float* out) {
   MEK_HIP(hipSetDevice(device_));
   const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
-  if (!d_feat_dbl_) {
-    // kept for later calls: 8 statistics per streaming block, t, tol and
-    // ||A_p||_1, true row ends, and the features
-    MEK_HIP(hipMalloc(&d_feat_dbl_, sizeof(double) * (8 * static_cast<size_t>(b.nsb) + 3 * P_)));
-    MEK_HIP(hipMalloc(&d_feat_int_, sizeof(int) * P_));
-    MEK_HIP(hipMalloc(&d_feat_out_, sizeof(float) * kNumFeatures * P_));
-  }
+  // kept for later calls (and freed by destroy()): 8 statistics per
+  // streaming block, t, tol and ||A_p||_1, true row ends, and the features
+  if (!d_feat_dbl_) MEK_HIP(hipMalloc(&d_feat_dbl_, sizeof(double) * (8 * static_cast<size_t>(b.nsb) + 3 * P_)));
+  if (!d_feat_int_) MEK_HIP(hipMalloc(&d_feat_int_, sizeof(int) * P_));
+  if (!d_feat_out_) MEK_HIP(hipMalloc(&d_feat_out_, sizeof(float) * kNumFeatures * P_));
   std::vector<double> tt(3 * static_cast<size_t>(P_));
   std::vector<int> row1(P_);
   for (int p = 0; p < P_; ++p) {
