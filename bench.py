@@ -104,22 +104,61 @@ def cpu_baseline(rp, ci, va, v, columns_per_solve, m, n, seconds_budget):
     per_step_ms = sum(a + b * (k + 1) for k in range(m))
     lift_ms = (m + 1) * b / 4.0  # one axpy per basis vector vs. two dots + two axpys
     solve_ms = columns_per_solve / m * (per_step_ms + lift_ms)
+    return {
+        "value": 1000.0 / solve_ms,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "host": _host_desc(core),
+        "sample": (f"oracle Arnoldi loop (SpMV + MGS2 + normalise, main_cpu.cpp:486-546 restated in C) "
+                   f"for the first {col_ms.size} columns of the same matrix and start vector "
+                   f"({wall:.1f} s wall), fitted as {a:.1f} + {b:.1f}*(j+1) ms per column and "
+                   f"extrapolated to the {columns_per_solve} columns (+ lifts) of one GPU solve"),
+    }
+
+
+def _host_desc(core):
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
+    return f"1 thread pinned to core {core} of {os.cpu_count()} ({model or 'unknown CPU'})"
+
+
+def cpu_baseline_batch(mats, ts, ms, vs, tol, seconds_budget):
+    """The oracle's reference solver (MGS2, reference step controller with the
+    first step t, tol absolute per step) on the batch's problems in turn, one
+    thread, until the budget is spent: problems/s on that sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    prev = os.sched_getaffinity(0)
+    core = min(prev)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.time()
+        k = conv = 0
+        for (rp, ci, va), t, m, v in zip(mats, ts, ms, vs):
+            _y, st, _log = pyoracle.expmv(rp, ci, va, v, t, tol, m, 0)
+            conv += int(st["converged"])
+            k += 1
+            if time.time() - t0 > seconds_budget:
+                break
+        wall = time.time() - t0
+    finally:
+        os.sched_setaffinity(0, prev)
+    rows = sum(int(rp.size - 1) for rp, _, _ in mats[:k])
     return {
-        "value": 1000.0 / solve_ms,
-        "unit": "solves/s",
+        "value": k / wall,
+        "unit": "problems/s",
         "cores": 1,
         "kind": "port",
-        "host": f"1 thread pinned to core {core} of {os.cpu_count()} ({model or 'unknown CPU'})",
-        "sample": (f"oracle Arnoldi loop (SpMV + MGS2 + normalise, main_cpu.cpp:486-546 restated in C) "
-                   f"for the first {col_ms.size} columns of the same matrix and start vector "
-                   f"({wall:.1f} s wall), fitted as {a:.1f} + {b:.1f}*(j+1) ms per column and "
-                   f"extrapolated to the {columns_per_solve} columns (+ lifts) of one GPU solve"),
+        "host": _host_desc(core),
+        "sample": (f"oracle solve (main_cpu.cpp:602-700 restated in C: MGS2, reference controller, "
+                   f"first step t, tol {tol:g} per step, each problem's m) of the first {k} of the "
+                   f"{len(mats)} problems ({rows} rows, {conv} converged) in turn, {wall:.1f} s wall"),
     }
 
 
@@ -189,7 +228,7 @@ def run_batch(args, mek, gen):
 
     heur_s = [0] * P  # Expokit's own first step
 
-    def batched(name, m_list, s_list, f32, select_s=0.0):
+    def batched(name, m_list, s_list, f32, select_s=0.0, spmv_timing=False):
         log(f"batched: {name}")
         B = mek.Batch(csrs, ts, m_list, tols, substeps=s_list, values_f32=f32, gs_mode=gs_mode,
                       step_control="expokit_rel", autotune=args.autotune)
@@ -207,13 +246,16 @@ def run_batch(args, mek, gen):
                "spmv_grid": B.spmv_grid() if args.autotune else None,
                "cycles": B.cycles(), "krylov_columns": int(sum(x["spmv_count"] for x in st)),
                "converged": int(sum(x["converged"] for x in st)), "max_rel_err": max_err(ys)}
+        if spmv_timing:
+            out["spmv_us"] = B.time_spmv(20)
         B.close()
         return out
 
     res = {"plain_fp64": {"problems_per_s": P / plain_s, "solve_s": plain_s, "krylov_columns": plain_cols,
                           "max_rel_err": plain_err}}
     res["batched_fp64"] = batched("FP64, Expokit step control", ms, heur_s, False)
-    res["batched_mixed"] = batched("FP32 values, Expokit step control", ms, heur_s, True)
+    res["batched_mixed"] = batched("FP32 values, Expokit step control", ms, heur_s, True,
+                                   spmv_timing=not os.path.exists(sel.WEIGHTS))
     learned = None
     if os.path.exists(sel.WEIGHTS):
         S = sel.Selector.load()
@@ -225,10 +267,24 @@ def run_batch(args, mek, gen):
         FB.close()
         lm = [min(m, n - 1) for (m, s), n in zip(choice, rows)]
         ls = [s for m, s in choice]
-        learned = batched("FP32 values, learned (m, sub-steps)", lm, ls, True, select_s)
+        learned = batched("FP32 values, learned (m, sub-steps)", lm, ls, True, select_s, spmv_timing=True)
         res["batched_mixed_learned"] = learned
     head = learned if learned is not None else res["batched_mixed"]
     form = "batched_mixed_learned" if learned is not None else "batched_mixed"
+    # the SpMV dominates the batched solve (about 60 % of its kernel time):
+    # one launch over every problem's start vector, FP32 values
+    rows_tot = int(sum(rows))
+    spmv_bytes = nnz_tot * (4 + 4) + 4.0 * (rows_tot + 1) + 16.0 * rows_tot
+    spmv_gbs = spmv_bytes / (head["spmv_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(spmv_gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "batch spmv",
+                "bytes_per_launch": spmv_bytes, "avg_launch_us": head["spmv_us"],
+                "note": ("first-column SpMV over all problems (median of 20 HIP-event-timed launches on "
+                         "the batch's stream); bound by the rate of random-column gathers, not HBM")}
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("CPU baseline (oracle, one thread, bounded sample)")
+        cpu = cpu_baseline_batch(mats, ts, ms, vs, tol, args.cpu_seconds)
     out = {
         "metric": "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64",
         "value": round(head["problems_per_s"], 3),
@@ -246,6 +302,8 @@ def run_batch(args, mek, gen):
                    "tol": tol, "gs": args.gs, "headline_form": form, "seed": 0},
         "rel_err": head["max_rel_err"],
         "speedup_vs_plain_fp64": round(head["problems_per_s"] / res["plain_fp64"]["problems_per_s"], 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
         "forms": res,
     }
     print(json.dumps(out), flush=True)

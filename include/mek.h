@@ -161,6 +161,12 @@ int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const 
                      int autotune, int device_id, mek_batch** out);
 /* SpMV grid the autotuner chose at the first solve (0 without autotune). */
 int mek_batch_tuning(const mek_batch* B, int* spmv_grid);
+/* Median microseconds of one batched SpMV over every problem's start vector
+ * (all problems active, as in the first Arnoldi column of a solve), over
+ * reps launches timed with HIP events on the batch's stream.  Call after
+ * mek_batch_upload_v; leaves the batch at its start state.  No reference
+ * counterpart (measurement of the batched form). */
+int mek_batch_time_spmv(mek_batch* B, int reps, double* us);
 /* v[p]: host arrays of each problem's rows, copied to the device. */
 int mek_batch_upload_v(mek_batch* B, const double* const* v);
 /* Solve every problem from the uploaded v; y stays in device memory. */

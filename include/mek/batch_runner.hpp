@@ -62,6 +62,10 @@ class BatchRunner {
   BatchProblemStats stats(int p) const;
   // The 16 learned-selector features of every problem (P x 16, host).
   void features(float* out);
+  // Median microseconds of the first column's SpMV over every problem (all
+  // of them active, the solver's SpMV grid) over `reps` timed launches; the
+  // batch is left at its start state.
+  double time_spmv(int reps);
   int cycles() const { return cycles_; }
   int spmv_grid() const { return tuned_spmv_grid_; }
   double solve_ms() const { return solve_ms_; }
@@ -69,6 +73,7 @@ class BatchRunner {
   void destroy();
 
  private:
+  void reset_state();
   void run_cycle_eager(int mcols);
   hipGraphExec_t cycle_graph(int mcols);
   int P_ = 0;

@@ -100,6 +100,7 @@ SIGNATURES = {
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "mek_batch_tuning": (ctypes.c_int, [_vp, _ip]),
+    "mek_batch_time_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mek_batch_upload_v": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
     "mek_batch_solve": (ctypes.c_int, [_vp]),
     "mek_batch_download_y": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
@@ -423,6 +424,12 @@ class Batch:
         g = ctypes.c_int()
         _check(lib().mek_batch_tuning(self.h, ctypes.byref(g)))
         return g.value
+
+    def time_spmv(self, reps=20):
+        """Median us of one SpMV over every problem (after upload)."""
+        us = ctypes.c_double()
+        _check(lib().mek_batch_time_spmv(self.h, int(reps), ctypes.byref(us)))
+        return us.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -366,6 +366,13 @@ int mek_batch_tuning(const mek_batch* B, int* spmv_grid) {
   });
 }
 
+int mek_batch_time_spmv(mek_batch* B, int reps, double* us) {
+  return guarded([&] {
+    need(B && us && reps >= 1, "batch, reps >= 1, out");
+    *us = B->B->time_spmv(reps);
+  });
+}
+
 int mek_batch_upload_v(mek_batch* B, const double* const* v) {
   return guarded([&] {
     need(B && v, "batch, v");

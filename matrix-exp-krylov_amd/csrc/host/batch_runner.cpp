@@ -240,9 +240,9 @@ hipGraphExec_t BatchRunner::cycle_graph(int mcols) {
   return graph_exec_[mcols];
 }
 
-void BatchRunner::solve_device() {
-  MEK_HIP(hipSetDevice(device_));
-  const double t0 = now_ms();
+// Every problem back at its start: V[:, 0] = v, controller state from the
+// problem descriptions, start norms and first steps on the device.
+void BatchRunner::reset_state() {
   const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
   MEK_HIP(hipMemcpyAsync(d_V_, d_v0_, sizeof(double) * n_tot_, hipMemcpyDeviceToDevice, stream_));
   MEK_HIP(hipMemcpyAsync(d_pint_, pint_init_.data(), sizeof(int) * pint_init_.size(), hipMemcpyHostToDevice, stream_));
@@ -251,6 +251,25 @@ void BatchRunner::solve_device() {
   batch_prepare();
   batch_launch_init(b, stream_);
   MEK_HIP_LAUNCHED();
+}
+
+double BatchRunner::time_spmv(int reps) {
+  if (reps < 1) throw Error("ERROR: time_spmv needs reps >= 1");
+  MEK_HIP(hipSetDevice(device_));
+  reset_state();
+  const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
+  std::vector<double> us;
+  time_candidates(
+      {plan_.spmv_grid[0]}, [&](int g) { batch_launch_spmv(b, 0, plan_.nrb[0], g, stream_); }, stream_, reps,
+      &us);
+  return us[0];
+}
+
+void BatchRunner::solve_device() {
+  MEK_HIP(hipSetDevice(device_));
+  const double t0 = now_ms();
+  const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
+  reset_state();
   if (autotune_ && tuned_spmv_grid_ == 0) {
     // every problem is active with its start block: time the column-0 SpMV
     // over all row blocks for each grid and scale the per-column grids

@@ -159,3 +159,20 @@ def test_batch_odd_rows_long_and_empty_rows(gpu, oracle):
         assert B.stats(p)["converged"] == st["converged"] == 1, p
         assert rel_err(y, yref) <= 1e-8, (p, rel_err(y, yref))
     B.close()
+
+
+def test_batch_time_spmv_leaves_solve_unchanged(gpu):
+    """The SpMV timing entry runs between solves: it reports a positive time
+    and the next solve gives the same bits as the one before it."""
+    mats, ts, ms, vs = _problems(P=5, seed=11)
+    B = mek.Batch(mats, ts, ms, 1e-9)
+    B.upload(vs)
+    B.solve()
+    y0 = B.download()
+    us = B.time_spmv(5)
+    print(f"[batch] spmv {us:.1f} us", flush=True)
+    assert us > 0.0
+    B.solve()
+    for a, b in zip(B.download(), y0):
+        np.testing.assert_array_equal(a, b)
+    B.close()
