@@ -187,6 +187,10 @@ def run_batch(args, mek, gen):
     rows = [q["rp"].size - 1 for q in probs]
     nnz_tot = int(sum(int(q["rp"][-1]) for q in probs))
     csrs = [mek.CSR.from_arrays(*A) for A in mats]
+    # FP32 value storage is lossless only for values exact in FP32: say so
+    f32_check = [c.f32_inexact() for c in csrs]
+    f32_values = {"inexact": int(sum(c for c, _ in f32_check)),
+                  "max_rel_change": max(m for _, m in f32_check)}
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
 
     log("reference solves (tol 1e-13)")
@@ -301,6 +305,7 @@ def run_batch(args, mek, gen):
         "config": {"workload": "batch256", "problems": P, "rows": int(sum(rows)), "nnz": nnz_tot,
                    "tol": tol, "gs": args.gs, "headline_form": form, "seed": 0},
         "rel_err": head["max_rel_err"],
+        "values_f32": f32_values,
         "speedup_vs_plain_fp64": round(head["problems_per_s"] / res["plain_fp64"]["problems_per_s"], 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -466,6 +471,10 @@ def main():
     R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
                    gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=True,
                    fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune)
+    f32_values = None
+    if w["f32"]:
+        c, mrel = A.f32_inexact()
+        f32_values = {"inexact": int(c), "max_rel_change": mrel}
     A.close()
     d_v = mek.DeviceBuffer.from_array(v)
     d_y = mek.DeviceBuffer(8 * n)
@@ -543,6 +552,7 @@ def main():
                    "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
                    "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "autotune": args.autotune, "seed": 0},
         "rel_err": rel,
+        "values_f32": f32_values,
         "solve": {"substeps": stats["accepted"], "expm_attempts": stats["attempts"],
                   "krylov_columns": stats["spmv_count"], "extra_spmv": stats["extra_spmv"],
                   "converged": bool(stats["converged"])},

@@ -62,6 +62,11 @@ int mek_csr_load_mtx(const char* path, mek_csr** out);
 int mek_csr_info(const mek_csr* A, int* rows, int* cols, long long* nnz, int* value_dtype);
 /* Copy the arrays out (row_ptr: rows+1, col_idx/values: nnz entries). */
 int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values);
+/* How many values FP32 storage (values_f32) would change, and the largest
+ * relative change among them (0 when FP32 storage is lossless).  No
+ * reference counterpart: the check a caller makes before choosing FP32
+ * values under a tight tolerance. */
+int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel);
 void mek_csr_destroy(mek_csr* A);
 
 /* ---------------------------------------------------------------- solver */

@@ -77,6 +77,9 @@ struct CSRHost {
   double max_abs_value() const;       // max |a_ij| (0 for an empty matrix)
   double norm_inf() const;            // max_i sum_j |a_ij|
   double norm_1() const;              // max_j sum_i |a_ij| (sums in storage order)
+  // Values that FP32 storage would change: their count, and the largest
+  // relative change |fl32(a) - a| / |a| in *max_rel (0 when all are exact).
+  long long f32_inexact(double* max_rel) const;
 };
 
 // reference: validate_ghost_maps (main.cu:695-744).  Prints the mismatch and

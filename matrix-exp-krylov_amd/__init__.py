@@ -81,6 +81,7 @@ SIGNATURES = {
     "mek_csr_load_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "mek_csr_info": (ctypes.c_int, [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_csr_export": (ctypes.c_int, [_vp, _ip, _ip, _dp]),
+    "mek_csr_f32_inexact": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "mek_csr_destroy": (None, [_vp]),
     "mek_solver_options_default": (None, [ctypes.POINTER(SolverOptions)]),
     "mek_runner_create": (ctypes.c_int, [_vp, ctypes.POINTER(ArnoldiParams),
@@ -257,6 +258,12 @@ class CSR:
         z = ctypes.c_longlong()
         _check(lib().mek_csr_info(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), ctypes.byref(d)))
         return {"rows": r.value, "cols": c.value, "nnz": z.value, "value_dtype": d.value}
+
+    def f32_inexact(self):
+        """(count, max relative change) of the values FP32 storage would change."""
+        c, m = ctypes.c_longlong(), ctypes.c_double()
+        _check(lib().mek_csr_f32_inexact(self.h, ctypes.byref(c), ctypes.byref(m)))
+        return c.value, m.value
 
     def arrays(self):
         i = self.info()

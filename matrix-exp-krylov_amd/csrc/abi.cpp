@@ -159,6 +159,13 @@ int mek_csr_info(const mek_csr* A, int* rows, int* cols, long long* nnz, int* va
   });
 }
 
+int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel) {
+  return guarded([&] {
+    need(A && count && max_rel, "matrix, outputs");
+    *count = A->A.f32_inexact(max_rel);
+  });
+}
+
 int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values) {
   return guarded([&] {
     need(A, "csr");

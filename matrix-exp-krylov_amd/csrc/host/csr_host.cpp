@@ -429,6 +429,21 @@ double CSRHost::max_abs_value() const {
   return m;
 }
 
+long long CSRHost::f32_inexact(double* max_rel) const {
+  long long count = 0;
+  double worst = 0.0;
+  for (long long k = 0; k < nnz; ++k) {
+    const double a = values[k];
+    const double r = static_cast<double>(static_cast<float>(a));
+    if (r != a) {
+      ++count;
+      worst = std::max(worst, std::isfinite(r) ? std::abs(r - a) / std::abs(a) : HUGE_VAL);
+    }
+  }
+  if (max_rel) *max_rel = worst;
+  return count;
+}
+
 double CSRHost::norm_1() const {
   std::vector<double> s(static_cast<size_t>(std::max(cols, 0)), 0.0);
   for (long long k = 0; k < nnz; ++k) s[col_idx[k]] += std::abs(values[k]);

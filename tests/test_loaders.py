@@ -208,3 +208,17 @@ def test_generated_matrix_roundtrip_through_bincsr(tmp_path):
     r2, c2, v2 = mek.CSR.load_bincsr(p).arrays()
     np.testing.assert_array_equal(r2, rp)
     np.testing.assert_array_equal(v2, va)  # stencil coefficients are exact in f32
+
+
+def test_f32_inexact_count():
+    """FP32 storage check: stencil coefficients are exact, 0.1 is not (its
+    FP32 rounding changes it by 1.49e-9 relative)."""
+    A = mek.CSR.from_arrays(np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32),
+                            np.array([-4.0, 1.0, 13.5, -93.5]))
+    assert A.f32_inexact() == (0, 0.0)
+    B = mek.CSR.from_arrays(np.array([0, 2, 3], np.int32), np.array([0, 1, 1], np.int32),
+                            np.array([0.1, 0.5, 1e-40]))
+    n, worst = B.f32_inexact()
+    assert n == 2  # 0.1, and 1e-40, which is subnormal in FP32
+    r = abs(float(np.float32(0.1)) - 0.1) / 0.1
+    assert worst >= r and worst <= 1.0
