@@ -6,18 +6,33 @@ device-resident v and y).  Default workload: 2-D 5-point Laplacian on a
 indices), v ~ N(0,1) normalised (seed 0), t = 10, m = 40, tol 1e-8, sub-steps
 by the Expokit heuristic.
 
-Prints one JSON line with solves/s, the relative error against the DST-I closed
-form, a roofline object for the dominant kernel (algorithmic bytes per launch /
-HIP-event launch time, on the solver's own stream), the SpMV kernel's HBM
-GB/s, and a CPU baseline: the oracle's single-thread Arnoldi column loop timed
-on a bounded sample of the same matrix and extrapolated to the same solve.
+Prints one JSON line with
+* value: solves/s of that workload, v and y resident in HBM, timed with no
+  per-kernel events in the loop; beside it the same solve on the reference's
+  METRIC span (host v in, host y out: main.cu:1412-1570);
+* rel_err against the DST-I closed form;
+* roofline: the dominant kernel's algorithmic bytes per launch over its
+  HIP-event launch time on the solver's own stream (a separate pass with one
+  event pair per launch), and the SpMV's;
+* learned_mixed_batched: 256 independent problems (n log-uniform in
+  [1e3, 2e5], random rows, U[-1,1] FP64 values) as plain FP64 solves one at a
+  time, as one batched FP64 solve, with the FP32-value request (which falls
+  back to FP64 storage because these values are not exact in FP32), with the
+  learned (m, first sub-step) choice (selection time included), and with FP32
+  storage forced (reported, not a 1e-8 form); speed-up over plain FP64;
+* cpu_baseline: the oracle (one thread) on one full sub-step of the same
+  solve (40 columns, exponential, lift), timed, times the sub-step count;
+  and the reference's own CPU case (n = 2000 random matrix, m = 30, t = 1,
+  tol 1e-6, 10 restarts), median of 3, beside the GPU on the same case.
 """
 from __future__ import annotations
 
 import argparse
 import importlib.util
 import json
+import math
 import os
+import statistics
 import sys
 import time
 
@@ -25,6 +40,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK_HBM_GBS = 8000.0
+METRIC = "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64"
 
 
 def _load(name, path):
@@ -46,6 +62,11 @@ WORKLOADS = {
     "convdiff256_f32": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=True),
     "convdiff256_f64": dict(kind="convdiff", N=256, t=0.5, m=50, tol=1e-8, f32=False),
 }
+
+
+def _finite(x):
+    """JSON-safe float (None for inf / NaN)."""
+    return float(x) if x is not None and math.isfinite(x) else None
 
 
 def reference_solution(w, v, gen):
@@ -71,52 +92,6 @@ def measured_traffic(workload):
         return json.load(f).get("bytes_per_launch", {})
 
 
-def cpu_baseline(rp, ci, va, v, columns_per_solve, m, n, seconds_budget):
-    """Time the oracle's reference Arnoldi loop (SpMV + MGS2 + normalise, one
-    thread) on the first columns of the same matrix and extrapolate linearly in
-    the basis size to the columns the GPU solve built."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    pyoracle.build()
-    # pick a column count that fits the time budget (first call is a probe)
-    probe = pyoracle.arnoldi_columns(rp, ci, va, v, 3)
-    slope = max(float(probe[2] - probe[1]), 0.0)
-    ncols, total = 0, 0.0
-    while ncols < m:  # largest column count whose predicted time fits the budget
-        nxt = float(probe[0]) + slope * ncols
-        if ncols >= 3 and total + nxt > seconds_budget * 1000.0:
-            break
-        total += nxt
-        ncols += 1
-    # one thread pinned to one core, as the reference's single-threaded CPU
-    # binary runs (the calling thread's affinity; restored afterwards)
-    prev = os.sched_getaffinity(0)
-    core = min(prev)
-    os.sched_setaffinity(0, {core})
-    try:
-        t0 = time.time()
-        col_ms = pyoracle.arnoldi_columns(rp, ci, va, v, ncols)
-        wall = time.time() - t0
-    finally:
-        os.sched_setaffinity(0, prev)
-    j = np.arange(1, col_ms.size + 1, dtype=float)
-    b, a = np.polyfit(j, col_ms, 1)  # ms per column = a + b * (basis vectors)
-    per_step_ms = sum(a + b * (k + 1) for k in range(m))
-    lift_ms = (m + 1) * b / 4.0  # one axpy per basis vector vs. two dots + two axpys
-    solve_ms = columns_per_solve / m * (per_step_ms + lift_ms)
-    return {
-        "value": 1000.0 / solve_ms,
-        "unit": "solves/s",
-        "cores": 1,
-        "kind": "port",
-        "host": _host_desc(core),
-        "sample": (f"oracle Arnoldi loop (SpMV + MGS2 + normalise, main_cpu.cpp:486-546 restated in C) "
-                   f"for the first {col_ms.size} columns of the same matrix and start vector "
-                   f"({wall:.1f} s wall), fitted as {a:.1f} + {b:.1f}*(j+1) ms per column and "
-                   f"extrapolated to the {columns_per_solve} columns (+ lifts) of one GPU solve"),
-    }
-
-
 def _host_desc(core):
     model = ""
     try:
@@ -127,28 +102,68 @@ def _host_desc(core):
     return f"1 thread pinned to core {core} of {os.cpu_count()} ({model or 'unknown CPU'})"
 
 
+class _one_core:
+    """Pin the calling thread to one core (the reference's CPU binary is
+    single-threaded), restoring the affinity afterwards."""
+
+    def __enter__(self):
+        self.prev = os.sched_getaffinity(0)
+        self.core = min(self.prev)
+        os.sched_setaffinity(0, {self.core})
+        return self.core
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.prev)
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    return pyoracle
+
+
+def cpu_baseline_substep(rp, ci, va, v, m, dt, substeps):
+    """The oracle's reference solver (main_cpu.cpp:602-700 restated in C: MGS
+    with two passes, one thread) on one full sub-step of the GPU solve: one
+    attempt of length dt from the same start vector (m columns, the
+    exponential, the lift; the tolerance is set so that the attempt is
+    accepted), timed whole.  The solve is `substeps` such sub-steps of the same
+    cost, so its rate is 1 / (substeps x that time)."""
+    pyoracle = _oracle()
+    with _one_core() as core:
+        t0 = time.perf_counter()
+        _y, st, _log = pyoracle.expmv(rp, ci, va, v, dt, 1e300, m, 1)
+        wall = time.perf_counter() - t0
+    assert st["attempts"] == 1 and st["spmv_count"] == m, st
+    return {
+        "value": 1.0 / (substeps * wall),
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "host": _host_desc(core),
+        "sample": (f"oracle solver (main_cpu.cpp:602-700 restated in C: SpMV, MGS with two passes, "
+                   f"Pade expm, lift) on one full sub-step of the same solve (dt = {dt:.4g}, {m} columns "
+                   f"from the same start vector), {wall:.1f} s wall; the GPU solve is {substeps} such "
+                   f"sub-steps"),
+    }
+
+
 def cpu_baseline_batch(mats, ts, ms, vs, tol, seconds_budget):
     """The oracle's reference solver (MGS2, reference step controller with the
     first step t, tol absolute per step) on the batch's problems in turn, one
     thread, until the budget is spent: problems/s on that sample."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    pyoracle.build()
-    prev = os.sched_getaffinity(0)
-    core = min(prev)
-    os.sched_setaffinity(0, {core})
-    try:
-        t0 = time.time()
+    pyoracle = _oracle()
+    with _one_core() as core:
+        t0 = time.perf_counter()
         k = conv = 0
         for (rp, ci, va), t, m, v in zip(mats, ts, ms, vs):
             _y, st, _log = pyoracle.expmv(rp, ci, va, v, t, tol, m, 0)
             conv += int(st["converged"])
             k += 1
-            if time.time() - t0 > seconds_budget:
+            if time.perf_counter() - t0 > seconds_budget:
                 break
-        wall = time.time() - t0
-    finally:
-        os.sched_setaffinity(0, prev)
+        wall = time.perf_counter() - t0
     rows = sum(int(rp.size - 1) for rp, _, _ in mats[:k])
     return {
         "value": k / wall,
@@ -162,23 +177,72 @@ def cpu_baseline_batch(mats, ts, ms, vs, tol, seconds_budget):
     }
 
 
-def run_batch(args, mek, gen):
-    """256 independent problems (n log-uniform in [1e3, 2e5], random rows with
-    5-30 nonzeros and a Gershgorin-negative diagonal, t ~ U[0.1, 5],
-    m ~ U{20..60}).  Plain form: one ArnoldiRunner solve per problem in turn
-    (FP64 values, Expokit step control).  Batched forms: every problem in one
-    lockstep solve replayed from a hipGraph with Expokit step control on the
-    device, FP64 then FP32 values, then with (m, first sub-step) from the
-    learned selector and FP32 values.  Every form uses the DGKS second pass
-    and Expokit's per-unit-time tolerance tol/t, relative to the norm of each
-    step's start vector.
-    Errors are against a tight-tolerance (1e-13, m = 60) FP64 solve of each
-    problem."""
+def random_n2000_case(mek, gen, with_cpu):
+    """The reference's own CPU-runnable case: n = 2000, int(5e-3 n^2)
+    uniformly random off-diagonal draws (duplicates summed), values U[-1,1]
+    from seed 42, a_ii = -1 - sum_j |a_ij|; v = 0.1 ones; t = 1.0 (exact, not
+    the CLI's clamp), m = 30, tol 1e-6, 10 restarts, the reference
+    controller.  GPU: median of 5 solves after one warm-up; oracle: median of
+    3 runs on one core.  Relative error of the GPU y against the oracle's."""
+    rp, ci, va = gen.random_negative(2000, 5e-3, seed=42)
+    n = rp.size - 1
+    v = np.full(n, 0.1)
+    t, m, tol, R_ = 1.0, 30, 1e-6, 10
+    R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=m, t=t, tol=tol, max_restarts=R_)
+    y = R.expmv(v)
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        y = R.expmv(v)
+        times.append(time.perf_counter() - t0)
+    st = R.stats()
+    R.close()
+    out = {"workload": "random n=2000, density 5e-3, seed 42; v = 0.1; t = 1, m = 30, tol 1e-6, "
+                       "10 restarts, reference controller",
+           "nnz": int(rp[-1]), "gpu_solves_per_s": 1.0 / statistics.median(times),
+           "gpu_span": "compute_expmv, host v and y (the METRIC span)",
+           "accepted_steps": st["accepted"], "attempts": st["attempts"], "converged": bool(st["converged"])}
+    if with_cpu:
+        pyoracle = _oracle()
+        cpu = []
+        with _one_core() as core:
+            for _ in range(3):
+                t0 = time.perf_counter()
+                yo, sto, _ = pyoracle.expmv(rp, ci, va, v, t, tol, m, R_)
+                cpu.append(time.perf_counter() - t0)
+        out["cpu_solves_per_s"] = 1.0 / statistics.median(cpu)
+        out["cpu"] = {"kind": "port", "cores": 1, "host": _host_desc(core),
+                      "sample": "the whole solve, median of 3 runs"}
+        out["rel_err_vs_oracle"] = float(np.linalg.norm(y - yo) / np.linalg.norm(yo))
+        out["oracle_steps"] = {"accepted": sto["restarts_done"], "attempts": sto["attempts"]}
+    return out
+
+
+# --------------------------------------------------------------------- batch
+
+def batch_comparison(args, mek, gen, reps, cpu_seconds):
+    """256 independent problems: n_i = round(exp(U[ln 1e3, ln 2e5])), random
+    rows with 5-30 nonzeros, values U[-1,1] in FP64, Gershgorin-negative
+    diagonal, t_i ~ U[0.1, 5], m_i ~ U{20..60}, v_i ~ N(0,1) normalised (seed 0).
+
+    Forms (all DGKS second pass, Expokit step control with the per-unit-time
+    tolerance tol/t_i relative to each step's start norm):
+    * plain_fp64: one ArnoldiRunner solve per problem in turn (m_i), v and y
+      resident;
+    * batched_fp64: every problem in one lockstep solve replayed from hipGraphs;
+    * batched_mixed: the same with FP32 values requested -- stored in FP32 only
+      where that is exact, so here FP64 (values_f32_used says which);
+    * batched_mixed_learned: (m, first sub-step) from the learned selector; its
+      problems/s includes the selection (device features, MLP, decoding);
+    * batched_f32_forced: FP32 storage regardless, to show what it costs in
+      error (not a form that meets 1e-8).
+    Errors are against a tight (1e-13, m = 60, second pass always) FP64 solve
+    of each problem.  setup_s (host assembly and upload, excluded from the
+    rates) is reported per form."""
     sel = _load("mek_selector", os.path.join(ROOT, "matrix-exp-krylov_amd", "selector.py"))
-    w = WORKLOADS["batch256"]
-    tol = w["tol"]
-    log("generating 256 problems")
-    probs = gen.batch_problems(P=w["P"], seed=0, f32_exact=True)
+    tol = 1e-8
+    log("batch: generating 256 problems (FP64 values)")
+    probs = gen.batch_problems(P=256, seed=0, f32_exact=False)
     P = len(probs)
     mats = [(q["rp"], q["ci"], q["va"]) for q in probs]
     ts = [q["t"] for q in probs]
@@ -187,13 +251,13 @@ def run_batch(args, mek, gen):
     rows = [q["rp"].size - 1 for q in probs]
     nnz_tot = int(sum(int(q["rp"][-1]) for q in probs))
     csrs = [mek.CSR.from_arrays(*A) for A in mats]
-    # FP32 value storage is lossless only for values exact in FP32: say so
-    f32_check = [c.f32_inexact() for c in csrs]
-    f32_values = {"inexact": int(sum(c for c, _ in f32_check)),
-                  "max_rel_change": max(m for _, m in f32_check)}
+    chk = [c.f32_inexact(nonfinite=True) for c in csrs]
+    f32_check = {"inexact": int(sum(c for c, _, _ in chk)), "nonfinite": int(sum(b for _, _, b in chk)),
+                 "max_rel_change": _finite(max(m for _, m, _ in chk)), "values": nnz_tot}
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+    tols = [tol / t for t in ts]
 
-    log("reference solves (tol 1e-13)")
+    log("batch: tight FP64 reference solves (tol 1e-13)")
     refs = []
     for c, t, v, n in zip(csrs, ts, vs, rows):
         R = mek.Runner(c, m=min(60, n - 1), t=t, tol=1e-13, max_restarts=0, step_control="expokit_rel",
@@ -204,20 +268,16 @@ def run_batch(args, mek, gen):
     def max_err(ys):
         return max(float(np.linalg.norm(y - r) / max(np.linalg.norm(r), 1e-300)) for y, r in zip(ys, refs))
 
-    # plain FP64: one runner per problem, solves in turn, v and y resident
-    log("plain FP64, one problem at a time")
-    # Expokit's tolerance is per unit time: tol / t bounds the summed local
-    # errors of each problem by about tol, measured relative to the norm of
-    # each step's start vector (relative mode)
-    tols = [tol / t for t in ts]
+    log("batch: plain FP64, one problem at a time")
+    t0 = time.perf_counter()
     runners = [mek.Runner(c, m=m, t=t, tol=tl, max_restarts=0, step_control="expokit_rel", gs_mode=gs_mode)
                for c, m, t, tl in zip(csrs, ms, ts, tols)]
     dv = [mek.DeviceBuffer.from_array(v) for v in vs]
     dy = [mek.DeviceBuffer(8 * n) for n in rows]
+    setup_plain = time.perf_counter() - t0
     for R, a, b in zip(runners, dv, dy):  # warm-up
         R.expmv_device(a, b)
     mek.synchronize()
-    reps = max(1, args.steps)
     t0 = time.perf_counter()
     for _ in range(reps):
         for R, a, b in zip(runners, dv, dy):
@@ -230,14 +290,14 @@ def run_batch(args, mek, gen):
         R.close()
     del dv, dy
 
-    heur_s = [0] * P  # Expokit's own first step
-
     def batched(name, m_list, s_list, f32, select_s=0.0, spmv_timing=False):
-        log(f"batched: {name}")
+        log(f"batch: {name}")
+        t0 = time.perf_counter()
         B = mek.Batch(csrs, ts, m_list, tols, substeps=s_list, values_f32=f32, gs_mode=gs_mode,
                       step_control="expokit_rel", autotune=args.autotune)
         B.upload(vs)
-        B.solve()  # warm-up: captures the cycle graph
+        setup = time.perf_counter() - t0
+        B.solve()  # warm-up: captures the cycle graphs
         mek.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -246,8 +306,8 @@ def run_batch(args, mek, gen):
         el = (time.perf_counter() - t0) / reps
         ys = B.download()
         st = [B.stats(p) for p in range(P)]
-        out = {"problems_per_s": P / (el + select_s), "solve_s": el, "select_s": select_s,
-               "spmv_grid": B.spmv_grid() if args.autotune else None,
+        out = {"problems_per_s": P / (el + select_s), "solve_s": el, "select_s": select_s, "setup_s": setup,
+               "values_f32_used": B.values_f32_used(), "spmv_grid": B.spmv_grid() if args.autotune else None,
                "cycles": B.cycles(), "krylov_columns": int(sum(x["spmv_count"] for x in st)),
                "converged": int(sum(x["converged"] for x in st)), "max_rel_err": max_err(ys)}
         if spmv_timing:
@@ -255,15 +315,18 @@ def run_batch(args, mek, gen):
         B.close()
         return out
 
-    res = {"plain_fp64": {"problems_per_s": P / plain_s, "solve_s": plain_s, "krylov_columns": plain_cols,
-                          "max_rel_err": plain_err}}
-    res["batched_fp64"] = batched("FP64, Expokit step control", ms, heur_s, False)
-    res["batched_mixed"] = batched("FP32 values, Expokit step control", ms, heur_s, True,
-                                   spmv_timing=not os.path.exists(sel.WEIGHTS))
+    heur_s = [0] * P  # Expokit's own first step
+    res = {"plain_fp64": {"problems_per_s": P / plain_s, "solve_s": plain_s, "setup_s": setup_plain,
+                          "krylov_columns": plain_cols, "max_rel_err": plain_err}}
+    res["batched_fp64"] = batched("FP64 values, Expokit step control", ms, heur_s, False)
+    res["batched_mixed"] = batched("FP32 values requested, Expokit step control", ms, heur_s, True)
     learned = None
     if os.path.exists(sel.WEIGHTS):
         S = sel.Selector.load()
+        t0 = time.perf_counter()
         FB = mek.Batch(csrs, ts, ms, tol, values_f32=True, gs_mode=gs_mode)
+        fsetup = time.perf_counter() - t0
+        mek.synchronize()
         t0 = time.perf_counter()
         F = FB.features()
         choice = S.predict_features(F)
@@ -271,45 +334,75 @@ def run_batch(args, mek, gen):
         FB.close()
         lm = [min(m, n - 1) for (m, s), n in zip(choice, rows)]
         ls = [s for m, s in choice]
-        learned = batched("FP32 values, learned (m, sub-steps)", lm, ls, True, select_s, spmv_timing=True)
+        learned = batched("FP32 values requested, learned (m, sub-steps)", lm, ls, True, select_s,
+                          spmv_timing=True)
+        learned["setup_s"] += fsetup
+        learned["m_chosen_hist"] = np.bincount(lm, minlength=61)[10::5].tolist()
         res["batched_mixed_learned"] = learned
+    forced = batched("FP32 values forced (lossy)", ms, heur_s, "lossy")
+    forced["meets_1e-8"] = forced["max_rel_err"] <= 1e-8
+    res["batched_f32_forced"] = forced
     head = learned if learned is not None else res["batched_mixed"]
     form = "batched_mixed_learned" if learned is not None else "batched_mixed"
-    # the SpMV dominates the batched solve (about 60 % of its kernel time):
-    # one launch over every problem's start vector, FP32 values
+    if "spmv_us" not in head:
+        head["spmv_us"] = float("nan")
+    # the batched SpMV: one launch over every problem's start vector, with the
+    # headline form's value storage
     rows_tot = int(sum(rows))
-    spmv_bytes = nnz_tot * (4 + 4) + 4.0 * (rows_tot + 1) + 16.0 * rows_tot
+    vb = 4 if head["values_f32_used"] else 8
+    spmv_bytes = nnz_tot * (vb + 4) + 4.0 * (rows_tot + 1) + 16.0 * rows_tot
     spmv_gbs = spmv_bytes / (head["spmv_us"] * 1e-6) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(spmv_gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "batch spmv",
-                "bytes_per_launch": spmv_bytes, "avg_launch_us": head["spmv_us"],
+    roofline = {"bound": "hbm", "achieved": _finite(round(spmv_gbs, 1)), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": _finite(round(spmv_gbs / PEAK_HBM_GBS, 4)), "traffic": None, "kernel": "batch spmv",
+                "bytes_per_launch": spmv_bytes, "avg_launch_us": _finite(head["spmv_us"]),
                 "note": ("first-column SpMV over all problems (median of 20 HIP-event-timed launches on "
-                         "the batch's stream); bound by the rate of random-column gathers, not HBM")}
+                         "the batch's stream)")}
     cpu = None
-    if args.cpu_seconds > 0:
-        log("CPU baseline (oracle, one thread, bounded sample)")
-        cpu = cpu_baseline_batch(mats, ts, ms, vs, tol, args.cpu_seconds)
-    out = {
-        "metric": "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64",
+    if cpu_seconds > 0:
+        log("batch: CPU baseline (oracle, one thread, bounded sample)")
+        cpu = cpu_baseline_batch(mats, ts, ms, vs, tol, cpu_seconds)
+    speed = head["problems_per_s"] / res["plain_fp64"]["problems_per_s"]
+    return {
+        "workload": ("batch256: 256 problems, n_i log-uniform in [1e3, 2e5], random rows with 5-30 "
+                     "nonzeros, values U[-1,1] FP64, Gershgorin-negative diagonal, t_i ~ U[0.1, 5], "
+                     "m_i ~ U{20..60}, seed 0"),
+        "problems": P, "rows": rows_tot, "nnz": nnz_tot, "tol": tol, "gs": args.gs,
+        "unit": "problems/s", "headline_form": form,
         "value": round(head["problems_per_s"], 3),
+        "rel_err": head["max_rel_err"],
+        "speedup_vs_plain_fp64": round(speed, 3),
+        "target_speedup": 3.0, "target_met": bool(speed >= 3.0 and head["max_rel_err"] <= 1e-8),
+        "values_f32_check": f32_check,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": res,
+    }
+
+
+def run_batch(args, mek, gen):
+    b = batch_comparison(args, mek, gen, max(1, args.steps), args.cpu_seconds)
+    head = b["forms"][b["headline_form"]]
+    out = {
+        "metric": METRIC,
+        "value": b["value"],
         "unit": "problems/s",
         "n_gpus": 1,
-        "steps": reps,
+        "steps": max(1, args.steps),
         "warmup": 1,
         "ms_per_step": round(1000.0 * (head["solve_s"] + head.get("select_s", 0.0)), 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-values/f64",
+        "dtype": "f32-values/f64" if head["values_f32_used"] else "f64",
         "data": "synthetic",
-        "config": {"workload": "batch256", "problems": P, "rows": int(sum(rows)), "nnz": nnz_tot,
-                   "tol": tol, "gs": args.gs, "headline_form": form, "seed": 0},
-        "rel_err": head["max_rel_err"],
-        "values_f32": f32_values,
-        "speedup_vs_plain_fp64": round(head["problems_per_s"] / res["plain_fp64"]["problems_per_s"], 3),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "forms": res,
+        "config": {"workload": "batch256", "problems": b["problems"], "rows": b["rows"], "nnz": b["nnz"],
+                   "tol": b["tol"], "gs": b["gs"], "headline_form": b["headline_form"], "seed": 0},
+        "rel_err": b["rel_err"],
+        "values_f32": b["values_f32_check"],
+        "speedup_vs_plain_fp64": b["speedup_vs_plain_fp64"],
+        "roofline": b["roofline"],
+        "cpu_baseline": b["cpu_baseline"],
+        "forms": b["forms"],
     }
     print(json.dumps(out), flush=True)
 
@@ -323,7 +416,7 @@ def run_block(args, mek, gen):
     single-vector solver (m = 40, DGKS, Expokit steps) on 8 of the columns in
     turn.  Errors against the DST-I closed form of every column."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from closed_forms import laplacian_2d_expmv
+    from closed_forms import laplacian_2d_expmv_block
     w = WORKLOADS["block2048"]
     N, t, tol, m = w["N"], w["t"], w["tol"], args.block_m or w["m"]
     n = N * N
@@ -333,26 +426,36 @@ def run_block(args, mek, gen):
     V /= np.linalg.norm(V, axis=0)
     A = mek.CSR.from_arrays(rp, ci, va)
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
-    B = mek.Block(A, m=m, t=t, tol=tol, gs_mode=gs_mode, timing=True, autotune=args.autotune)
+    B = mek.Block(A, m=m, t=t, tol=tol, gs_mode=gs_mode, timing=False, autotune=args.autotune)
     d_v = mek.DeviceBuffer.from_array(V.ravel())
     d_y = mek.DeviceBuffer(8 * V.size)
     for i in range(args.warmup):
         log(f"warmup {i + 1}/{args.warmup}")
         B.expmv_device(d_v, d_y)
     mek.synchronize()
-    kern = {k: {"ms": 0.0, "launches": 0, "bytes": 0.0} for k in mek.BLOCK_KERNEL_KINDS}
     log(f"timing {args.steps} block solves")
     t0 = time.perf_counter()
     for i in range(args.steps):
         B.expmv_device(d_v, d_y)
-        st = B.stats()
-        for k in kern:
-            for f in kern[k]:
-                kern[k][f] += st["kernels"][k][f]
     mek.synchronize()
     ms_per_step = (time.perf_counter() - t0) * 1000.0 / args.steps
+    st = B.stats()
     Y = d_y.to_array().reshape(V.shape)
     B.close()
+    # per-kernel event times in a pass of their own (event pairs around every
+    # launch would sit inside the timed loop otherwise)
+    log("event-timed pass")
+    Bt = mek.Block(A, m=m, t=t, tol=tol, gs_mode=gs_mode, timing=True, autotune=args.autotune)
+    Bt.expmv_device(d_v, d_y)
+    kern = {k: {"ms": 0.0, "launches": 0, "bytes": 0.0} for k in mek.BLOCK_KERNEL_KINDS}
+    tsteps = 2
+    for _ in range(tsteps):
+        Bt.expmv_device(d_v, d_y)
+        s2 = Bt.stats()
+        for k in kern:
+            for f in kern[k]:
+                kern[k][f] += s2["kernels"][k][f]
+    Bt.close()
     log("plain single-vector solves of 8 columns")
     R = mek.Runner(A, m=40, t=t, tol=tol, max_restarts=0, step_control="expokit", gs_mode=2)
     cols = [mek.DeviceBuffer.from_array(np.ascontiguousarray(V[:, c])) for c in range(8)]
@@ -367,8 +470,8 @@ def run_block(args, mek, gen):
     single_cols = R.stats()["spmv_count"]
     R.close()
     log("closed-form check of the 32 columns")
-    errs = [float(np.linalg.norm(Y[:, c] - laplacian_2d_expmv(N, t, V[:, c]))
-                  / np.linalg.norm(laplacian_2d_expmv(N, t, V[:, c]))) for c in range(32)]
+    ref = laplacian_2d_expmv_block(N, t, V)
+    errs = [float(np.linalg.norm(Y[:, c] - ref[:, c]) / np.linalg.norm(ref[:, c])) for c in range(32)]
 
     def roof(k):
         kt = kern[k]
@@ -390,13 +493,17 @@ def run_block(args, mek, gen):
             roofs[k]["traffic"] = tr["block_" + k]
     cpu = None
     if args.cpu_seconds > 0:
-        log("CPU baseline (oracle, one thread, bounded sample)")
-        cpu = cpu_baseline(rp, ci, va, V[:, 0].copy(), single_cols, 40, n, args.cpu_seconds)
+        log("CPU baseline (oracle, one thread, one sub-step of one column's solve)")
+        R = mek.Runner(A, m=40, t=t, tol=tol, max_restarts=0, step_control="expokit", gs_mode=2)
+        R.expmv(V[:, 0].copy())
+        steps = R.steps()
+        R.close()
+        cpu = cpu_baseline_substep(rp, ci, va, V[:, 0].copy(), 40, steps[0]["dt"], len(steps))
         cpu["unit"] = "rhs/s"  # one right-hand side per single-vector solve
         cpu["sample"] += "; one right-hand side per solve"
     rhs_per_s = 32 * 1000.0 / ms_per_step
     out = {
-        "metric": "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64",
+        "metric": METRIC,
         "value": round(rhs_per_s, 3),
         "unit": "rhs/s",
         "n_gpus": 1,
@@ -419,43 +526,13 @@ def run_block(args, mek, gen):
         "speedup_vs_single_vector": round(rhs_per_s / (1000.0 / single_ms), 3),
         "roofline": roofs[dominant],
         "spmm": roofs["spmm"],
-        "kernels_ms_per_solve": {k: round(kern[k]["ms"] / args.steps, 3) for k in kern if kern[k]["launches"]},
+        "kernels_ms_per_solve": {k: round(kern[k]["ms"] / tsteps, 3) for k in kern if kern[k]["launches"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS))
-    ap.add_argument("--step-control", default="expokit", choices=["expokit", "reference"])
-    ap.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"],
-                    help="second GS pass: dgks = only when ||w'|| < 0.1 ||w|| (default), always = reference MGS2 semantics")
-    ap.add_argument("--fuse", action="store_true", help="first-pass dots inside the SpMV kernel")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="keep the default launch shapes (default: time candidates when the runner is created)")
-    ap.add_argument("--lanczos", action="store_true",
-                    help="symmetric Lanczos recurrence instead of Gram-Schmidt (laplacian4096 only)")
-    ap.add_argument("--cpu-seconds", type=float, default=25.0, help="CPU baseline budget (0: skip)")
-    ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: the solver runs on one GPU; use --gpus 1 (each problem's rows would be split "
-              "across GPUs with a halo exchange per SpMV and one all-reduce per Gram-Schmidt pass)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    mek = _load("mek", os.path.join(ROOT, "matrix-exp-krylov_amd", "__init__.py"))
-    gen = _load("mek_generate", os.path.join(ROOT, "matrix-exp-krylov_amd", "generate.py"))
-    if args.workload == "batch256":
-        run_batch(args, mek, gen)
-        return
-    if args.workload == "block2048":
-        run_block(args, mek, gen)
-        return
+def run_single(args, mek, gen):
     w = WORKLOADS[args.workload]
     log(f"building {args.workload}")
     if w["kind"] == "laplacian":
@@ -468,14 +545,18 @@ def main():
     v = gen.normal_vector(n, seed=0)
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
     A = mek.CSR.from_arrays(rp, ci, va)
-    R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
-                   gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=True,
-                   fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune)
+
+    def runner(timing):
+        return mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
+                          gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=timing,
+                          fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune)
+
+    R = runner(False)
     f32_values = None
     if w["f32"]:
-        c, mrel = A.f32_inexact()
-        f32_values = {"inexact": int(c), "max_rel_change": mrel}
-    A.close()
+        c, mrel, bad = A.f32_inexact(nonfinite=True)
+        f32_values = {"inexact": int(c), "nonfinite": int(bad), "max_rel_change": _finite(mrel),
+                      "values_f32_used": R.values_f32_used()}
     d_v = mek.DeviceBuffer.from_array(v)
     d_y = mek.DeviceBuffer(8 * n)
     mek.synchronize()
@@ -484,23 +565,45 @@ def main():
         log(f"warmup {i + 1}/{args.warmup}")
         R.expmv_device(d_v, d_y)
     mek.synchronize()
-    kern = {k: {"ms": 0.0, "launches": 0, "bytes": 0.0} for k in mek.KERNEL_KINDS}
-    log(f"timing {args.steps} solves")
+    log(f"timing {args.steps} solves (no per-kernel events)")
     t0 = time.perf_counter()
-    stats = None
     for i in range(args.steps):
         R.expmv_device(d_v, d_y)
-        for k in kern:  # per-solve event totals (each solve resets them)
-            kt = R.kernel_time(k)
-            for f in kern[k]:
-                kern[k][f] += kt[f]
-        stats = R.stats()
     mek.synchronize()
     elapsed = time.perf_counter() - t0
+    stats = R.stats()
     ms_per_step = elapsed * 1000.0 / args.steps
-    log(f"{ms_per_step:.1f} ms per solve; checking against the closed form")
-
+    log(f"{ms_per_step:.1f} ms per solve")
     y = d_y.to_array()
+
+    # the reference's METRIC span: host v in, host y out
+    span_steps = max(1, min(args.steps, 5))
+    log(f"timing {span_steps} solves on the METRIC span (host v and y)")
+    span = []
+    for _ in range(span_steps):
+        t0 = time.perf_counter()
+        yh = R.expmv(v)
+        span.append(time.perf_counter() - t0)
+    assert np.array_equal(yh, y)
+    tuning = R.tuning() if args.autotune else None
+    R.close()
+
+    # per-kernel event timing in a pass of its own
+    log("event-timed pass")
+    Rt = runner(True)
+    Rt.expmv_device(d_v, d_y)
+    kern = {k: {"ms": 0.0, "launches": 0, "bytes": 0.0} for k in mek.KERNEL_KINDS}
+    tsteps = 2
+    for _ in range(tsteps):
+        Rt.expmv_device(d_v, d_y)
+        for k in kern:  # per-solve event totals (each solve resets them)
+            kt = Rt.kernel_time(k)
+            for f in kern[k]:
+                kern[k][f] += kt[f]
+    Rt.close()
+    A.close()
+
+    log("checking against the closed form")
     ref = reference_solution(w, v, gen)
     s = max(np.abs(ref).max(), 1e-300)
     rel = float(np.linalg.norm((y - ref) / s) / np.linalg.norm(ref / s))
@@ -529,14 +632,9 @@ def main():
     if spmv_roof is not None and "spmv" in tr:
         spmv_roof["traffic"] = tr["spmv"]
 
-    cpu = None
-    if args.cpu_seconds > 0:
-        log("CPU baseline (oracle, one thread, bounded sample)")
-        cpu = cpu_baseline(rp, ci, va, v, stats["spmv_count"], w["m"], n, args.cpu_seconds)
-
     solves_per_s = 1000.0 / ms_per_step
     out = {
-        "metric": "exp(tA)v solves/s + SpMV HBM GB/s at rel err <=1e-8; learned+mixed vs plain FP64",
+        "metric": METRIC,
         "value": round(solves_per_s, 4),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -546,26 +644,81 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-values/f64" if w["f32"] else "f64",
+        "dtype": "f32-values/f64" if (w["f32"] and f32_values["values_f32_used"]) else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "t": w["t"], "m": w["m"],
                    "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
                    "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "autotune": args.autotune, "seed": 0},
         "rel_err": rel,
         "values_f32": f32_values,
+        "metric_span": {"solves_per_s": round(1.0 / statistics.median(span), 4),
+                        "ms_per_solve": round(1000.0 * statistics.median(span), 3),
+                        "span": "compute_expmv with host v and y (main.cu:1412-1570): the H2D of v and the "
+                                "D2H of y over PCIe included; median of the solves"},
         "solve": {"substeps": stats["accepted"], "expm_attempts": stats["attempts"],
                   "krylov_columns": stats["spmv_count"], "extra_spmv": stats["extra_spmv"],
                   "converged": bool(stats["converged"])},
         "roofline": roofline,
         "spmv": spmv_roof,
-        "kernels_ms_per_solve": {k: round(kern[k]["ms"] / args.steps, 3) for k in kern if kern[k]["launches"]},
+        "kernels_ms_per_solve": {k: round(kern[k]["ms"] / tsteps, 3) for k in kern if kern[k]["launches"]},
         # algorithmic bytes of every timed kernel of a solve / the solve's wall time
-        "solve_effective_gbs": round(sum(kern[k]["bytes"] for k in kern) / args.steps / (ms_per_step * 1e-3) / 1e9, 1),
-        "launch_shapes": R.tuning() if args.autotune else None,
-        "cpu_baseline": cpu,
+        "solve_effective_gbs": round(sum(kern[k]["bytes"] for k in kern) / tsteps / (ms_per_step * 1e-3) / 1e9, 1),
+        "launch_shapes": tuning,
+        "cpu_baseline": None,
     }
+    if args.cpu_seconds > 0:
+        log("CPU baseline (oracle, one thread, one full sub-step)")
+        steps = [None]
+        Rs = runner(False)
+        Rs.expmv(v)
+        steps = Rs.steps()
+        Rs.close()
+        out["cpu_baseline"] = cpu_baseline_substep(rp, ci, va, v, w["m"], steps[0]["dt"], len(steps))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS))
+    ap.add_argument("--step-control", default="expokit", choices=["expokit", "reference"])
+    ap.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"],
+                    help="second GS pass: dgks = only when ||w'|| < 0.1 ||w|| (default), always = reference MGS2 semantics")
+    ap.add_argument("--fuse", action="store_true", help="first-pass dots inside the SpMV kernel")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shapes (default: time candidates when the runner is created)")
+    ap.add_argument("--lanczos", action="store_true",
+                    help="symmetric Lanczos recurrence instead of Gram-Schmidt (laplacian4096 only)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline budget of the batch sample (0: skip every CPU baseline)")
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="leave the batched comparison out of the default line")
+    ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: the solver runs on one GPU; use --gpus 1 (each problem's rows would be split "
+              "across GPUs with a halo exchange per SpMV and one all-reduce per Gram-Schmidt pass)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    mek = _load("mek", os.path.join(ROOT, "matrix-exp-krylov_amd", "__init__.py"))
+    gen = _load("mek_generate", os.path.join(ROOT, "matrix-exp-krylov_amd", "generate.py"))
+    if args.workload == "batch256":
+        run_batch(args, mek, gen)
+        return
+    if args.workload == "block2048":
+        run_block(args, mek, gen)
+        return
+    out = run_single(args, mek, gen)
+    if args.workload == "laplacian4096" and not (args.fuse or args.lanczos):
+        log("reference CPU case (n = 2000)")
+        out["random_n2000"] = random_n2000_case(mek, gen, args.cpu_seconds > 0)
+        if args.batch:
+            out["learned_mixed_batched"] = batch_comparison(args, mek, gen, max(1, min(args.steps, 3)),
+                                                            args.cpu_seconds)
     print(json.dumps(out), flush=True)
-    R.close()
 
 
 if __name__ == "__main__":

@@ -63,10 +63,11 @@ int mek_csr_info(const mek_csr* A, int* rows, int* cols, long long* nnz, int* va
 /* Copy the arrays out (row_ptr: rows+1, col_idx/values: nnz entries). */
 int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values);
 /* How many values FP32 storage (values_f32) would change, and the largest
- * relative change among them (0 when FP32 storage is lossless).  No
- * reference counterpart: the check a caller makes before choosing FP32
- * values under a tight tolerance. */
-int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel);
+ * relative change among the finite ones (0 when FP32 storage is lossless).
+ * *nonfinite (may be NULL) counts the values that are NaN or infinite or lie
+ * beyond FP32's range; they are included in *count.  No reference
+ * counterpart: the check the solvers make for values_f32 = 1. */
+int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel, long long* nonfinite);
 void mek_csr_destroy(mek_csr* A);
 
 /* ---------------------------------------------------------------- solver */
@@ -89,7 +90,9 @@ typedef struct {
     int gs_mode;        /* -1 follow reorthogonalize, 0 never, 1 always,
                            2 conditional (DGKS, ||w'|| < dgks_eta ||w||)      */
     double dgks_eta;
-    int values_f32;     /* store matrix values in FP32 on the device          */
+    int values_f32;     /* matrix values on the device: 0 FP64; 1 FP32 when
+                           every value is exact in FP32, else FP64; 2 FP32
+                           always (the caller accepts the perturbation)       */
     int verbose;        /* print the reference's per-column / [STEP] / METRIC */
     int timing;         /* HIP-event timing of every kernel launch            */
     int fuse_spmv_dot;  /* first-pass dot products inside the SpMV kernel     */
@@ -117,7 +120,9 @@ typedef struct {
 typedef struct mek_runner mek_runner;
 
 /* ArnoldiRunner(1, params, rows, cols, nnz) + init_all_devices on device
- * `device_id` with the whole matrix (one GPU).  reference: main.cu:1286-1409 */
+ * `device_id` with the whole matrix (one GPU).  reference: main.cu:1286-1409.
+ * params->t must be finite and >= 0 (t = 0 gives y = v; the reference's
+ * controller never runs for t <= 0 and leaves y unset). */
 int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* params,
                       const mek_solver_options* options, int device_id, mek_runner** out);
 /* y = exp(t A) v, host vectors of n doubles.  reference:
@@ -125,8 +130,12 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* params,
 int mek_runner_expmv(mek_runner* R, const double* v, double* y);
 /* Same with device-resident v and y (n doubles each). */
 int mek_runner_expmv_device(mek_runner* R, const double* d_v, double* d_y);
-/* Change t (and the derived TimeSteppingParams) between solves. */
+/* Change t (and the derived TimeSteppingParams) between solves; t finite
+ * and >= 0. */
 int mek_runner_set_t(mek_runner* R, double t);
+/* 1 when the runner stores the matrix values in FP32 (values_f32 = 2, or 1
+ * with every value exact in FP32), else 0. */
+int mek_runner_values_f32(const mek_runner* R, int* used);
 int mek_runner_stats(const mek_runner* R, mek_solve_stats* out);
 /* Per-attempt [STEP] records of the last solve: t, dt, err (3 doubles) and
  * m_used, accepts, rejects (3 ints) per record; returns the count in *n. */
@@ -158,7 +167,9 @@ void mek_runner_destroy(mek_runner* R);
  * 0 keeps max_restarts).  step_control 1: Expokit step control (first step
  * from Expokit's formula, or t/substeps[p] when substeps[p] > 0, e.g. from
  * the learned selector); 2: the same with the local error measured relative
- * to the norm of each step's start vector. */
+ * to the norm of each step's start vector.  t[p] finite and >= 0 (t[p] = 0
+ * gives y_p = v_p).  values_f32: 0 FP64 values; 1 FP32 when every value of
+ * every problem is exact in FP32, else FP64; 2 FP32 always. */
 typedef struct mek_batch mek_batch;
 int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const int* m,
                      const double* tol, const int* max_restarts, const int* substeps,
@@ -166,6 +177,9 @@ int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const 
                      int autotune, int device_id, mek_batch** out);
 /* SpMV grid the autotuner chose at the first solve (0 without autotune). */
 int mek_batch_tuning(const mek_batch* B, int* spmv_grid);
+/* 1 when the batch stores the values in FP32 (values_f32 = 2, or 1 with
+ * every value of every problem exact in FP32), else 0. */
+int mek_batch_values_f32(const mek_batch* B, int* used);
 /* Median microseconds of one batched SpMV over every problem's start vector
  * (all problems active, as in the first Arnoldi column of a solve), over
  * reps launches timed with HIP events on the batch's stream.  Call after

@@ -63,7 +63,10 @@ struct SolverOptions {
   // (always / never); 2 = conditional (DGKS test ||w'|| < eta ||w||).
   int gs_mode = -1;
   double dgks_eta = 0.1;
-  bool values_f32 = false;  // store matrix values in FP32 on the device
+  // Matrix value storage on the device: 0 FP64; 1 FP32 when every value is
+  // exact in FP32, else FP64 (FP32 storage would move y by about t ||dA||,
+  // far above a 1e-8 tolerance); 2 FP32 always (the caller accepts the loss).
+  int values_f32 = 0;
   bool verbose = true;      // print the reference's per-column and [STEP] lines
   bool timing = false;      // HIP-event timing of every kernel launch
   bool fuse_spmv_dot = false;  // first-pass dots inside the SpMV kernel
@@ -126,6 +129,7 @@ struct ArnoldiRunner {
   mek::SolverOptions options;
   mek::SolveStats stats;
   mek::TuneChoice tuning;  // additive: launch shapes chosen when options.autotune
+  bool values_f32_used = false;  // additive: the values are stored in FP32
   double anorm_inf = 0.0;   // ||A||_inf, for the Expokit initial step
 
   ArnoldiRunner(int n_gpus, const ArnoldiParams& arnoldi_params, int total_rows, int total_cols,

@@ -38,9 +38,12 @@ struct BatchProblemStats {
 
 class BatchRunner {
  public:
+  // values_f32: 0 FP64 values; 1 FP32 when every value of every problem is
+  // exact in FP32, else FP64; 2 FP32 always (the caller accepts the loss).
   // gs_mode: 0 never, 1 always (reference), 2 DGKS with eta.
   // autotune: time the SpMV grid at the first solve and keep the fastest.
-  BatchRunner(int device_id, const std::vector<BatchProblem>& problems, bool values_f32,
+  // t_p must be finite and >= 0; a problem with t_p = 0 returns y_p = v_p.
+  BatchRunner(int device_id, const std::vector<BatchProblem>& problems, int values_f32,
               int gs_mode, double dgks_eta, bool use_graph, bool autotune = false);
   ~BatchRunner();
   BatchRunner(const BatchRunner&) = delete;
@@ -68,6 +71,7 @@ class BatchRunner {
   double time_spmv(int reps);
   int cycles() const { return cycles_; }
   int spmv_grid() const { return tuned_spmv_grid_; }
+  bool values_f32_used() const { return values_f32_used_; }
   double solve_ms() const { return solve_ms_; }
   hipStream_t stream() const { return stream_; }
   void destroy();
@@ -86,6 +90,7 @@ class BatchRunner {
   int gs_mode_ = 1;
   double eta_ = 0.1;
   bool autotune_ = false;
+  bool values_f32_used_ = false;
   int tuned_spmv_grid_ = 0;
   std::vector<int> ext2int_;   // caller's problem index -> storage order
   BatchPlan plan_;

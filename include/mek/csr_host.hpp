@@ -78,8 +78,10 @@ struct CSRHost {
   double norm_inf() const;            // max_i sum_j |a_ij|
   double norm_1() const;              // max_j sum_i |a_ij| (sums in storage order)
   // Values that FP32 storage would change: their count, and the largest
-  // relative change |fl32(a) - a| / |a| in *max_rel (0 when all are exact).
-  long long f32_inexact(double* max_rel) const;
+  // relative change |fl32(a) - a| / |a| among the finite ones in *max_rel (0
+  // when all are exact).  Values that are not finite, or whose FP32 image is
+  // not (beyond FP32's range), are counted and also reported in *nonfinite.
+  long long f32_inexact(double* max_rel, long long* nonfinite = nullptr) const;
 };
 
 // reference: validate_ghost_maps (main.cu:695-744).  Prints the mismatch and

@@ -81,7 +81,8 @@ SIGNATURES = {
     "mek_csr_load_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "mek_csr_info": (ctypes.c_int, [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_csr_export": (ctypes.c_int, [_vp, _ip, _ip, _dp]),
-    "mek_csr_f32_inexact": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _dp]),
+    "mek_csr_f32_inexact": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _dp,
+                                           ctypes.POINTER(ctypes.c_longlong)]),
     "mek_csr_destroy": (None, [_vp]),
     "mek_solver_options_default": (None, [ctypes.POINTER(SolverOptions)]),
     "mek_runner_create": (ctypes.c_int, [_vp, ctypes.POINTER(ArnoldiParams),
@@ -90,6 +91,7 @@ SIGNATURES = {
     "mek_runner_expmv": (ctypes.c_int, [_vp, _dp, _dp]),
     "mek_runner_expmv_device": (ctypes.c_int, [_vp, _vp, _vp]),
     "mek_runner_set_t": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "mek_runner_values_f32": (ctypes.c_int, [_vp, _ip]),
     "mek_runner_stats": (ctypes.c_int, [_vp, ctypes.POINTER(SolveStats)]),
     "mek_runner_steps": (ctypes.c_int, [_vp, _dp, _ip, ctypes.c_int, _ip]),
     "mek_runner_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, _dp,
@@ -101,6 +103,7 @@ SIGNATURES = {
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "mek_batch_tuning": (ctypes.c_int, [_vp, _ip]),
+    "mek_batch_values_f32": (ctypes.c_int, [_vp, _ip]),
     "mek_batch_time_spmv": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "mek_batch_upload_v": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
     "mek_batch_solve": (ctypes.c_int, [_vp]),
@@ -168,6 +171,17 @@ def _f64(a):
 
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _value_storage(values_f32):
+    """values_f32 argument -> the ABI's storage mode: False 0 (FP64), True 1
+    (FP32 only when every value is exact in FP32, else FP64), "lossy" 2 (FP32
+    always, the caller accepts the perturbation)."""
+    if values_f32 == "lossy":
+        return 2
+    if values_f32 in (False, True, 0, 1):
+        return 1 if values_f32 else 0
+    raise ValueError(f"values_f32 must be False, True or 'lossy', not {values_f32!r}")
 
 
 def _pd(a):
@@ -259,11 +273,13 @@ class CSR:
         _check(lib().mek_csr_info(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), ctypes.byref(d)))
         return {"rows": r.value, "cols": c.value, "nnz": z.value, "value_dtype": d.value}
 
-    def f32_inexact(self):
-        """(count, max relative change) of the values FP32 storage would change."""
-        c, m = ctypes.c_longlong(), ctypes.c_double()
-        _check(lib().mek_csr_f32_inexact(self.h, ctypes.byref(c), ctypes.byref(m)))
-        return c.value, m.value
+    def f32_inexact(self, nonfinite=False):
+        """(count, max relative change) of the values FP32 storage would change;
+        with nonfinite=True also the count of values that are not finite or
+        lie beyond FP32's range (included in count, not in the max)."""
+        c, m, b = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().mek_csr_f32_inexact(self.h, ctypes.byref(c), ctypes.byref(m), ctypes.byref(b)))
+        return (c.value, m.value, b.value) if nonfinite else (c.value, m.value)
 
     def arrays(self):
         i = self.info()
@@ -299,7 +315,7 @@ class Runner:
         o.step_control = STEP_CONTROL[step_control]
         o.gs_mode = int(gs_mode)
         o.dgks_eta = float(dgks_eta)
-        o.values_f32 = 1 if values_f32 else 0
+        o.values_f32 = _value_storage(values_f32)
         o.verbose = 1 if verbose else 0
         o.timing = 1 if timing else 0
         o.fuse_spmv_dot = 1 if fuse_spmv_dot else 0
@@ -322,6 +338,12 @@ class Runner:
 
     def set_t(self, t):
         _check(lib().mek_runner_set_t(self.h, float(t)))
+
+    def values_f32_used(self):
+        """True when the matrix values are stored in FP32 on the device."""
+        u = ctypes.c_int()
+        _check(lib().mek_runner_values_f32(self.h, ctypes.byref(u)))
+        return bool(u.value)
 
     def stats(self):
         s = SolveStats()
@@ -392,7 +414,7 @@ class Batch:
         h = _vp()
         ctrl = STEP_CONTROL[step_control]
         _check(lib().mek_batch_create(P, handles, _pd(t), _pi(m), _pd(tol), _pi(mr), _pi(ss), ctrl,
-                                      1 if values_f32 else 0, int(gs_mode), float(dgks_eta),
+                                      _value_storage(values_f32), int(gs_mode), float(dgks_eta),
                                       1 if use_graph else 0, 1 if autotune else 0, int(device), ctypes.byref(h)))
         self.h = h
         self.P = P
@@ -425,6 +447,12 @@ class Batch:
         c = ctypes.c_int()
         _check(lib().mek_batch_cycles(self.h, ctypes.byref(c)))
         return c.value
+
+    def values_f32_used(self):
+        """True when the batch stores its values in FP32 on the device."""
+        u = ctypes.c_int()
+        _check(lib().mek_batch_values_f32(self.h, ctypes.byref(u)))
+        return bool(u.value)
 
     def spmv_grid(self):
         """SpMV grid chosen by the autotuner (0 without autotune)."""

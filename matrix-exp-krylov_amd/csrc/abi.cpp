@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -50,6 +51,11 @@ namespace {
 
 thread_local std::string g_err;
 
+// A caller's argument failed a check at the boundary (MEK_E_INVALID).
+struct InvalidArgument : mek::Error {
+  using mek::Error::Error;
+};
+
 int set_err(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -63,6 +69,8 @@ int guarded(F&& f) {
     return MEK_OK;
   } catch (const mek::LoadError& e) {
     return set_err(MEK_E_LOAD, e.what());
+  } catch (const InvalidArgument& e) {
+    return set_err(MEK_E_INVALID, e.what());
   } catch (const mek::Error& e) {
     const std::string m = e.what();
     return set_err(m.rfind("HIP error", 0) == 0 ? MEK_E_HIP : MEK_E_SOLVER, m);
@@ -76,7 +84,7 @@ int guarded(F&& f) {
 }
 
 void need(bool ok, const char* what) {
-  if (!ok) throw mek::Error(std::string("invalid argument: ") + what);
+  if (!ok) throw InvalidArgument(std::string("invalid argument: ") + what);
 }
 
 }  // namespace
@@ -159,10 +167,10 @@ int mek_csr_info(const mek_csr* A, int* rows, int* cols, long long* nnz, int* va
   });
 }
 
-int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel) {
+int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel, long long* nonfinite) {
   return guarded([&] {
     need(A && count && max_rel, "matrix, outputs");
-    *count = A->A.f32_inexact(max_rel);
+    *count = A->A.f32_inexact(max_rel, nonfinite);
   });
 }
 
@@ -198,6 +206,7 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
   return guarded([&] {
     need(A && p && out, "runner arguments");
     need(device_id >= 0, "device_id");
+    need(std::isfinite(p->t) && p->t >= 0.0, "t finite and >= 0");
     ArnoldiParams params(p->m, p->t, p->tol, p->max_restarts, p->reorthogonalize != 0);
     auto h = std::make_unique<mek_runner>();
     h->R = std::make_unique<ArnoldiRunner>(1, params, A->A.rows, A->A.cols, A->A.nnz);
@@ -208,7 +217,8 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       so.control = static_cast<mek::StepControl>(o->step_control);
       so.gs_mode = o->gs_mode;
       so.dgks_eta = o->dgks_eta;
-      so.values_f32 = o->values_f32 != 0;
+      need(o->values_f32 >= 0 && o->values_f32 <= 2, "values_f32 in {0, 1, 2}");
+      so.values_f32 = o->values_f32;
       so.verbose = o->verbose != 0;
       so.timing = o->timing != 0;
       so.fuse_spmv_dot = o->fuse_spmv_dot != 0;
@@ -248,6 +258,7 @@ int mek_runner_expmv_device(mek_runner* R, const double* d_v, double* d_y) {
 int mek_runner_set_t(mek_runner* R, double t) {
   return guarded([&] {
     need(R, "runner");
+    need(std::isfinite(t) && t >= 0.0, "t finite and >= 0");
     ArnoldiRunner& r = *R->R;
     r.params.t = t;
     r.ts_params.dt_max = t;
@@ -330,6 +341,13 @@ int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sw
   });
 }
 
+int mek_runner_values_f32(const mek_runner* R, int* used) {
+  return guarded([&] {
+    need(R && used, "runner, out");
+    *used = R->R->values_f32_used ? 1 : 0;
+  });
+}
+
 int mek_runner_stream(const mek_runner* R, void** stream) {
   return guarded([&] {
     need(R && stream, "runner, stream");
@@ -354,15 +372,24 @@ int mek_batch_create(int nprob, const mek_csr* const* A, const double* t, const 
     need(gs_mode >= 0 && gs_mode <= 2, "gs_mode");
     need(step_control >= 0 && step_control <= 2, "step_control");
     std::vector<mek::BatchProblem> probs(nprob);
+    need(values_f32 >= 0 && values_f32 <= 2, "values_f32 in {0, 1, 2}");
     for (int p = 0; p < nprob; ++p) {
       need(A[p] != nullptr, "matrix");
+      need(std::isfinite(t[p]) && t[p] >= 0.0, "t finite and >= 0");
       probs[p] = {&A[p]->A, t[p], m[p], tol[p], max_restarts ? max_restarts[p] : 10,
                   substeps ? substeps[p] : 0, step_control};
     }
     auto h = std::make_unique<mek_batch>();
-    h->B = std::make_unique<mek::BatchRunner>(device_id, probs, values_f32 != 0, gs_mode, dgks_eta,
+    h->B = std::make_unique<mek::BatchRunner>(device_id, probs, values_f32, gs_mode, dgks_eta,
                                              use_graph != 0, autotune != 0);
     *out = h.release();
+  });
+}
+
+int mek_batch_values_f32(const mek_batch* B, int* used) {
+  return guarded([&] {
+    need(B && used, "batch, out");
+    *used = B->B->values_f32_used() ? 1 : 0;
   });
 }
 

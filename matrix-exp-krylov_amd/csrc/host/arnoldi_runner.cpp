@@ -124,7 +124,11 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
   dc.create_streams();
   dc.log = options.verbose;
   const int alloc_m = std::max(params.m, ts_params.m_max > 0 ? ts_params.m_max : params.m);
-  dc.alloc_from(partitions[0], global_cols, alloc_m, options.values_f32);
+  if (options.values_f32 < 0 || options.values_f32 > 2)
+    throw mek::Error("ERROR: values_f32 must be 0, 1 or 2");
+  values_f32_used = options.values_f32 == 2 ||
+                    (options.values_f32 == 1 && partitions[0].f32_inexact(nullptr) == 0);
+  dc.alloc_from(partitions[0], global_cols, alloc_m, values_f32_used);
   static const CSRHost::GhostMap kEmpty;
   dc.create_descriptors(partitions[0], global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0,
                         alloc_m);
@@ -513,6 +517,21 @@ void ArnoldiRunner::solve_on_device(const double* d_v, const double* h_v) {
   ev_used_ = 0;
   if (mode != ExpmvMode::AdaptiveTimeStepping) {
     std::cerr << "Expmv mode not implemented" << std::endl;
+    return;
+  }
+  // the reference's controller never runs for t <= 0 and returns whatever
+  // d_y held (main.cu:1431, :1521-1529); here t < 0 is refused and
+  // exp(0 A) v = v is returned exactly
+  if (!(params.t >= 0.0) || !std::isfinite(params.t))
+    throw mek::Error("ERROR: t must be finite and >= 0 (t=" + std::to_string(params.t) + ")");
+  if (params.t == 0.0) {
+    const size_t bytes = sizeof(double) * dc.local_rows;
+    if (d_v) MEK_HIP(hipMemcpyAsync(dc.d_y, d_v, bytes, hipMemcpyDeviceToDevice, dc.stream_compute));
+    else MEK_HIP(hipMemcpyAsync(dc.d_y, h_v, bytes, hipMemcpyHostToDevice, dc.stream_compute));
+    MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+    converged = true;
+    restarts_done = 0;
+    last_residual = 0.0;
     return;
   }
   init_q1_device(d_v, h_v);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -32,11 +33,15 @@ T* upload(const std::vector<T>& h) {
 
 }  // namespace
 
-BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_in, bool values_f32,
+BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_in, int values_f32,
                          int gs_mode, double dgks_eta, bool use_graph, bool autotune)
     : P_(static_cast<int>(probs_in.size())), device_(device_id), use_graph_(use_graph),
       gs_mode_(gs_mode), eta_(dgks_eta), autotune_(autotune) {
   if (P_ < 1) throw Error("ERROR: a batch needs at least one problem");
+  if (values_f32 < 0 || values_f32 > 2) throw Error("ERROR: values_f32 must be 0, 1 or 2");
+  for (int p = 0; p < P_; ++p)
+    if (!(probs_in[p].t >= 0.0) || !std::isfinite(probs_in[p].t))
+      throw Error("ERROR: batch problem " + std::to_string(p) + ": t must be finite and >= 0");
   // a failure part way releases what was already allocated (the destructor
   // does not run for an object whose constructor threw)
   try {
@@ -135,11 +140,20 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
       if (probs[p].control >= 1) any_expokit = 1;
     }
 
+    // FP32 values only where that stores A exactly, unless the caller asked
+    // for FP32 regardless (values_f32 == 2)
+    values_f32_used_ = values_f32 == 2;
+    if (values_f32 == 1) {
+      values_f32_used_ = true;
+      for (int p = 0; p < P_ && values_f32_used_; ++p)
+        if (probs[p].A->f32_inexact(nullptr) > 0) values_f32_used_ = false;
+    }
+
     // ---- device
     MEK_HIP(hipStreamCreate(&stream_));
     d_row_ptr_ = upload(row_ptr);
     d_col_idx_ = upload(col_idx);
-    if (values_f32) {
+    if (values_f32_used_) {
       std::vector<float> f(vals.begin(), vals.end());
       d_vals_ = upload(f);
     } else {
@@ -181,7 +195,7 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     b->row_ptr = d_row_ptr_;
     b->col_idx = d_col_idx_;
     b->vals = d_vals_;
-    b->vals_f32 = values_f32 ? 1 : 0;
+    b->vals_f32 = values_f32_used_ ? 1 : 0;
     b->rb_start = d_rb_start_;
     b->rb_prob = d_rb_prob_;
     b->nrb = nrb;
@@ -245,6 +259,9 @@ hipGraphExec_t BatchRunner::cycle_graph(int mcols) {
 void BatchRunner::reset_state() {
   const BatchBufs& b = *static_cast<BatchBufs*>(bufs_);
   MEK_HIP(hipMemcpyAsync(d_V_, d_v0_, sizeof(double) * n_tot_, hipMemcpyDeviceToDevice, stream_));
+  // y = v to start with: a problem with t = 0 never becomes active and keeps
+  // exp(0 A) v = v; every active problem overwrites its rows in the lift
+  MEK_HIP(hipMemcpyAsync(d_y_, d_v0_, sizeof(double) * n_tot_, hipMemcpyDeviceToDevice, stream_));
   MEK_HIP(hipMemcpyAsync(d_pint_, pint_init_.data(), sizeof(int) * pint_init_.size(), hipMemcpyHostToDevice, stream_));
   MEK_HIP(hipMemcpyAsync(d_pscal_, pscal_init_.data(), sizeof(double) * pscal_init_.size(),
                          hipMemcpyHostToDevice, stream_));

@@ -152,6 +152,11 @@ void CSRHost::load_matrix_market(const std::string& filename) {
     break;
   }
   if (!dims) fail("Matrix dimensions line not found in file " + filename);
+  // the reference sizes row_ptr from these without a check (main.cu:166);
+  // negative sizes are rejected here instead of reaching the CSR build
+  if (rows < 0 || cols < 0 || nnz < 0)
+    fail("Invalid matrix dimensions " + std::to_string(rows) + " " + std::to_string(cols) + " " +
+         std::to_string(nnz));
 
   Triplets t;
   const size_t reserve = static_cast<size_t>(std::max<long long>(nnz, 0)) * (symmetric ? 2 : 1);
@@ -429,18 +434,24 @@ double CSRHost::max_abs_value() const {
   return m;
 }
 
-long long CSRHost::f32_inexact(double* max_rel) const {
-  long long count = 0;
+long long CSRHost::f32_inexact(double* max_rel, long long* nonfinite) const {
+  long long count = 0, bad = 0;
   double worst = 0.0;
   for (long long k = 0; k < nnz; ++k) {
     const double a = values[k];
     const double r = static_cast<double>(static_cast<float>(a));
-    if (r != a) {
+    if (!std::isfinite(a) || !std::isfinite(r)) {
+      // NaN or infinite values, and finite values beyond FP32's range: counted
+      // apart so that max_rel stays a finite relative change
       ++count;
-      worst = std::max(worst, std::isfinite(r) ? std::abs(r - a) / std::abs(a) : HUGE_VAL);
+      ++bad;
+    } else if (r != a) {
+      ++count;
+      worst = std::max(worst, std::abs(r - a) / std::abs(a));
     }
   }
   if (max_rel) *max_rel = worst;
+  if (nonfinite) *nonfinite = bad;
   return count;
 }
 

@@ -17,6 +17,7 @@
 #include "batch.hpp"
 #include "expm_core.hpp"
 #include "sweeps.hpp"
+#include "../host/hip_check.hpp"
 
 namespace mek {
 
@@ -686,10 +687,11 @@ __global__ __launch_bounds__(1024) void kb_control(BatchBufs b) {
 // ------------------------------------------------------------ launchers
 
 void batch_prepare() {
-  // the in-LDS exponential of the largest problems needs 154 KB of LDS
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&kb_expm<false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(5 * kMaxKLds * kMaxKLds * sizeof(double)));
+  // the in-LDS exponential of the largest problems needs 154 KB of LDS (set
+  // on the current device before every solve)
+  MEK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&kb_expm<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(5 * kMaxKLds * kMaxKLds * sizeof(double))));
 }
 
 void batch_launch_init(const BatchBufs& b, hipStream_t s) {

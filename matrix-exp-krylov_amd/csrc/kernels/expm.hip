@@ -3,6 +3,7 @@
 // `A.exp()` (main.cu:33-36, used at main.cu:1966-1978 and :2007-2014).  The
 // algorithm lives in expm_core.hpp.
 #include "expm_core.hpp"
+#include "../host/hip_check.hpp"
 
 namespace mek {
 
@@ -65,14 +66,12 @@ size_t expm_lds_bytes(int kk) { return static_cast<size_t>(5) * kk * kk * sizeof
 
 void launch_expm(const ExpmArgs& a, int kk_max, hipStream_t s) {
   if (kk_max <= kMaxKLds) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      // allow the 144 KB dynamic LDS of the largest in-LDS problem
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_expm<false>),
+    // allow the dynamic LDS of the largest in-LDS problem on the current
+    // device; set before every launch (a cheap call) so that each device and
+    // each calling thread has it, and report a failure as a launch error
+    MEK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_expm<false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(expm_lds_bytes(kMaxKLds)));
-      attr_set = true;
-    }
+                                static_cast<int>(expm_lds_bytes(kMaxKLds))));
     hipLaunchKernelGGL(k_expm<false>, dim3(1), dim3(kBlock), expm_lds_bytes(kk_max), s, a);
   } else {
     hipLaunchKernelGGL(k_expm<true>, dim3(1), dim3(kBlock), 0, s, a);

@@ -20,6 +20,17 @@ def laplacian_2d_expmv(N: int, t: float, v: np.ndarray) -> np.ndarray:
     return scipy.fft.idstn(W, type=1, norm="ortho").ravel()
 
 
+def laplacian_2d_expmv_block(N: int, t: float, V: np.ndarray) -> np.ndarray:
+    """laplacian_2d_expmv of every column of V (n x k) at once, the DST-I over
+    the two grid axes of an (N, N, k) array on all cores."""
+    k = V.shape[1]
+    W = scipy.fft.dstn(np.asarray(V, dtype=np.float64).reshape(N, N, k), type=1, norm="ortho",
+                       axes=(0, 1), workers=-1)
+    lam = -2.0 + 2.0 * np.cos(np.pi * np.arange(1, N + 1) / (N + 1))
+    W *= np.exp(t * (lam[:, None] + lam[None, :]))[:, :, None]
+    return scipy.fft.idstn(W, type=1, norm="ortho", axes=(0, 1), workers=-1).reshape(N * N, k)
+
+
 def laplacian_2d_eigvec(N: int, ky: int, kx: int):
     """(eigenvector, eigenvalue) of the 2-D Laplacian for mode (ky, kx)."""
     j = np.arange(1, N + 1)

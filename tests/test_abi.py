@@ -37,7 +37,7 @@ def test_errors_cross_the_boundary_as_status_codes(tmp_path):
     assert "status -2" in str(e.value) and "Could not open binary CSR file" in str(e.value)
     with pytest.raises(mek.MekError) as e:
         mek.CSR.from_arrays([0, 1, 3], [0, 5, 1], [1.0, 2.0, 3.0])  # column 5 out of range
-    assert "status -4" in str(e.value) and "column index in range" in str(e.value)
+    assert "status -1" in str(e.value) and "column index in range" in str(e.value)
     assert mek.lib().mek_version().decode().startswith("mek")
 
 
@@ -88,3 +88,35 @@ def test_header_is_plain_c_and_links(tmp_path):
 
 def test_cli_binary_is_built():
     assert os.access(mek.CLI_PATH, os.X_OK)
+
+
+def _tiny():
+    from conftest import gen
+    return mek.CSR.from_arrays(*gen.laplacian_2d(3))
+
+
+@pytest.mark.parametrize("t", [-1.0, float("nan"), float("inf")])
+def test_negative_or_nonfinite_t_refused_before_any_device_call(t):
+    """t < 0 or not finite is refused by the ABI with a status code (the
+    reference's controller would never run and leave y unset); these checks
+    come before any HIP call, so they run without a GPU."""
+    with pytest.raises(mek.MekError) as e:
+        mek.Runner(_tiny(), m=3, t=t)
+    assert "status -1" in str(e.value) and "t finite and >= 0" in str(e.value)
+    with pytest.raises(mek.MekError) as e:
+        mek.Batch([_tiny()], [t], [3], [1e-8])
+    assert "t finite and >= 0" in str(e.value)
+
+
+def test_value_storage_mode_checked():
+    with pytest.raises(ValueError):
+        mek.Runner(_tiny(), m=3, t=1.0, values_f32="sometimes")
+    o = mek.SolverOptions()
+    mek.lib().mek_solver_options_default(o)
+    o.values_f32 = 3
+    p = mek.ArnoldiParams(3, 1.0, 1e-8, 10, 1)
+    h = mek._vp()
+    A = _tiny()
+    import ctypes
+    st = mek.lib().mek_runner_create(A.h, ctypes.byref(p), ctypes.byref(o), 0, ctypes.byref(h))
+    assert st == -1 and b"values_f32" in mek.lib().mek_last_error()

@@ -196,6 +196,9 @@ def test_mtx_symmetric_pattern_and_gzip(tmp_path):
     ("%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1 1.0\n", "Expected 3 entries but read 1"),
     ("%%MatrixMarket matrix coordinate real general\n2 2 1\n1 1\n", "Failed to read value for entry 0"),
     ("%%MatrixMarket matrix coordinate real general\n% only comments\n", "Matrix dimensions line not found"),
+    ("%%MatrixMarket matrix coordinate real general\n-1 2 0\n", "Invalid matrix dimensions -1 2 0"),
+    ("%%MatrixMarket matrix coordinate real general\n-5 -5 1\n1 1 1.0\n", "Invalid matrix dimensions -5 -5 1"),
+    ("%%MatrixMarket matrix coordinate real general\n2 2 -1\n", "Invalid matrix dimensions 2 2 -1"),
 ])
 def test_mtx_errors(tmp_path, text, msg):
     assert msg in _load_err(_mtx(tmp_path, text), "mtx")
@@ -222,3 +225,13 @@ def test_f32_inexact_count():
     assert n == 2  # 0.1, and 1e-40, which is subnormal in FP32
     r = abs(float(np.float32(0.1)) - 0.1) / 0.1
     assert worst >= r and worst <= 1.0
+
+
+def test_f32_inexact_nonfinite_apart():
+    """Values beyond FP32's range and non-finite values are counted apart;
+    the largest relative change stays finite (it is over the finite ones)."""
+    B = mek.CSR.from_arrays(np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32),
+                            np.array([1e300, 0.1, np.inf, 2.0]))
+    n, worst, bad = B.f32_inexact(nonfinite=True)
+    assert (n, bad) == (3, 2)
+    assert np.isfinite(worst) and worst == abs(float(np.float32(0.1)) - 0.1) / 0.1
