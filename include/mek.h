@@ -239,6 +239,14 @@ void mek_spmv_plan_destroy(mek_spmv_plan* P);
  * workgroup, FP64 Pade scaling and squaring).  Replaces the host Eigen
  * A.exp() of main.cu:33-36.  Synchronous. */
 int mek_expm_dense(int k, const double* d_A, double* d_E);
+/* The same exponential, allocating nothing: asynchronous on `stream`, with
+ * its working memory (scalars, the first column and, for k > 62, six k x k
+ * matrices) in a static device array of the library, 770 KiB per GPU, held
+ * until the process ends.  k in [1, 128]; larger k is refused.  Calls that
+ * overlap on one GPU share that array and race.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc). */
+int mek_expm_dense_noalloc(int k, const double* d_A, double* d_E, void* stream);
 
 /* ------------------------------------------------------- block of RHS */
 

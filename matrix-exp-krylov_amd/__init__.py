@@ -132,6 +132,7 @@ SIGNATURES = {
     "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mek_spmv_plan_destroy": (None, [_vp]),
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
+    "mek_expm_dense_noalloc": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp]),
     "mek_runner_tuning": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp, _dp]),
     "mek_block_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

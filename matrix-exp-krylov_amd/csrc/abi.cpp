@@ -609,6 +609,14 @@ int mek_expm_dense(int k, const double* d_A, double* d_E) {
   });
 }
 
+int mek_expm_dense_noalloc(int k, const double* d_A, double* d_E, void* stream) {
+  return guarded([&] {
+    need(d_A && d_E, "A, E");
+    need(k >= 1 && k <= mek::kExpmNoAllocMaxK, "k in [1, 128]");
+    mek::expm_dense_noalloc(k, d_A, d_E, static_cast<hipStream_t>(stream));
+  });
+}
+
 // ------------------------------------------------------- block of RHS
 
 int mek_block_create(const mek_csr* A, int m, double t, double tol, int substeps, int gs_mode,

@@ -2,7 +2,11 @@
 // squaring (Higham 2005), replacing the reference's host-side Eigen
 // `A.exp()` (main.cu:33-36, used at main.cu:1966-1978 and :2007-2014).  The
 // algorithm lives in expm_core.hpp.
+#include <mutex>
+#include <vector>
+
 #include "expm_core.hpp"
+#include "launch.hpp"
 #include "../host/hip_check.hpp"
 
 namespace mek {
@@ -76,6 +80,65 @@ void launch_expm(const ExpmArgs& a, int kk_max, hipStream_t s) {
   } else {
     hipLaunchKernelGGL(k_expm<true>, dim3(1), dim3(kBlock), 0, s, a);
   }
+}
+
+// ---------------------------------------------- allocation-free exponential
+//
+// Working memory of expm_dense_noalloc, one copy per device for the life of
+// the process: 16 scalar slots and 8 int slots (as 4 doubles), the first
+// column (k + 4 doubles), then at offset kScrWs the global workspace of six
+// k x k matrices that the kernel uses for k > kMaxKLds.
+constexpr int kScrWs = 256;
+constexpr size_t kScratchDoubles = kScrWs + 6 * static_cast<size_t>(kExpmNoAllocMaxK) * kExpmNoAllocMaxK;
+static_assert(24 + kExpmNoAllocMaxK + 4 <= kScrWs, "first column overlaps the workspace");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// The device address of this device's `scratch`, looked up once per device
+// with that device current (host code never passes the symbol itself).
+double* scratch_on_current_device() {
+  int dev = 0;
+  MEK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (static_cast<int>(g_scratch_ptr.size()) <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MEK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+void expm_dense_noalloc(int k, const double* d_A, double* d_E, hipStream_t s) {
+  if (k < 1 || k > kExpmNoAllocMaxK)
+    throw Error("ERROR: expm_dense_noalloc: k=" + std::to_string(k) + " outside [1, " +
+                std::to_string(kExpmNoAllocMaxK) + "] (the static scratch holds six " +
+                std::to_string(kExpmNoAllocMaxK) + "^2 matrices)");
+  double* base = scratch_on_current_device();
+  ExpmArgs a{};
+  a.H = d_A;
+  a.ldh = k;
+  // mode 2 with k_override reads no slot before writing it: nothing in the
+  // scratch needs clearing between calls
+  a.iscal = reinterpret_cast<int*>(base + 16);
+  a.iscal_out = reinterpret_cast<int*>(base + 16);
+  a.scal = base;
+  a.out = base + 24;
+  a.out_len = k;
+  a.dt = 1.0;
+  a.mode = 2;  // plain exponential: no residual
+  a.ws = base + kScrWs;
+  a.k_override = k;
+  a.breakdown_override = 0;
+  a.full_out = d_E;
+  launch_expm(a, k, s);
+  MEK_HIP(hipGetLastError());
 }
 
 }  // namespace mek

@@ -47,4 +47,11 @@ void launch_expm(const ExpmArgs& a, int kk_max, hipStream_t s);
 size_t expm_lds_bytes(int kk);
 constexpr int kExpmMaxKLds = 62;  // must equal kMaxKLds in expm_core.hpp
 
+// exp(A) of a dense k x k column-major matrix into d_E, asynchronous on s,
+// with every working buffer in the library's static device scratch (no
+// allocation); k in [1, kExpmNoAllocMaxK].  Calls that overlap on one device
+// share that scratch and race.
+constexpr int kExpmNoAllocMaxK = 128;
+void expm_dense_noalloc(int k, const double* d_A, double* d_E, hipStream_t s);
+
 }  // namespace mek
