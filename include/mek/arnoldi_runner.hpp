@@ -138,6 +138,9 @@ struct ArnoldiRunner {
   // reference: main.cu:1308-1409.  One GPU: device_ids.size() must be 1.
   void init_all_devices(const std::vector<CSRHost>& partitions, const std::vector<int>& device_ids,
                         const std::vector<CSRHost::GhostMap>& ghost_maps_in);
+  // Additive: the same for the whole matrix on one device, reading A in
+  // place (no partition copy).
+  void init_device(const CSRHost& A, int device_id);
   // reference: main.cu:1411-1571.  Host v in, host y out (y pre-sized).
   void compute_expmv(const std::vector<double>& v_host, std::vector<double>& y_host);
   void destroy();
@@ -149,6 +152,7 @@ struct ArnoldiRunner {
   int arnoldi_decomposition(const std::vector<double>& v_host, std::vector<double>& H_out);
 
  private:
+  void init_one(const CSRHost& part, int device_id);
   void solve_on_device(const double* d_v_or_null, const double* h_v_or_null);
   void run_columns(int m_cols);
   void run_lanczos_columns(int m_cols);

@@ -45,8 +45,13 @@ struct CSRHost {
   void load_binary_csr(const std::string& filename);
 
   // reference: partition_rows (main.cu:526-562).  Balanced row blocks with
-  // re-based row_ptr.
-  std::vector<CSRHost> partition_rows(int num_gpus) const;
+  // re-based row_ptr (copies of the arrays).
+  std::vector<CSRHost> partition_rows(int num_gpus) const&;
+  // Additive: the same on a matrix the caller gives up
+  // (std::move(A).partition_rows(p)).  One block takes the arrays over
+  // without a copy (A's arrays are left empty); for p > 1 the blocks are
+  // copies, as above.
+  std::vector<CSRHost> partition_rows(int num_gpus) &&;
 
   // reference: GhostMap (main.cu:567-585).  Kept for API compatibility; on the
   // single-GPU path every map is empty.

@@ -225,11 +225,8 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       so.lanczos = o->lanczos != 0;
       so.autotune = o->autotune != 0;
     }
-    std::vector<CSRHost> parts{A->A};
-    std::vector<int> ids{device_id};
-    std::vector<CSRHost::GhostMap> maps(1);
     try {
-      h->R->init_all_devices(parts, ids, maps);
+      h->R->init_device(A->A, device_id);  // reads the matrix in place
     } catch (...) {
       h->R->destroy();
       throw;

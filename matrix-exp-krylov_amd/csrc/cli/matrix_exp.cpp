@@ -11,7 +11,8 @@
 //   --v-file PATH        start vector, one value per line (default 0.1)
 //   --step-control reference|expokit|expokit-rel
 //   --gs always|never|dgks
-//   --f32-values         store matrix values in FP32 on the device
+//   --f32-values         store matrix values in FP32 on the device when every
+//                        value is exact in FP32 (else FP64)
 //   --lanczos            symmetric A: Lanczos three-term recurrence instead of Gram-Schmidt
 #include <hip/hip_runtime.h>
 
@@ -162,39 +163,42 @@ int main(int argc, char* argv[]) {
                 << ". Clamping t to " << t_limit << " to avoid overflow." << std::endl;
       t_safe = t_limit;
     }
-    auto partitions = A.partition_rows(num_gpus_to_use);
+    const int rows = A.rows, cols = A.cols;
+    const long long nnz = A.nnz;
+    // one GPU: the single row block takes A's arrays over (no copy)
+    auto partitions = std::move(A).partition_rows(num_gpus_to_use);
     std::cout << "Partitioned matrix across " << num_gpus_to_use << " GPUs" << std::endl;
     std::vector<CSRHost::GhostMap> ghost_maps(num_gpus_to_use);
     for (int i = 0; i < num_gpus_to_use; ++i)
-      ghost_maps[i] = A.build_owner_ghost_maps(partitions, i, num_gpus_to_use);
+      ghost_maps[i] = partitions[i].build_owner_ghost_maps(partitions, i, num_gpus_to_use);
     validate_ghost_maps(ghost_maps, num_gpus_to_use);
 
     int m_val = m_param;
-    if (m_val <= 0) m_val = std::min(30, std::max(5, A.rows - 1));
-    m_val = std::max(1, std::min(m_val, A.rows - 1));
+    if (m_val <= 0) m_val = std::min(30, std::max(5, rows - 1));
+    m_val = std::max(1, std::min(m_val, rows - 1));
     ArnoldiParams params(m_val, t_safe, tol_cli, max_restarts_cli, true);
     std::cout << "Arnoldi params: m=" << params.m << " t=" << params.t << " tol=" << params.tol
               << " max_restarts=" << params.max_restarts << std::endl;
 
-    ArnoldiRunner runner(num_gpus_to_use, params, A.rows, A.cols, A.nnz);
+    ArnoldiRunner runner(num_gpus_to_use, params, rows, cols, nnz);
     runner.options = opts;
     runner.init_all_devices(partitions, {0}, ghost_maps);
 
-    std::vector<double> v_host(A.rows, 0.1);
+    std::vector<double> v_host(rows, 0.1);
     if (!v_file.empty()) {
       std::ifstream vf(v_file);
       if (!vf) {
         std::cerr << "Error: could not open --v-file " << v_file << std::endl;
         return EXIT_FAILURE;
       }
-      for (int i = 0; i < A.rows; ++i) {
+      for (int i = 0; i < rows; ++i) {
         if (!(vf >> v_host[i])) {
-          std::cerr << "Error: --v-file " << v_file << " holds fewer than " << A.rows << " values" << std::endl;
+          std::cerr << "Error: --v-file " << v_file << " holds fewer than " << rows << " values" << std::endl;
           return EXIT_FAILURE;
         }
       }
     }
-    std::vector<double> y_host(A.rows, 0.0);
+    std::vector<double> y_host(rows, 0.0);
     runner.compute_expmv(v_host, y_host);
 
     if (const char* out_path = std::getenv("OUTPUT_Y_FILE"); out_path && out_path[0] != '\0') {

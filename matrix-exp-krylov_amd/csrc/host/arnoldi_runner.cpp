@@ -107,7 +107,20 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
   if (num_gpus != 1)
     throw mek::Error("ERROR: the solve runs on one GPU (num_gpus=" +
                      std::to_string(num_gpus) + ")");
-  if (options.lanczos && !csr_is_symmetric(partitions[0]))
+  init_one(partitions[0], device_ids[0]);
+}
+
+void ArnoldiRunner::init_device(const CSRHost& A, int device_id) {
+  if (num_gpus != 1)
+    throw mek::Error("ERROR: the solve runs on one GPU (num_gpus=" + std::to_string(num_gpus) + ")");
+  if (A.rows != global_rows || A.cols != global_cols || A.nnz != global_nnz)
+    throw mek::Error("ERROR: init_device: the matrix does not have the runner's rows, cols and nnz");
+  ghost_maps.assign(1, CSRHost::GhostMap{});
+  init_one(A, device_id);
+}
+
+void ArnoldiRunner::init_one(const CSRHost& part, int device_id) {
+  if (options.lanczos && !csr_is_symmetric(part))
     throw mek::Error("ERROR: the Lanczos path needs a symmetric matrix");
   if (params.m >= global_rows)
     throw mek::Error("ERROR: Requested Arnoldi m=" + std::to_string(params.m) +
@@ -119,7 +132,7 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
     throw mek::Error("ERROR: Arnoldi m=" + std::to_string(params.m) + " outside [1, " +
                      std::to_string(mek::kMaxKrylov) + "]");
   DeviceContext& dc = device_contexts[0];
-  dc.device_id = device_ids[0];
+  dc.device_id = device_id;
   dc.init_handles();
   dc.create_streams();
   dc.log = options.verbose;
@@ -127,13 +140,12 @@ void ArnoldiRunner::init_all_devices(const std::vector<CSRHost>& partitions,
   if (options.values_f32 < 0 || options.values_f32 > 2)
     throw mek::Error("ERROR: values_f32 must be 0, 1 or 2");
   values_f32_used = options.values_f32 == 2 ||
-                    (options.values_f32 == 1 && partitions[0].f32_inexact(nullptr) == 0);
-  dc.alloc_from(partitions[0], global_cols, alloc_m, values_f32_used);
+                    (options.values_f32 == 1 && part.f32_inexact(nullptr) == 0);
+  dc.alloc_from(part, global_cols, alloc_m, values_f32_used);
   static const CSRHost::GhostMap kEmpty;
-  dc.create_descriptors(partitions[0], global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0,
-                        alloc_m);
-  if (options.autotune) tuning = mek::autotune_single(dc, partitions[0], alloc_m);
-  anorm_inf = partitions[0].norm_inf();
+  dc.create_descriptors(part, global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0, alloc_m);
+  if (options.autotune) tuning = mek::autotune_single(dc, part, alloc_m);
+  anorm_inf = part.norm_inf();
 }
 
 // ------------------------------------------------------------ timing

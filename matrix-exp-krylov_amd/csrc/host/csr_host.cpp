@@ -321,7 +321,19 @@ void CSRHost::from_binary_csr(const std::string& filename) {
 
 // ----------------------------------------------------------- partitions
 
-std::vector<CSRHost> CSRHost::partition_rows(int num_gpus) const {
+std::vector<CSRHost> CSRHost::partition_rows(int num_gpus) && {
+  if (num_gpus != 1) return static_cast<const CSRHost&>(*this).partition_rows(num_gpus);
+  std::vector<CSRHost> parts(1);
+  parts[0] = std::move(*this);  // row_ptr already starts at 0: nothing to re-base
+  row_ptr.clear();
+  col_idx.clear();
+  values.clear();
+  nnz = 0;
+  return parts;
+}
+
+std::vector<CSRHost> CSRHost::partition_rows(int num_gpus) const& {
+  if (num_gpus < 1) throw std::invalid_argument("partition_rows: num_gpus must be >= 1");
   std::vector<CSRHost> parts(num_gpus);
   const int per = rows / num_gpus, rem = rows % num_gpus;
   int start = 0;
