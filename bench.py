@@ -4,7 +4,9 @@ A step is one full solve y = exp(t*A)*v (ArnoldiRunner::compute_expmv on
 device-resident v and y).  Default workload: 2-D 5-point Laplacian on a
 4096 x 4096 grid (n = 16,777,216, nnz = 83,869,696, FP64 values, i32
 indices), v ~ N(0,1) normalised (seed 0), t = 10, m = 40, tol 1e-8, sub-steps
-by the Expokit heuristic.
+by the Expokit heuristic.  FP64 Arnoldi with the basis grown by s-step
+blocks of 12 vectors after the first 6 columns of each cycle (--sstep 0 gives
+the reference's column-by-column Gram-Schmidt).
 
 Prints one JSON line with
 * value: solves/s of that workload, v and y resident in HBM, timed with no
@@ -549,7 +551,8 @@ def run_single(args, mek, gen):
     def runner(timing):
         return mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
                           gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=timing,
-                          fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune)
+                          fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune,
+                          sstep=args.sstep, sstep_first=args.sstep_first)
 
     R = runner(False)
     f32_values = None
@@ -649,7 +652,8 @@ def run_single(args, mek, gen):
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "t": w["t"], "m": w["m"],
                    "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
-                   "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "autotune": args.autotune, "seed": 0},
+                   "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "autotune": args.autotune,
+                   "sstep": args.sstep, "sstep_first": args.sstep_first, "seed": 0},
         "rel_err": rel,
         "values_f32": f32_values,
         "metric_span": {"solves_per_s": round(1.0 / statistics.median(span), 4),
@@ -658,7 +662,8 @@ def run_single(args, mek, gen):
                                 "D2H of y over PCIe included; median of the solves"},
         "solve": {"substeps": stats["accepted"], "expm_attempts": stats["attempts"],
                   "krylov_columns": stats["spmv_count"], "extra_spmv": stats["extra_spmv"],
-                  "converged": bool(stats["converged"])},
+                  "converged": bool(stats["converged"]), "sstep_blocks": stats["sstep_blocks"],
+                  "sstep_second": stats["sstep_second"], "sstep_fallbacks": stats["sstep_fallbacks"]},
         "roofline": roofline,
         "spmv": spmv_roof,
         "kernels_ms_per_solve": {k: round(kern[k]["ms"] / tsteps, 3) for k in kern if kern[k]["launches"]},
@@ -692,6 +697,10 @@ def main():
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="leave the batched comparison out of the default line")
     ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
+    ap.add_argument("--sstep", type=int, default=12,
+                    help="s-step blocks of this many basis vectors (0: column by column, as the reference)")
+    ap.add_argument("--sstep-first", type=int, default=6,
+                    help="columns of each cycle built one by one before the blocks (-1: --sstep)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the solver runs on one GPU; use --gpus 1 (each problem's rows would be split "

@@ -98,6 +98,10 @@ typedef struct {
     int fuse_spmv_dot;  /* first-pass dot products inside the SpMV kernel     */
     int lanczos;        /* symmetric A: Lanczos three-term recurrence        */
     int autotune;       /* time SpMV / sweep launch shapes at create, keep best */
+    int sstep;          /* s-step blocks of this many basis vectors (2..16;
+                           0: column by column); needs m + 1 <= 64        */
+    int sstep_first;    /* columns of each cycle built one by one before the
+                           blocks (-1: sstep)                               */
 } mek_solver_options;
 
 void mek_solver_options_default(mek_solver_options* opt);
@@ -115,6 +119,9 @@ typedef struct {
     int reorth_columns;
     int aborted;
     double solve_ms;
+    int sstep_blocks;     /* s-step blocks accepted in the last solve        */
+    int sstep_second;     /* of them, with a second update or Cholesky pass  */
+    int sstep_fallbacks;  /* cycles rebuilt column by column                 */
 } mek_solve_stats;
 
 typedef struct mek_runner mek_runner;

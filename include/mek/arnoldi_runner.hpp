@@ -76,6 +76,16 @@ struct SolverOptions {
   // Time candidate SpMV row-block caps and grids and sweep grids on this
   // matrix at init and keep the fastest (mek/autotune.hpp).
   bool autotune = false;
+  // s-step blocks (2..16; 0 off): after the first sstep_first columns of
+  // each Arnoldi cycle (-1: sstep columns) the basis grows sstep vectors at a
+  // time from a Newton recurrence with shifts at Chebyshev points of the
+  // Gershgorin interval, each block orthogonalised by one block Gram-Schmidt
+  // sweep pair (two when needed) and Cholesky QR, so the basis is read twice
+  // per block instead of twice per column.  Needs m + 1 <= 64; a block that
+  // cannot be completed falls back to the column-by-column path for the
+  // rest of that cycle.
+  int sstep = 0;
+  int sstep_first = -1;
 };
 
 struct StepRecord {
@@ -103,6 +113,9 @@ struct SolveStats {
   double solve_ms = 0.0;    // the METRIC total_compute_expmv_ms span
   double err_sum = 0.0;     // Expokit: summed local error estimates
   bool aborted = false;
+  int sstep_blocks = 0;     // s-step blocks accepted
+  int sstep_second = 0;     // of them, blocks that took a second update or Cholesky pass
+  int sstep_fallbacks = 0;  // cycles rebuilt column by column after a failed block
   std::vector<StepRecord> steps;
   KernelTiming kernels[kNumKinds];
 };
@@ -155,6 +168,8 @@ struct ArnoldiRunner {
   void init_one(const CSRHost& part, int device_id);
   void solve_on_device(const double* d_v_or_null, const double* h_v_or_null);
   void run_columns(int m_cols);
+  void run_columns_range(int j0, int j1);
+  void run_sstep_block(int k, int ss);
   void run_lanczos_columns(int m_cols);
   int read_status();
   void begin_timed(int kind);
@@ -167,6 +182,9 @@ struct ArnoldiRunner {
   void reference_controller();
   void expokit_controller();
 
+  bool ss_active_ = false;
+  std::vector<double> ss_shifts_;
+  int ss_blocks_before_ = 0, ss_second_before_ = 0;
   struct EventPair { hipEvent_t a, b; int kind; double bytes; };
   std::vector<EventPair> ev_pool_;
   size_t ev_used_ = 0;

@@ -17,6 +17,7 @@
 
 namespace mek {
 struct ArnoldiBufs;
+struct SsArgs;
 }
 
 struct DeviceContext {
@@ -43,6 +44,12 @@ struct DeviceContext {
   int grid = 0;                   // blocks of the streaming kernels
   int spmv_grid = 1;              // blocks of the SpMV kernel
 
+  // s-step blocks (allocated by alloc_sstep): state, flags, sweep partials
+  double* d_ss_state = nullptr;
+  int* d_ss_flags = nullptr;
+  double* d_ss_partials = nullptr;
+  int ss_grid = 0;
+
   int local_rows = 0;
   long long local_nnz = 0;
   int allocated_m_max = 0;
@@ -60,6 +67,9 @@ struct DeviceContext {
   void create_descriptors(const CSRHost& part, int global_cols, const CSRHost::GhostMap& ghost_map,
                           int global_row_offset, int m);
   void destroy();
+  // Additive: buffers of the s-step block path (sstep.hip), shifts uploaded.
+  void alloc_sstep(const std::vector<double>& shifts);
+  mek::SsArgs ss_args() const;
 
   // Additive: device buffer bundle handed to the kernels.
   mek::ArnoldiBufs bufs() const;

@@ -41,7 +41,8 @@ class SolverOptions(ctypes.Structure):
     _fields_ = [("step_control", ctypes.c_int), ("gs_mode", ctypes.c_int),
                 ("dgks_eta", ctypes.c_double), ("values_f32", ctypes.c_int),
                 ("verbose", ctypes.c_int), ("timing", ctypes.c_int),
-                ("fuse_spmv_dot", ctypes.c_int), ("lanczos", ctypes.c_int), ("autotune", ctypes.c_int)]
+                ("fuse_spmv_dot", ctypes.c_int), ("lanczos", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("sstep", ctypes.c_int), ("sstep_first", ctypes.c_int)]
 
 
 class ArnoldiParams(ctypes.Structure):
@@ -55,7 +56,9 @@ class SolveStats(ctypes.Structure):
                 ("attempts", ctypes.c_int), ("accepted", ctypes.c_int),
                 ("rejected", ctypes.c_int), ("spmv_count", ctypes.c_int),
                 ("extra_spmv", ctypes.c_int), ("reorth_columns", ctypes.c_int),
-                ("aborted", ctypes.c_int), ("solve_ms", ctypes.c_double)]
+                ("aborted", ctypes.c_int), ("solve_ms", ctypes.c_double),
+                ("sstep_blocks", ctypes.c_int), ("sstep_second", ctypes.c_int),
+                ("sstep_fallbacks", ctypes.c_int)]
 
 
 _lib = None
@@ -309,7 +312,8 @@ class Runner:
 
     def __init__(self, A: CSR, m=30, t=1.0, tol=1e-8, max_restarts=10, reorthogonalize=True,
                  step_control="reference", gs_mode=-1, dgks_eta=0.1, values_f32=False,
-                 verbose=False, timing=False, fuse_spmv_dot=False, lanczos=False, autotune=False, device=0):
+                 verbose=False, timing=False, fuse_spmv_dot=False, lanczos=False, autotune=False, device=0,
+                 sstep=0, sstep_first=-1):
         p = ArnoldiParams(int(m), float(t), float(tol), int(max_restarts), 1 if reorthogonalize else 0)
         o = SolverOptions()
         lib().mek_solver_options_default(ctypes.byref(o))
@@ -322,6 +326,8 @@ class Runner:
         o.fuse_spmv_dot = 1 if fuse_spmv_dot else 0
         o.lanczos = 1 if lanczos else 0
         o.autotune = 1 if autotune else 0
+        o.sstep = int(sstep)
+        o.sstep_first = int(sstep_first)
         h = _vp()
         _check(lib().mek_runner_create(A.h, ctypes.byref(p), ctypes.byref(o), int(device), ctypes.byref(h)))
         self.h = h

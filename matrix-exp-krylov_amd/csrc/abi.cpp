@@ -199,6 +199,8 @@ void mek_solver_options_default(mek_solver_options* o) {
   o->fuse_spmv_dot = d.fuse_spmv_dot;
   o->lanczos = d.lanczos;
   o->autotune = d.autotune;
+  o->sstep = d.sstep;
+  o->sstep_first = d.sstep_first;
 }
 
 int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_solver_options* o,
@@ -224,6 +226,10 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       so.fuse_spmv_dot = o->fuse_spmv_dot != 0;
       so.lanczos = o->lanczos != 0;
       so.autotune = o->autotune != 0;
+      need(o->sstep == 0 || (o->sstep >= 2 && o->sstep <= mek::kSsMaxS), "sstep 0 or in [2, 16]");
+      need(o->sstep_first >= -1, "sstep_first >= -1");
+      so.sstep = o->sstep;
+      so.sstep_first = o->sstep_first;
     }
     try {
       h->R->init_device(A->A, device_id);  // reads the matrix in place
@@ -281,6 +287,9 @@ int mek_runner_stats(const mek_runner* R, mek_solve_stats* o) {
     o->reorth_columns = r.stats.reorth_columns;
     o->aborted = r.stats.aborted ? 1 : 0;
     o->solve_ms = r.stats.solve_ms;
+    o->sstep_blocks = r.stats.sstep_blocks;
+    o->sstep_second = r.stats.sstep_second;
+    o->sstep_fallbacks = r.stats.sstep_fallbacks;
   });
 }
 

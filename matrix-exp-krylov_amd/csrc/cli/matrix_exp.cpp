@@ -14,6 +14,7 @@
 //   --f32-values         store matrix values in FP32 on the device when every
 //                        value is exact in FP32 (else FP64)
 //   --lanczos            symmetric A: Lanczos three-term recurrence instead of Gram-Schmidt
+//   --sstep S            s-step blocks of S basis vectors (2..16) after the first S columns
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -73,6 +74,7 @@ int main(int argc, char* argv[]) {
     if (a == "--v-file" && has_next) { v_file = args[++i]; continue; }
     if (a == "--f32-values") { opts.values_f32 = true; continue; }
     if (a == "--lanczos") { opts.lanczos = true; continue; }
+    if (a == "--sstep" && has_next) { opts.sstep = std::atoi(args[++i].c_str()); continue; }
     if (a == "--step-control" && has_next) {
       const std::string v = args[++i];
       opts.control = (v == "expokit")       ? mek::StepControl::Expokit

@@ -68,6 +68,45 @@ bool csr_is_symmetric(const CSRHost& A) {
   return true;
 }
 
+// s Chebyshev points of the Gershgorin interval of A (the real parts of its
+// eigenvalues lie there) in Leja order: the largest in magnitude first, then
+// each maximising the product of distances to those already taken, so every
+// prefix is a good shift set for a shorter block.
+std::vector<double> sstep_shifts(const CSRHost& A, int s) {
+  double lo = HUGE_VAL, hi = -HUGE_VAL;
+  for (int i = 0; i < A.rows; ++i) {
+    double d = 0.0, r = 0.0;
+    for (int k = A.row_ptr[i]; k < A.row_ptr[i + 1]; ++k) {
+      if (A.col_idx[k] == i) d += A.values[k];
+      else r += std::abs(A.values[k]);
+    }
+    lo = std::min(lo, d - r);
+    hi = std::max(hi, d + r);
+  }
+  if (!(lo <= hi)) lo = hi = 0.0;
+  std::vector<double> pts(s), out;
+  for (int i = 0; i < s; ++i) pts[i] = 0.5 * (lo + hi) + 0.5 * (hi - lo) * std::cos(M_PI * (2 * i + 1) / (2.0 * s));
+  std::vector<bool> used(s, false);
+  for (int n = 0; n < s; ++n) {
+    int best = -1;
+    double bv = -1.0;
+    for (int i = 0; i < s; ++i) {
+      if (used[i]) continue;
+      double v = 1.0;
+      if (out.empty()) v = std::abs(pts[i]);
+      else
+        for (double x : out) v *= std::abs(pts[i] - x);
+      if (v > bv) {
+        bv = v;
+        best = i;
+      }
+    }
+    used[best] = true;
+    out.push_back(pts[best]);
+  }
+  return out;
+}
+
 double round_step(double x) {
   const double s = std::pow(10.0, std::floor(std::log10(x)) - 1.0);
   return std::ceil(x / s) * s;
@@ -146,6 +185,16 @@ void ArnoldiRunner::init_one(const CSRHost& part, int device_id) {
   dc.create_descriptors(part, global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0, alloc_m);
   if (options.autotune) tuning = mek::autotune_single(dc, part, alloc_m);
   anorm_inf = part.norm_inf();
+  if (options.sstep < 0 || options.sstep == 1 || options.sstep > mek::kSsMaxS)
+    throw mek::Error("ERROR: sstep must be 0 or in [2, " + std::to_string(mek::kSsMaxS) + "]");
+  // the block path covers bases of up to 64 vectors; beyond that (or with
+  // the Lanczos recurrence) the columns are built one by one
+  ss_active_ = options.sstep >= 2 && !options.lanczos && !options.fuse_spmv_dot &&
+               params.m + 1 <= mek::kSsMaxCols;
+  if (ss_active_) {
+    ss_shifts_ = sstep_shifts(part, options.sstep);
+    dc.alloc_sstep(ss_shifts_);
+  }
 }
 
 // ------------------------------------------------------------ timing
@@ -215,6 +264,86 @@ void ArnoldiRunner::run_columns(int m_cols) {
     run_lanczos_columns(m_cols);
     return;
   }
+  const int s0 = options.sstep_first >= 0 ? std::min(options.sstep_first, m_cols) : std::min(options.sstep, m_cols);
+  if (!ss_active_ || s0 >= m_cols) {
+    run_columns_range(0, m_cols);
+    return;
+  }
+  run_columns_range(0, s0);
+  for (int k = s0 + 1; k <= m_cols;) {
+    const int ss = std::min(options.sstep, m_cols + 1 - k);
+    run_sstep_block(k, ss);
+    k += ss;
+  }
+  DeviceContext& dc = device_contexts[0];
+  read_status();
+  if (reinterpret_cast<const int*>(dc.h_status + 16)[mek::kSStepFail]) {
+    // a block could not be completed: the blocks before it stand (orthonormal
+    // columns, their H columns, k_used); clear the flag and build the
+    // remaining columns one by one from q_{k-1} of the failed block
+    int fk = 0;
+    MEK_HIP(hipMemcpy(&fk, dc.d_ss_flags + mek::kSsFailK, sizeof(int), hipMemcpyDeviceToHost));
+    const int j0 = (fk >= s0 + 1 && fk <= m_cols) ? fk - 1 : s0;
+    const int zero = 0;
+    MEK_HIP(hipMemcpyAsync(reinterpret_cast<int*>(dc.d_small + 16) + mek::kSStepFail, &zero, sizeof(int),
+                           hipMemcpyHostToDevice, dc.stream_compute));
+    MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+    stats.spmv_count -= m_cols - j0;
+    stats.sstep_fallbacks++;
+    run_columns_range(j0, m_cols);
+  }
+}
+
+// One s-step block: columns k .. k+ss-1 (launch sequence in launch.hpp).
+void ArnoldiRunner::run_sstep_block(int k, int ss) {
+  Range r("orthogonalize_sstep");
+  DeviceContext& dc = device_contexts[0];
+  const mek::ArnoldiBufs b = dc.bufs();
+  const mek::SsArgs a = dc.ss_args();
+  hipStream_t s = dc.stream_compute;
+  const double rows = static_cast<double>(a.rows);
+  const double bspmv = static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
+                       4.0 * (dc.local_rows + 1.0) + 16.0 * dc.local_rows;
+  const int gs_mode = (options.gs_mode >= 0) ? options.gs_mode : (params.reorthogonalize ? 1 : 0);
+  const int force2 = gs_mode == 1 ? 1 : 0;
+  for (int i = 0; i < ss; ++i) {
+    begin_timed(mek::kKSpmv);
+    mek::launch_ss_spmv(b, a, k, i, s);
+    end_timed(mek::kKSpmv, bspmv);
+    mek::launch_ss_sigma(b, a, k, i, s);
+  }
+  begin_timed(mek::kKDots);
+  mek::launch_ss_dots(a, k, ss, s);
+  end_timed(mek::kKDots, 8.0 * rows * (k + ss));
+  mek::launch_ss_reduce(a, k + ss, ss, -1, s);
+  mek::launch_ss_plan(a, mek::kSsPhaseA, 0, k, ss, force2, -1, s);
+  begin_timed(mek::kKUpdateDot);
+  mek::launch_ss_update(a, 0, k + ss, k, ss, -1, s);
+  end_timed(mek::kKUpdateDot, 8.0 * rows * (k + 2 * ss));
+  mek::launch_ss_reduce(a, k + ss, ss, -1, s);
+  mek::launch_ss_plan(a, mek::kSsPhaseB, 0, k, ss, force2, -1, s);
+  // second update, only when the side dots asked for it (no bytes counted)
+  begin_timed(mek::kKUpdateNorm);
+  mek::launch_ss_update(a, 0, k + ss, k, ss, mek::kSsPass2, s);
+  end_timed(mek::kKUpdateNorm, 0.0);
+  mek::launch_ss_reduce(a, k + ss, ss, mek::kSsPass2, s);
+  mek::launch_ss_plan(a, mek::kSsPhaseB, 1, k, ss, force2, mek::kSsPass2, s);
+  // normalising sweep over the block's own columns
+  begin_timed(mek::kKLift);
+  mek::launch_ss_update(a, k, ss, k, ss, mek::kSsNormReady, s);
+  end_timed(mek::kKLift, 8.0 * rows * 2 * ss);
+  mek::launch_ss_reduce(a, ss, ss, mek::kSsNormReady, s);
+  mek::launch_ss_plan(a, mek::kSsPhaseD, 0, k, ss, force2, mek::kSsNormReady, s);
+  begin_timed(mek::kKUpdateNorm);
+  mek::launch_ss_update(a, k, ss, k, ss, mek::kSsNorm2, s);
+  end_timed(mek::kKUpdateNorm, 0.0);
+  mek::launch_ss_reduce(a, ss, ss, mek::kSsNorm2, s);
+  mek::launch_ss_plan(a, mek::kSsPhaseD, 1, k, ss, force2, mek::kSsNorm2, s);
+  MEK_HIP_LAUNCHED();
+  stats.spmv_count += ss;
+}
+
+void ArnoldiRunner::run_columns_range(int j0, int j1) {
   Range r("orthogonalize_mgs");
   DeviceContext& dc = device_contexts[0];
   const mek::ArnoldiBufs b = dc.bufs();
@@ -226,7 +355,7 @@ void ArnoldiRunner::run_columns(int m_cols) {
   // quantity slot for the SpMV's ||w||^2 partial when the dots kernel
   // computes the norm itself (partials then share one block stride)
   const int spare_q = dc.allocated_m_max + 3;
-  for (int j = 0; j < m_cols; ++j) {
+  for (int j = j0; j < j1; ++j) {
     const int nq = j + 1;
     if (nq <= mek::kMaxFusedDots && options.fuse_spmv_dot) {
       begin_timed(mek::kKSpmv);
@@ -266,7 +395,7 @@ void ArnoldiRunner::run_columns(int m_cols) {
     mek::launch_fin_col(b, j, gs_mode, options.dgks_eta, s);
     MEK_HIP_LAUNCHED();
   }
-  stats.spmv_count += m_cols;  // launched; columns after a breakdown exit at once
+  stats.spmv_count += j1 - j0;  // launched; columns after a breakdown exit at once
 }
 
 // Symmetric Lanczos: per column one SpMV (with the fused alpha partial), its
@@ -553,6 +682,15 @@ void ArnoldiRunner::solve_on_device(const double* d_v, const double* h_v) {
   flush_timing();
   stats.reorth_columns = reinterpret_cast<const int*>(dc.h_status + 16)[mek::kReorthCount];
   v_norm = dc.h_status[mek::kVNorm];
+  if (ss_active_) {
+    // block counters are kept across solves: report this solve's share
+    int fl[mek::kSsNumFlags];
+    MEK_HIP(hipMemcpy(fl, dc.d_ss_flags, sizeof(fl), hipMemcpyDeviceToHost));
+    stats.sstep_blocks = fl[mek::kSsBlocks] - ss_blocks_before_;
+    stats.sstep_second = fl[mek::kSsSecond] - ss_second_before_;
+    ss_blocks_before_ = fl[mek::kSsBlocks];
+    ss_second_before_ = fl[mek::kSsSecond];
+  }
 }
 
 void ArnoldiRunner::compute_expmv(const std::vector<double>& v_host, std::vector<double>& y_host) {

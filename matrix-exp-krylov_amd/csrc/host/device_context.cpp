@@ -148,6 +148,38 @@ mek::ArnoldiBufs DeviceContext::bufs() const {
   return b;
 }
 
+void DeviceContext::alloc_sstep(const std::vector<double>& shifts) {
+  MEK_HIP(hipSetDevice(device_id));
+  if (shifts.size() > static_cast<size_t>(mek::kSsMaxS)) throw mek::Error("ERROR: more than 16 s-step shifts");
+  ss_grid = std::max(8, std::min(grid, 1024));
+  if (!d_ss_state) MEK_HIP(hipMalloc(&d_ss_state, sizeof(double) * mek::kSsDoubles));
+  if (!d_ss_flags) MEK_HIP(hipMalloc(&d_ss_flags, sizeof(int) * mek::kSsNumFlags));
+  if (!d_ss_partials) MEK_HIP(hipMalloc(&d_ss_partials, sizeof(double) * mek::sstep_partials_doubles(ss_grid)));
+  std::vector<double> st(mek::kSsDoubles, 0.0);
+  std::copy(shifts.begin(), shifts.end(), st.begin() + mek::kSsTheta);
+  MEK_HIP(hipMemcpy(d_ss_state, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));
+  MEK_HIP(hipMemset(d_ss_flags, 0, sizeof(int) * mek::kSsNumFlags));
+}
+
+mek::SsArgs DeviceContext::ss_args() const {
+  mek::SsArgs a{};
+  const mek::ArnoldiBufs b = bufs();
+  a.V = b.V;
+  a.ld = b.ld;
+  a.rows = (static_cast<long long>(local_rows) + 15) / 16 * 16;
+  a.colscale = b.colscale;
+  a.H = b.H;
+  a.ldh = b.ldh;
+  a.iscal = b.iscal;
+  a.spmv_partials = b.partials;
+  a.spmv_grid = b.spmv_grid;
+  a.st = d_ss_state;
+  a.fl = d_ss_flags;
+  a.partials = d_ss_partials;
+  a.grid = ss_grid;
+  return a;
+}
+
 void DeviceContext::destroy() {
   if (device_id >= 0) (void)hipSetDevice(device_id);
   auto fr = [](auto*& p) {
@@ -164,6 +196,9 @@ void DeviceContext::destroy() {
   fr(d_small);
   fr(d_partials);
   fr(d_expm_ws);
+  fr(d_ss_state);
+  fr(d_ss_flags);
+  fr(d_ss_partials);
   if (h_status) (void)hipHostFree(h_status);
   h_status = nullptr;
   if (stream_compute) (void)hipStreamDestroy(stream_compute);

@@ -137,12 +137,19 @@ __device__ __forceinline__ double sweep_combination(const double* __restrict__ V
 // dependent memory latencies (nonzeros, then x).  w is written with
 // non-temporal stores: its stream then interferes less with the reads
 // (measured 7 % on a 5-point stencil).
+//
+// With shift_ptr set, w = A x' - theta x' (x' = xs x, theta = *shift_ptr):
+// the s-step blocks' Newton recurrence; x[r] is read once per row, mostly
+// from the lines its gathers brought in.  check_breakdown 1 returns at once
+// after a breakdown, 2 also after a failed s-step block.
 template <typename VT, int MAXJ>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double* x_in,
                                                      const double* xs_ptr, double* wout,
                                                      int ndots, int normq, int check_breakdown,
-                                                     int i0) {
-  if (check_breakdown && b.iscal[kBreakdown]) return;
+                                                     int i0, const double* shift_ptr) {
+  if (check_breakdown && (b.iscal[kBreakdown] || (check_breakdown == 2 && b.iscal[kSStepFail]))) return;
+  const bool has_shift = shift_ptr != nullptr;
+  const double sh = has_shift ? *shift_ptr : 0.0;
   constexpr int U = kNnzCap / kBlock;
   __shared__ double prod[kNnzCap];
   __shared__ double red[kWavesPerBlock * (MAXJ + 1)];
@@ -214,6 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
         }
         double a = 0.0;
         for (int e = e0; e < e1; ++e) a += prod[e];
+        if (has_shift) a -= sh * (xs * x[r]);
         __builtin_nontemporal_store(a, w + r);
         acc[MAXJ] += a * a;
 #pragma unroll
@@ -232,6 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
       if (tid == 0) {
         double t = prod[0];
         for (int q = 1; q < kWavesPerBlock; ++q) t += prod[q];
+        if (has_shift) t -= sh * (xs * x[r0]);
         __builtin_nontemporal_store(t, w + r0);
         acc[MAXJ] += t * t;
 #pragma unroll
@@ -685,13 +694,22 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_fin(ArnoldiBufs b, int j, in
 
 template <int MAXJ>
 static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const double* xs, double* w,
-                              int ndots, int normq, int chk, hipStream_t s, int i0 = 0) {
+                              int ndots, int normq, int chk, hipStream_t s, int i0 = 0,
+                              const double* shift = nullptr) {
   if (b.vals_f32)
     hipLaunchKernelGGL((k_spmv_dot<float, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
-                       ndots, normq, chk, i0);
+                       ndots, normq, chk, i0, shift);
   else
     hipLaunchKernelGGL((k_spmv_dot<double, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
-                       ndots, normq, chk, i0);
+                       ndots, normq, chk, i0, shift);
+}
+
+// s-step block vector i (columns k + i from k - 1 + i): y = (A - theta_i) p
+// into V[:, k+i], ||y||^2 partial at quantity 0 (spmv_grid blocks).
+void launch_ss_spmv(const ArnoldiBufs& b, const SsArgs& a, int k, int i, hipStream_t s) {
+  const long long in = k - 1 + i;
+  launch_spmv_dot_t<1>(b, b.V + in * b.ld, b.colscale + in, b.V + (in + 1) * b.ld, 0, 0, 2, s, 0,
+                       a.st + kSsTheta + i);
 }
 
 // Arnoldi column j: w = A q_j with the ||w||^2 partial at quantity normq

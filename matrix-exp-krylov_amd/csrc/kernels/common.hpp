@@ -122,6 +122,7 @@ enum IntSlot : int {
   kReorth,        // 1 if the current column takes the second pass
   kReorthCount,   // columns that took the second pass (statistics)
   kExpmSquarings,
+  kSStepFail,     // 1 when an s-step block could not be completed
   kNumInts = 8
 };
 

@@ -43,6 +43,80 @@ void launch_lanczos_fin(const ArnoldiBufs& b, int j, hipStream_t s);
 void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, double* y,
                        hipStream_t s);
 
+// ------------------------------------------------------------ s-step blocks
+//
+// Block (s-step) Arnoldi: s basis vectors at a time from a Newton recurrence
+// p_i = (A - theta_i) p_{i-1} / sigma_i, orthogonalised against the basis by
+// one block Gram-Schmidt sweep pair (a second pair when the verification dots
+// ask for it) and among themselves by Cholesky QR; the Hessenberg columns
+// follow from the change of basis (sstep.hip).
+constexpr int kSsMaxS = 16;     // block size limit: one 16-wide MFMA tile
+constexpr int kSsMaxCols = 64;  // columns one block sweep covers: four tiles
+
+// State of the block in progress (doubles), then its flags (ints).
+enum SsSlot : int {
+  kSsTheta = 0,                       // shifts theta_i
+  kSsSigma = kSsTheta + kSsMaxS,      // sigma_i = ||(A - theta_i) p_{i-1}||
+  kSsDots = kSsSigma + kSsMaxS,       // reduced sweep dots [64][16]
+  kSsCtot = kSsDots + kSsMaxCols * kSsMaxS,  // P on Q_k [64][16]
+  kSsM = kSsCtot + kSsMaxCols * kSsMaxS,     // raw-column coefficients of the next update [64][16]
+  kSsR = kSsM + kSsMaxCols * kSsMaxS,        // R_tot [16][16]
+  kSsDoubles = kSsR + kSsMaxS * kSsMaxS
+};
+enum SsFlag : int {
+  kSsPass2 = 0,    // the second Gram-Schmidt update is due
+  kSsNormReady,    // Cholesky factor ready: the normalising sweep is due
+  kSsNorm2,        // the second Cholesky QR sweep is due
+  kSsPasses,       // Gram-Schmidt updates taken in this block
+  kSsBlocks,       // blocks accepted (statistics, kept across solves)
+  kSsSecond,       // blocks that took a second update or Cholesky pass (statistics)
+  kSsFailK,        // k of the block that failed (iscal[kSStepFail] set)
+  kSsNumFlags = 8
+};
+
+struct SsArgs {
+  double* V;             // the basis (ArnoldiBufs::V)
+  long long ld;
+  long long rows;        // rows swept: n rounded up to 16 (rows >= n are zero)
+  double* colscale;
+  double* H;
+  int ldh;
+  int* iscal;            // ArnoldiBufs::iscal (breakdown, kKUsed, kSStepFail)
+  const double* spmv_partials;  // ArnoldiBufs::partials (the SpMV's ||y||^2)
+  int spmv_grid;
+  double* st;            // SsSlot state
+  int* fl;               // SsFlag flags
+  double* partials;      // block sweep partials: quantity c*16 + j of block g at (c*16 + j)*grid + g
+  int grid;              // blocks of the block sweeps
+};
+
+// Launch sequence of one block, columns k .. k+s-1 from q_{k-1} (k >= 1,
+// k + s <= 64, s <= 16); every launch returns at once after a breakdown or a
+// failed block (iscal[kSStepFail]):
+//   for i < s: launch_ss_spmv(i), launch_ss_sigma(i)      p_i, sigma_i
+//   launch_ss_dots                                        Q^T P over [0, k+s)
+//   launch_ss_reduce, launch_ss_plan(kSsPhaseA)
+//   launch_ss_update(window [0, k+s), cond -1)            P' = P - Q_k C
+//   launch_ss_reduce, launch_ss_plan(kSsPhaseB, call 0)
+//   the same three with cond kSsPass2 and call 1         second update if due
+//   launch_ss_update(window [k, k+s), cond kSsNormReady)  Q_new = P' R^-1
+//   launch_ss_reduce, launch_ss_plan(kSsPhaseD, call 0)
+//   the same three with cond kSsNorm2 and call 1         second Cholesky QR if due
+// A block that cannot be completed sets iscal[kSStepFail]; the caller then
+// rebuilds the columns with the column-by-column path.
+enum SsPhase : int { kSsPhaseA = 0, kSsPhaseB, kSsPhaseD };
+void launch_ss_spmv(const ArnoldiBufs& b, const SsArgs& a, int k, int i, hipStream_t s);
+void launch_ss_sigma(const ArnoldiBufs& b, const SsArgs& a, int k, int i, hipStream_t s);
+void launch_ss_dots(const SsArgs& a, int k, int s, hipStream_t st);
+void launch_ss_update(const SsArgs& a, int c0, int ncol, int out0, int s, int cond, hipStream_t st);
+void launch_ss_reduce(const SsArgs& a, int ncol, int s, int cond, hipStream_t st);
+void launch_ss_plan(const SsArgs& a, int phase, int call, int k, int s, int force_pass2, int cond,
+                    hipStream_t st);
+// Doubles of SsArgs::partials for a sweep grid.
+inline size_t sstep_partials_doubles(int grid) {
+  return static_cast<size_t>(kSsMaxCols) * kSsMaxS * static_cast<size_t>(grid);
+}
+
 void launch_expm(const ExpmArgs& a, int kk_max, hipStream_t s);
 size_t expm_lds_bytes(int kk);
 constexpr int kExpmMaxKLds = 62;  // must equal kMaxKLds in expm_core.hpp
