@@ -159,6 +159,9 @@ int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used);
  * SpMV row-block nonzero cap and grid, sweep grid, and their timings (us). */
 int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sweep_grid, double* spmv_us,
                       double* sweep_us);
+/* Workgroups of the s-step block sweeps (0 when the block path is off) and,
+ * with autotune, the timed dots + update pair of the chosen grid (us). */
+int mek_runner_sstep_grid(const mek_runner* R, int* grid, double* us);
 /* The runner's stream (hipStream_t). */
 int mek_runner_stream(const mek_runner* R, void** stream);
 void mek_runner_destroy(mek_runner* R);

@@ -26,6 +26,8 @@ struct TuneChoice {
   int sweep_grid = 0;   // Gram-Schmidt sweep workgroups
   double spmv_us = 0.0;
   double sweep_us = 0.0;
+  int sstep_grid = 0;   // s-step block sweep workgroups (when s-step is on)
+  double sstep_us = 0.0;
   bool cached = false;
   std::string log;      // every candidate and its time
 };

@@ -137,6 +137,7 @@ SIGNATURES = {
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
     "mek_expm_dense_noalloc": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp]),
     "mek_runner_tuning": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp, _dp]),
+    "mek_runner_sstep_grid": (ctypes.c_int, [_vp, _ip, _dp]),
     "mek_block_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(_vp)]),
@@ -380,8 +381,10 @@ class Runner:
         a, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().mek_runner_tuning(self.h, ctypes.byref(c), ctypes.byref(g), ctypes.byref(w),
                                        ctypes.byref(a), ctypes.byref(b)))
+        sg, su = ctypes.c_int(), ctypes.c_double()
+        _check(lib().mek_runner_sstep_grid(self.h, ctypes.byref(sg), ctypes.byref(su)))
         return {"nnz_cap": c.value, "spmv_grid": g.value, "sweep_grid": w.value,
-                "spmv_us": a.value, "sweep_us": b.value}
+                "spmv_us": a.value, "sweep_us": b.value, "sstep_grid": sg.value, "sstep_us": su.value}
 
     def arnoldi(self, v):
         v = _f64(v)

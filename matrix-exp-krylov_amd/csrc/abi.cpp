@@ -334,6 +334,14 @@ int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used) {
   });
 }
 
+int mek_runner_sstep_grid(const mek_runner* R, int* grid, double* us) {
+  return guarded([&] {
+    need(R && grid && us, "runner, outputs");
+    *grid = R->R->device_contexts[0].ss_grid;
+    *us = R->R->tuning.sstep_us;
+  });
+}
+
 int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sweep_grid, double* spmv_us,
                       double* sweep_us) {
   return guarded([&] {

@@ -194,6 +194,33 @@ void ArnoldiRunner::init_one(const CSRHost& part, int device_id) {
   if (ss_active_) {
     ss_shifts_ = sstep_shifts(part, options.sstep);
     dc.alloc_sstep(ss_shifts_);
+    if (options.autotune) {
+      // block sweep grid: the dots and the update of the largest block of a
+      // cycle, timed on the basis buffer (its contents are scratch here)
+      const int ss = std::min(options.sstep, params.m);
+      const int k = params.m + 1 - ss;
+      std::vector<int> grids;
+      const int ntiles = static_cast<int>((dc.local_rows + 127) / 128);
+      for (int g : {256, 512, 1024, mek::kSsMaxGrid})
+        if (g <= std::max(8, ntiles)) grids.push_back(g);
+      if (grids.empty()) grids.push_back(dc.ss_grid);
+      mek::SsArgs a = dc.ss_args();
+      std::vector<double> us;
+      const int i = mek::time_candidates(
+          grids,
+          [&](int g) {
+            a.grid = g;
+            mek::launch_ss_dots(a, k, ss, dc.stream_compute);
+            mek::launch_ss_update(a, 0, k + ss, k, ss, -1, dc.stream_compute);
+          },
+          dc.stream_compute, 5, &us);
+      dc.ss_grid = grids[i];
+      tuning.sstep_grid = grids[i];
+      tuning.sstep_us = us[i];
+      MEK_HIP(hipMemsetAsync(dc.d_V_m, 0, sizeof(double) * static_cast<size_t>(dc.ld) * (dc.allocated_m_max + 2),
+                             dc.stream_compute));
+      MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+    }
   }
 }
 

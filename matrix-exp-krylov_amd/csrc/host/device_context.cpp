@@ -154,7 +154,9 @@ void DeviceContext::alloc_sstep(const std::vector<double>& shifts) {
   ss_grid = std::max(8, std::min(grid, 1024));
   if (!d_ss_state) MEK_HIP(hipMalloc(&d_ss_state, sizeof(double) * mek::kSsDoubles));
   if (!d_ss_flags) MEK_HIP(hipMalloc(&d_ss_flags, sizeof(int) * mek::kSsNumFlags));
-  if (!d_ss_partials) MEK_HIP(hipMalloc(&d_ss_partials, sizeof(double) * mek::sstep_partials_doubles(ss_grid)));
+  // sized for the largest grid the launch-shape tuning may pick
+  if (!d_ss_partials)
+    MEK_HIP(hipMalloc(&d_ss_partials, sizeof(double) * mek::sstep_partials_doubles(mek::kSsMaxGrid)));
   std::vector<double> st(mek::kSsDoubles, 0.0);
   std::copy(shifts.begin(), shifts.end(), st.begin() + mek::kSsTheta);
   MEK_HIP(hipMemcpy(d_ss_state, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));

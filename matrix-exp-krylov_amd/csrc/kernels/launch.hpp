@@ -52,6 +52,7 @@ void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, 
 // follow from the change of basis (sstep.hip).
 constexpr int kSsMaxS = 16;     // block size limit: one 16-wide MFMA tile
 constexpr int kSsMaxCols = 64;  // columns one block sweep covers: four tiles
+constexpr int kSsMaxGrid = 2048;  // workgroups of a block sweep
 
 // State of the block in progress (doubles), then its flags (ints).
 enum SsSlot : int {

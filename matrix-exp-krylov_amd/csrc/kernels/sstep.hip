@@ -296,20 +296,33 @@ __device__ bool chol_inv(const double (*G)[kSsMaxS], int s, double (*R)[kSsMaxS]
 // either the second update (call 0 only) or the Cholesky factor and the
 // coefficients of the normalising sweep.  Phase D: from the normalising
 // sweep's Gram matrix, either a second Cholesky QR sweep (call 0 only) or the
-// Hessenberg columns.  Any failure sets iscal[kSStepFail].
+// Hessenberg columns.  Any failure sets iscal[kSStepFail] (and records k).
+// The operands are staged in LDS first; every loop over the block's entries
+// is spread over the workgroup.
 __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int call, int k, int s,
                                                     int force2, int cond) {
   if (ss_skip(a, cond)) return;
+  __shared__ double sD[kSsMaxCols][kSsMaxS];  // reduced dots
+  __shared__ double sC[kSsMaxCols][kSsMaxS];  // C_tot
+  __shared__ double sX[kSsMaxCols][kSsMaxS];  // T B - H_old U
+  __shared__ double sH[kSsMaxCols][kSsMaxCols - 1];  // H_old rows < k, columns < k - 1
+  __shared__ double sCs[kSsMaxCols];          // column scales of the window
   __shared__ double G[kSsMaxS][kSsMaxS];
   __shared__ double R[kSsMaxS][kSsMaxS];
   __shared__ double Ri[kSsMaxS][kSsMaxS];
+  __shared__ int need_col[kSsMaxS];
   __shared__ int state;  // 0 go on, 1 second pass, 2 failed
   const int tid = threadIdx.x;
-  const double* D = a.st + kSsDots;
   double* Ct = a.st + kSsCtot;
   double* M = a.st + kSsM;
   double* Rt = a.st + kSsR;
-  const double* cs = a.colscale;
+  const int nd = (phase == kSsPhaseD) ? s : k + s;  // rows of the reduced dots
+  for (int e = tid; e < kSsMaxCols * kSsMaxS; e += kBlock) {
+    const int c = e / kSsMaxS, j = e % kSsMaxS;
+    sD[c][j] = (c < nd && j < s) ? a.st[kSsDots + e] : 0.0;
+  }
+  for (int c = tid; c < kSsMaxCols; c += kBlock) sCs[c] = (c < k + s) ? a.colscale[c] : 0.0;
+  __syncthreads();
   if (phase == kSsPhaseA) {
     // true C1 = Q_k^T P; update coefficients over the window [0, k+s)
     for (int e = tid; e < kSsMaxCols * kSsMaxS; e += kBlock) {
@@ -317,15 +330,15 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
       double mv = 0.0;
       if (j < s && c < k + s) {
         if (c < k) {
-          const double c1 = D[c * kSsMaxS + j] * cs[c] * cs[k + j];
+          const double c1 = sD[c][j] * sCs[c] * sCs[k + j];
           Ct[e] = c1;
-          mv = -c1 * cs[c];
+          mv = -c1 * sCs[c];
           if (!isfinite(c1)) {
             a.iscal[kSStepFail] = 1;
             a.fl[kSsFailK] = k;
           }
         } else {
-          mv = (c - k == j) ? cs[k + j] : 0.0;
+          mv = (c - k == j) ? sCs[k + j] : 0.0;
         }
       }
       M[e] = mv;
@@ -336,20 +349,23 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
     // side dots of P' (stored true, column scale 1 from here on)
     if (tid < kSsMaxS * kSsMaxS) {
       const int l = tid / kSsMaxS, j = tid % kSsMaxS;
-      G[l][j] = (l < s && j < s) ? D[(k + l) * kSsMaxS + j] : 0.0;
+      G[l][j] = (l < s && j < s) ? sD[k + l][j] : 0.0;
+    }
+    if (tid < s) {
+      // second update due when ||Q_k^T p'_j|| > 1e-12 ||p'_j|| (also on NaN)
+      double n2 = 0.0;
+      for (int c = 0; c < k; ++c) {
+        const double x = sD[c][tid] * sCs[c];
+        n2 += x * x;
+      }
+      const double g2 = sD[k + tid][tid];
+      need_col[tid] = !(n2 <= 1e-24 * g2);
+      a.colscale[k + tid] = 1.0;
     }
     __syncthreads();
     if (tid == 0) {
-      for (int j = 0; j < s; ++j) a.colscale[k + j] = 1.0;
       bool need = force2 && call == 0;
-      for (int j = 0; j < s && !need; ++j) {
-        double n2 = 0.0;
-        for (int c = 0; c < k; ++c) {
-          const double x = D[c * kSsMaxS + j] * cs[c];
-          n2 += x * x;
-        }
-        if (!(n2 <= 1e-24 * G[j][j])) need = true;  // also on NaN
-      }
+      for (int j = 0; j < s; ++j) need = need || need_col[j];
       if (need) {
         state = (call == 0) ? 1 : 2;
       } else {
@@ -381,9 +397,9 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
         double mv = 0.0;
         if (j < s && c < k + s) {
           if (c < k) {
-            const double c2 = D[c * kSsMaxS + j] * cs[c];
+            const double c2 = sD[c][j] * sCs[c];
             Ct[e] += c2;
-            mv = -c2 * cs[c];
+            mv = -c2 * sCs[c];
           } else {
             mv = (c - k == j) ? 1.0 : 0.0;
           }
@@ -414,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
   // phase D: Gram matrix of Q_new
   if (tid < kSsMaxS * kSsMaxS) {
     const int l = tid / kSsMaxS, j = tid % kSsMaxS;
-    G[l][j] = (l < s && j < s) ? D[l * kSsMaxS + j] : 0.0;
+    G[l][j] = (l < s && j < s) ? sD[l][j] : 0.0;
   }
   __syncthreads();
   if (tid == 0) {
@@ -459,21 +475,35 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
     }
     return;
   }
-  // accepted: Hessenberg columns k-1 .. k+s-2 = (T B - H_old U) W^-1
+  // accepted: Hessenberg columns k-1 .. k+s-2 = (T B - H_old U) W^-1, with
+  // [p_0 .. p_s] = [Q_k Q_new] T
   const double* th = a.st + kSsTheta;
   const double* sg = a.st + kSsSigma;
   const int ldh = a.ldh;
-  auto T = [&](int r, int c) -> double {  // [p_0 .. p_s] = [Q_k Q_new] T
+  for (int e = tid; e < kSsMaxCols * kSsMaxS; e += kBlock) sC[e / kSsMaxS][e % kSsMaxS] = Ct[e];
+  if (tid < kSsMaxS * kSsMaxS) R[tid / kSsMaxS][tid % kSsMaxS] = Rt[tid];
+  for (int e = tid; e < k * (k - 1); e += kBlock) {
+    const int r = e % k, q = e / k;
+    sH[r][q] = a.H[static_cast<long long>(q) * ldh + r];
+  }
+  __syncthreads();
+  auto T = [&](int r, int c) -> double {
     if (c == 0) return r == k - 1 ? 1.0 : 0.0;
-    if (r < k) return Ct[r * kSsMaxS + c - 1];
-    return (r - k <= c - 1) ? Rt[(r - k) * kSsMaxS + c - 1] : 0.0;
+    if (r < k) return sC[r][c - 1];
+    return (r - k <= c - 1) ? R[r - k][c - 1] : 0.0;
   };
+  for (int e = tid; e < (k + s) * s; e += kBlock) {
+    const int r = e / s, c = e % s;
+    double x = T(r, c) * th[c] + T(r, c + 1) * sg[c];
+    if (r < k && c >= 1)
+      for (int q = 0; q < k - 1; ++q) x -= sH[r][q] * sC[q][c - 1];
+    sX[r][c] = x;
+  }
+  __syncthreads();
   for (int r = tid; r < k + s; r += kBlock) {
     double hn[kSsMaxS];
     for (int c = 0; c < s; ++c) {
-      double x = T(r, c) * th[c] + T(r, c + 1) * sg[c];
-      if (r < k && c >= 1)
-        for (int q = 0; q < k - 1; ++q) x -= a.H[static_cast<long long>(q) * ldh + r] * Ct[q * kSsMaxS + c - 1];
+      double x = sX[r][c];
       for (int l = 0; l < c; ++l) x -= hn[l] * T(k - 1 + l, c);
       hn[c] = x / T(k - 1 + c, c);
     }

@@ -51,6 +51,9 @@ KINDS = [
     ("kb_expm", "expm"),
     ("kb_fin", "finalise"),
     ("kb_", "batch_other"),
+    ("k_ss_dots", "sstep_dots"),
+    ("k_ss_update", "sstep_update"),
+    ("k_ss_", "finalise"),
     ("k_spmv_dot", "spmv"),
     ("k_update_dot", "update_dot"),
     ("k_update_norm", "update_norm"),
@@ -151,8 +154,10 @@ def workload(args):
     v = gen.normal_vector(rp.size - 1, seed=0)
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
     A = mek.CSR.from_arrays(rp, ci, va)
+    # the bench's runner: autotuned launch shapes, s-step blocks as given
     R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control="expokit",
-                   gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=False)
+                   gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=False, autotune=True,
+                   sstep=args.sstep, sstep_first=args.sstep_first)
     A.close()
     d_v = mek.DeviceBuffer.from_array(v)
     d_y = mek.DeviceBuffer(8 * v.size)
@@ -318,6 +323,8 @@ def main():
     a = sub.add_parser("workload")
     a.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS) + ["batch256", "block2048"])
     a.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"])
+    a.add_argument("--sstep", type=int, default=12, help="s-step block size (0: column by column)")
+    a.add_argument("--sstep-first", type=int, default=6)
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch", default=None)
