@@ -31,7 +31,7 @@
 
 namespace mek {
 
-constexpr int kNnzCap = 2048;  // nnz of one short-row SpMV block (LDS products)
+constexpr int kNnzCap = 4096;  // nnz of one short-row SpMV block (LDS products)
 
 template <typename VT>
 __device__ __forceinline__ double load_val(const void* vals, long long k) {
@@ -201,9 +201,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
       }
       const int rr = r0 + tid;
       int e0 = 0, e1 = 0;
+      // the shift's x[rr] is loaded with the block's nonzeros, so the row
+      // phase waits on no further memory round trip
+      double xrr = 0.0;
       if (rr < r1) {
         e0 = rp[rr] - k0;
         e1 = rp[rr + 1] - k0;
+        if (has_shift) xrr = x[rr];
       }
       double pr[U];
 #pragma unroll
@@ -218,10 +222,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
         if (r != rr) {
           e0 = rp[r] - k0;
           e1 = rp[r + 1] - k0;
+          if (has_shift) xrr = x[r];
         }
         double a = 0.0;
         for (int e = e0; e < e1; ++e) a += prod[e];
-        if (has_shift) a -= sh * (xs * x[r]);
+        if (has_shift) a -= sh * (xs * xrr);
         __builtin_nontemporal_store(a, w + r);
         acc[MAXJ] += a * a;
 #pragma unroll

@@ -8,7 +8,7 @@
 namespace mek {
 
 constexpr int kMaxFusedDots = 64;  // dots carried in registers by one sweep
-constexpr int kSpmvNnzCap = 2048;  // must equal kNnzCap in arnoldi.hip
+constexpr int kSpmvNnzCap = 4096;  // must equal kNnzCap in arnoldi.hip
 constexpr int kSpmvRowCap = 2048;
 constexpr int kMaxKrylov = 500;    // largest supported m
 constexpr int kMaxGrid = 4096;     // blocks of the streaming and SpMV kernels
