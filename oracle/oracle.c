@@ -377,7 +377,7 @@ int oracle_expmv(int n, const int* row_ptr, const int* col_idx,
 }
 
 int oracle_arnoldi(int n, const int* row_ptr, const int* col_idx, const double* vals,
-                   const double* v, int m, int reorthogonalize, double* H) {
+                   const double* v, int m, int reorthogonalize, double* H, double* V) {
     arnoldi_state st;
     st.n = n; st.row_ptr = row_ptr; st.col_idx = col_idx; st.vals = vals; st.m = m;
     st.V = (double*)calloc((size_t)n * (size_t)(m + 1), sizeof(double));
@@ -394,6 +394,7 @@ int oracle_arnoldi(int n, const int* row_ptr, const int* col_idx, const double* 
         k_used = j + 1;
         if (!ok) break;
     }
+    if (V) memcpy(V, st.V, (size_t)n * (size_t)(m + 1) * sizeof(double));
     free(st.V); free(st.w);
     return k_used;
 }

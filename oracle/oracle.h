@@ -84,9 +84,11 @@ int oracle_expmv(int n, const int* row_ptr, const int* col_idx,
 
 /* One Arnoldi decomposition from v (init_q1, then m columns of SpMV, MGS
  * (+ second pass if reorthogonalize), normalise; main_cpu.cpp:473-546).
- * H receives (m+1) x m column-major.  Returns the columns built (k_used). */
+ * H receives (m+1) x m column-major; V, when not NULL, the basis as n x (m+1)
+ * column-major (columns past k_used stay zero).  Returns the columns built
+ * (k_used). */
 int oracle_arnoldi(int n, const int* row_ptr, const int* col_idx, const double* vals,
-                   const double* v, int m, int reorthogonalize, double* H);
+                   const double* v, int m, int reorthogonalize, double* H, double* V);
 
 /* Bounded timing sample of the reference's Arnoldi column loop: init_q1 then
  * `ncols` columns of (SpMV, two MGS passes, normalise) from v.  col_ms[j]

@@ -64,7 +64,7 @@ def lib():
                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    dp, ctypes.POINTER(Stats), ctypes.POINTER(Step), ctypes.c_int]
         L.oracle_expmv.restype = ctypes.c_int
-        L.oracle_arnoldi.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.c_int, ctypes.c_int, dp]
+        L.oracle_arnoldi.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.oracle_arnoldi.restype = ctypes.c_int
         L.oracle_arnoldi_columns.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.c_int, dp]
         L.oracle_arnoldi_columns.restype = ctypes.c_int
@@ -128,15 +128,21 @@ def expmv(row_ptr, col_idx, vals, v, t, tol=1e-6, m=30, max_restarts=10,
     return y, stats, log
 
 
-def arnoldi(row_ptr, col_idx, vals, v, m, reorthogonalize=True):
-    """(H as an (m+1) x m array, k_used) of one MGS(2) Arnoldi decomposition."""
+def arnoldi(row_ptr, col_idx, vals, v, m, reorthogonalize=True, basis=False):
+    """(H as an (m+1) x m array, k_used) of one MGS(2) Arnoldi decomposition;
+    with basis=True, (H, V, k_used) with V the n x (m+1) basis."""
     rp, ci, va = _csr(row_ptr, col_idx, vals)
     n = rp.size - 1
     v = np.ascontiguousarray(v, dtype=np.float64)
     H = np.zeros((m + 1) * m, dtype=np.float64)
+    V = np.zeros((m + 1) * n, dtype=np.float64) if basis else None
     k = lib().oracle_arnoldi(n, _ip(rp), _ip(ci), _dp(va), _dp(v), int(m),
-                             1 if reorthogonalize else 0, _dp(H))
-    return H.reshape(m, m + 1).T.copy(), k
+                             1 if reorthogonalize else 0, _dp(H),
+                             _dp(V) if basis else None)
+    H = H.reshape(m, m + 1).T.copy()
+    if basis:
+        return H, V.reshape(m + 1, n).T.copy(), k
+    return H, k
 
 
 def arnoldi_columns(row_ptr, col_idx, vals, v, ncols):

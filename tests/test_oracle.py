@@ -255,3 +255,68 @@ def test_controller_trace_independent_restatement(oracle, seed, m, tol, t, R):
     for f in ("converged", "restarts_done", "attempts", "aborted"):
         assert st[f] == sref[f], f
     assert rel_err(y, yref) < 1e-10
+
+
+# ------------------------------------------------------------------ Arnoldi
+
+def _dense(rp, ci, va, n):
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, ci[rp[r]:rp[r + 1]]] += va[rp[r]:rp[r + 1]]
+    return A
+
+
+@pytest.mark.parametrize("kind", ["random", "stiff_diagonal"])
+def test_arnoldi_relation_and_orthonormality(oracle, kind):
+    """The decomposition itself, not only the y it feeds: A V_m = V_{m+1} Hbar_m
+    to rounding, V^T V = I to rounding, H upper Hessenberg with a positive
+    subdiagonal.  Without the second MGS pass the basis of the same inputs
+    loses orthogonality by two orders of magnitude more, so a dropped second
+    pass fails the orthonormality bound."""
+    n, m = 200, 40
+    if kind == "random":
+        rp, ci, va = gen.random_negative(n, 5e-2, seed=4)
+    else:
+        S = sp.diags(np.linspace(-1e4, -1.0, n)).tocsr()
+        rp, ci, va = S.indptr, S.indices, S.data
+    A = _dense(rp, ci, va, n)
+    v = gen.normal_vector(n, seed=5)
+    for reorth, orth_bound in ((True, 5e-14), (False, None)):
+        H, V, k = oracle.arnoldi(rp, ci, va, v, m, reorth, basis=True)
+        assert k == m
+        Vk = V[:, :k + 1]
+        rel = np.linalg.norm(A @ Vk[:, :k] - Vk @ H[:k + 1, :k]) / np.linalg.norm(A)
+        assert rel < 1e-14
+        orth = np.linalg.norm(Vk.T @ Vk - np.eye(k + 1))
+        if reorth:
+            assert orth < orth_bound
+            orth_two = orth
+        else:
+            assert orth > 20 * orth_two  # the check above can tell the passes apart
+        assert np.all(np.tril(H[:k + 1, :k], -2) == 0.0)
+        assert np.all(np.diag(H[:k + 1, :k], -1) > 0.0)
+    np.testing.assert_allclose(V[:, 0], v / np.linalg.norm(v), rtol=1e-14, atol=1e-300)
+
+
+def test_arnoldi_h_against_householder_krylov_qr(oracle):
+    """H of a short decomposition against an independent construction: the
+    orthonormal factor Q of the Krylov matrix [v, Av, ..., A^m v] by
+    Householder QR (R's diagonal made positive, as Gram-Schmidt with positive
+    norms produces), then Hbar = Q^T A Q[:, :m].  A transposed operand or a
+    wrong index in the MGS loop changes H far above the 1e-9 bound."""
+    n, m = 40, 6
+    rp, ci, va = gen.random_negative(n, 0.15, seed=8)
+    A = _dense(rp, ci, va, n)
+    v = gen.normal_vector(n, seed=9)
+    K = np.empty((n, m + 1))
+    K[:, 0] = v / np.linalg.norm(v)
+    for j in range(m):
+        K[:, j + 1] = A @ K[:, j]
+        K[:, j + 1] /= np.linalg.norm(K[:, j + 1])  # column scaling keeps Q unchanged
+    Q, R = np.linalg.qr(K)
+    Q = Q * np.sign(np.diag(R))
+    Href = Q.T @ A @ Q[:, :m]
+    H, V, k = oracle.arnoldi(rp, ci, va, v, m, True, basis=True)
+    assert k == m
+    np.testing.assert_allclose(V[:, :m + 1], Q, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(H, Href, rtol=0, atol=1e-9 * np.abs(Href).max())
