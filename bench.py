@@ -234,6 +234,8 @@ def batch_comparison(args, mek, gen, reps, cpu_seconds):
     * batched_fp64: every problem in one lockstep solve replayed from hipGraphs;
     * batched_mixed: the same with FP32 values requested -- stored in FP32 only
       where that is exact, so here FP64 (values_f32_used says which);
+    * batched_mixed_heuristic: what the selector replaces, m = 30 for every
+      problem and Expokit's own first step;
     * batched_mixed_learned: (m, first sub-step) from the learned selector; its
       problems/s includes the selection (device features, MLP, decoding);
     * batched_f32_forced: FP32 storage regardless, to show what it costs in
@@ -322,6 +324,8 @@ def batch_comparison(args, mek, gen, reps, cpu_seconds):
                           "krylov_columns": plain_cols, "max_rel_err": plain_err}}
     res["batched_fp64"] = batched("FP64 values, Expokit step control", ms, heur_s, False)
     res["batched_mixed"] = batched("FP32 values requested, Expokit step control", ms, heur_s, True)
+    res["batched_mixed_heuristic"] = batched("FP32 values requested, m = 30, Expokit's first step",
+                                             [min(30, n - 1) for n in rows], heur_s, True)
     learned = None
     if os.path.exists(sel.WEIGHTS):
         S = sel.Selector.load()
@@ -339,7 +343,10 @@ def batch_comparison(args, mek, gen, reps, cpu_seconds):
         learned = batched("FP32 values requested, learned (m, sub-steps)", lm, ls, True, select_s,
                           spmv_timing=True)
         learned["setup_s"] += fsetup
-        learned["m_chosen_hist"] = np.bincount(lm, minlength=61)[10::5].tolist()
+        ms_, cnt = np.unique(lm, return_counts=True)
+        learned["m_chosen_hist"] = {int(a): int(c) for a, c in zip(ms_, cnt)}
+        learned["speedup_vs_heuristic"] = round(
+            learned["problems_per_s"] / res["batched_mixed_heuristic"]["problems_per_s"], 3)
         res["batched_mixed_learned"] = learned
     forced = batched("FP32 values forced (lossy)", ms, heur_s, "lossy")
     forced["meets_1e-8"] = forced["max_rel_err"] <= 1e-8
@@ -373,6 +380,7 @@ def batch_comparison(args, mek, gen, reps, cpu_seconds):
         "value": round(head["problems_per_s"], 3),
         "rel_err": head["max_rel_err"],
         "speedup_vs_plain_fp64": round(speed, 3),
+        "speedup_learned_vs_heuristic": learned["speedup_vs_heuristic"] if learned is not None else None,
         "target_speedup": 3.0, "target_met": bool(speed >= 3.0 and head["max_rel_err"] <= 1e-8),
         "values_f32_check": f32_check,
         "roofline": roofline,

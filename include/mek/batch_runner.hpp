@@ -104,6 +104,10 @@ class BatchRunner {
   // device buffers
   int *d_row_ptr_ = nullptr, *d_col_idx_ = nullptr, *d_rb_start_ = nullptr, *d_rb_prob_ = nullptr;
   int *d_sb_prob_ = nullptr, *d_sb_row0_ = nullptr, *d_p_sb0_ = nullptr, *d_pint_ = nullptr;
+  // per-cycle compaction: first row block of each problem, problems per
+  // column, the compacted block lists and their counts
+  int *d_p_rb0_ = nullptr, *d_np_col_ = nullptr, *d_rb_list_ = nullptr, *d_sb_list_ = nullptr;
+  int* d_list_cnt_ = nullptr;
   int *d_counters_ = nullptr, *h_counters_ = nullptr;
   void* d_vals_ = nullptr;
   double *d_V_ = nullptr, *d_w_ = nullptr, *d_y_ = nullptr, *d_v0_ = nullptr, *d_small_ = nullptr;

@@ -164,6 +164,12 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     d_sb_row0_ = upload(sb_row0);
     d_sb_prob_ = upload(sb_prob);
     d_p_sb0_ = upload(p_sb0);
+    d_p_rb0_ = upload(p_rb0);
+    d_np_col_ = upload(plan_.np);
+    MEK_HIP(hipMalloc(&d_rb_list_, sizeof(int) * std::max(nrb, 1)));
+    MEK_HIP(hipMalloc(&d_sb_list_, sizeof(int) * std::max(nsb, 1)));
+    MEK_HIP(hipMalloc(&d_list_cnt_, sizeof(int) * (2 * static_cast<size_t>(M_) + 2)));
+    MEK_HIP(hipMemset(d_list_cnt_, 0, sizeof(int) * (2 * static_cast<size_t>(M_) + 2)));
     d_row0_ = upload(std::vector<int>(row0_.begin(), row0_.end() - 1));
     const long long ld = (n_tot_ + 63) / 64 * 64;
     const size_t ncols = static_cast<size_t>(M_) + 2;
@@ -203,6 +209,11 @@ BatchRunner::BatchRunner(int device_id, const std::vector<BatchProblem>& probs_i
     b->sb_row0 = d_sb_row0_;
     b->nsb = nsb;
     b->p_sb0 = d_p_sb0_;
+    b->p_rb0 = d_p_rb0_;
+    b->np_col = d_np_col_;
+    b->rb_list = d_rb_list_;
+    b->sb_list = d_sb_list_;
+    b->list_cnt = d_list_cnt_;
     b->V = d_V_;
     b->ld = ld;
     b->w = d_w_;
@@ -398,6 +409,11 @@ void BatchRunner::destroy() {
   release(d_sb_prob_);
   release(d_sb_row0_);
   release(d_p_sb0_);
+  release(d_p_rb0_);
+  release(d_np_col_);
+  release(d_rb_list_);
+  release(d_sb_list_);
+  release(d_list_cnt_);
   release(d_pint_);
   release(d_counters_);
   release(d_vals_);

@@ -15,6 +15,9 @@
 // no problem is active.  A rejected problem keeps its basis and only redoes
 // the exponential and the lift, like the single-problem runner.
 #include "batch.hpp"
+
+#include <algorithm>
+
 #include "expm_core.hpp"
 #include "sweeps.hpp"
 #include "../host/hip_check.hpp"
@@ -201,15 +204,18 @@ __global__ __launch_bounds__(kBlock) void kb_cycle_begin(BatchBufs b) {
 // ------------------------------------------------------------- columns
 
 // w = A q_j for every row block of a problem in column j (block-diagonal CSR).
-// With av = 1 (Expokit), w = A q_{m_p} for the problems in av_active.  Row
-// blocks [0, nrb_lim) are split into one contiguous range per XCD, and the
-// workgroups of an XCD stride through its range together, so the rows in
-// flight on an XCD (and the x entries they gather) stay close and in its L2.
-// Short blocks stage val*x in LDS and sum each row in storage order.  All
-// nonzero loads and x gathers of a block are issued together, and the next
-// block's problem and bounds are loaded one iteration ahead.
+// With av = 1 (Expokit), w = A q_{m_p} for the problems in av_active.  The
+// blocks are list[0, *cnt) (the compacted blocks of this cycle's problems)
+// or, with list == nullptr, row blocks [0, nrb_lim).  They are split into one
+// contiguous range per XCD, and the workgroups of an XCD stride through its
+// range together, so the rows in flight on an XCD (and the x entries they
+// gather) stay close and in its L2.  Short blocks stage val*x in LDS and sum
+// each row in storage order.  All nonzero loads and x gathers of a block are
+// issued together, and the next block's problem and bounds are loaded one
+// iteration ahead.
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, int nrb_lim) {
+__global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, int nrb_lim, const int* list,
+                                                  const int* cnt) {
   constexpr int U = kBatchNnzCap / kBlock;
   __shared__ double prod[kBatchNnzCap];
   int xcd = 0, li = blockIdx.x, gx = gridDim.x, nx = 1;
@@ -222,24 +228,27 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, in
   const int tid = threadIdx.x;
   const int* __restrict__ rp = b.row_ptr;
   const int* __restrict__ ci = b.col_idx;
-  const long long nrb = nrb_lim;
+  const long long nrb = list ? *cnt : nrb_lim;
   const int rb0 = static_cast<int>(nrb * xcd / nx);
   const int rb1 = static_cast<int>(nrb * (xcd + 1) / nx);
   int rb = rb0 + li;
+  auto blk = [&](int i) { return list ? list[i] : i; };
   // current block: problem p, rows [r0, r1), nonzeros [k0, k1); next block:
   // problem np, rows [nr0, nr1)
   int p = 0, r0 = 0, r1 = 0, k0 = 0, k1 = 0, np = 0, nr0 = 0, nr1 = 0;
   if (rb < rb1) {
-    p = b.rb_prob[rb];
-    r0 = b.rb_start[rb];
-    r1 = b.rb_start[rb + 1];
+    const int g = blk(rb);
+    p = b.rb_prob[g];
+    r0 = b.rb_start[g];
+    r1 = b.rb_start[g + 1];
     k0 = rp[r0];
     k1 = rp[r1];
   }
   if (rb + gx < rb1) {
-    np = b.rb_prob[rb + gx];
-    nr0 = b.rb_start[rb + gx];
-    nr1 = b.rb_start[rb + gx + 1];
+    const int g = blk(rb + gx);
+    np = b.rb_prob[g];
+    nr0 = b.rb_start[g];
+    nr1 = b.rb_start[g + 1];
   }
   for (; rb < rb1; rb += gx) {
     int nk0 = 0, nk1 = 0, mp = 0, mr0 = 0, mr1 = 0;
@@ -248,9 +257,10 @@ __global__ __launch_bounds__(kBlock) void kb_spmv(BatchBufs b, int j, int av, in
       nk1 = rp[nr1];
     }
     if (rb + 2 * gx < rb1) {
-      mp = b.rb_prob[rb + 2 * gx];
-      mr0 = b.rb_start[rb + 2 * gx];
-      mr1 = b.rb_start[rb + 2 * gx + 1];
+      const int g = blk(rb + 2 * gx);
+      mp = b.rb_prob[g];
+      mr0 = b.rb_start[g];
+      mr1 = b.rb_start[g + 1];
     }
     if (av ? av_active(b, p) : col_active(b, p, j)) {  // uniform over the block
       const int jc = av ? pi(b, p)[kBM] : j;
@@ -321,34 +331,52 @@ __device__ __forceinline__ TileRange sb_tiles(const BatchBufs& b, int sb) {
   return block_tiles<2>(b.sb_row0[sb], b.sb_row0[sb + 1]);
 }
 
+// The per-column sweeps stride over the streaming blocks of column j in this
+// cycle's compacted list (sb_list[0, list_cnt[M + 1 + j])); every block is
+// one problem's row range, so the test of whether that problem still takes
+// part (a breakdown earlier in the cycle ends it) is uniform per block.
+__device__ __forceinline__ int sweep_count(const BatchBufs& b, int j) { return b.list_cnt[b.M + 1 + j]; }
+
 // partials of q_i . w (i <= j) and ||w||^2 (quantity j+1) per streaming block
 template <int MAXK>
 __global__ __launch_bounds__(kBlock) void kb_dots1(BatchBufs b, int j) {
-  const int sb = blockIdx.x;
-  const int p = b.sb_prob[sb];
-  if (!col_active(b, p, j)) return;
-  sweep_dots<MAXK, 2>(b.V, b.ld, 0, j + 1, pcs(b, p), b.w, sb_tiles(b, sb), true, b.partials, b.nsb, sb);
+  const int n = sweep_count(b, j);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int sb = b.sb_list[i];
+    const int p = b.sb_prob[sb];
+    if (col_active(b, p, j))
+      sweep_dots<MAXK, 2>(b.V, b.ld, 0, j + 1, pcs(b, p), b.w, sb_tiles(b, sb), true, b.partials, b.nsb, sb);
+    __syncthreads();  // LDS of this block's reduction before the next block's
+  }
 }
 
 // w' = w - sum h1_i q_i into V[:, j+1]; partials of q_i . w' and ||w'||^2
 template <int MAXK>
 __global__ __launch_bounds__(kBlock) void kb_update_dot(BatchBufs b, int j) {
-  const int sb = blockIdx.x;
-  const int p = b.sb_prob[sb];
-  if (!col_active(b, p, j)) return;
-  sweep_update_dot<MAXK, 2>(b.V, b.ld, j + 1, b.h1 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
-                         b.w, b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb), b.partials,
-                         b.nsb, sb);
+  const int n = sweep_count(b, j);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int sb = b.sb_list[i];
+    const int p = b.sb_prob[sb];
+    if (col_active(b, p, j))
+      sweep_update_dot<MAXK, 2>(b.V, b.ld, j + 1, b.h1 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
+                                b.w, b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb),
+                                b.partials, b.nsb, sb);
+    __syncthreads();
+  }
 }
 
 // w'' = w' - sum h2_i q_i in place; partial ||w''||^2 (only if re-orthogonalising)
 template <int MAXK>
 __global__ __launch_bounds__(kBlock) void kb_update_norm(BatchBufs b, int j) {
-  const int sb = blockIdx.x;
-  const int p = b.sb_prob[sb];
-  if (!col_active(b, p, j) || !pi(b, p)[kBReorth]) return;
-  sweep_update_norm<MAXK, 2>(b.V, b.ld, j + 1, b.h2 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
-                          b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb), b.partials, sb);
+  const int n = sweep_count(b, j);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int sb = b.sb_list[i];
+    const int p = b.sb_prob[sb];
+    if (col_active(b, p, j) && pi(b, p)[kBReorth])
+      sweep_update_norm<MAXK, 2>(b.V, b.ld, j + 1, b.h2 + static_cast<long long>(p) * (b.M + 2), pcs(b, p),
+                                 b.V + static_cast<long long>(j + 1) * b.ld, sb_tiles(b, sb), b.partials, sb);
+    __syncthreads();
+  }
 }
 
 // partials of ||w||^2 per streaming block of the av_active problems
@@ -433,6 +461,88 @@ __global__ __launch_bounds__(kBlock) void kb_fin_col(BatchBufs b, int j) {
   reduce_problem(b, p, 1, tmp);
   __syncthreads();
   if (threadIdx.x == 0) finish_column_b(b, p, j, tmp[0], 1);
+}
+
+// One workgroup: the row blocks and streaming blocks of the problems that
+// build a basis in this cycle (active and needing one), in storage order,
+// and the per-column counts (problems are in order of decreasing m, so the
+// blocks of column j are a prefix of the lists).  Problems are taken 1024 at
+// a time: a scan of their block counts gives each one's offset, then one
+// wave per problem writes its block indices.
+__global__ __launch_bounds__(1024) void kb_compact(BatchBufs b) {
+  constexpr int NT = 1024, NW = NT / kWave;
+  __shared__ int s_nr[NT], s_ns[NT], s_or[NT], s_os[NT];
+  __shared__ int wsum[2][NW];
+  __shared__ int carry[2];
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  if (tid == 0) {
+    carry[0] = 0;
+    carry[1] = 0;
+  }
+  __syncthreads();
+  for (int base = 0; base < b.P; base += NT) {
+    const int p = base + tid;
+    int nr = 0, ns = 0;
+    if (p < b.P) {
+      const int* it = pi(b, p);
+      if (it[kBActive] && it[kBNeedBasis]) {
+        nr = b.p_rb0[p + 1] - b.p_rb0[p];
+        ns = b.p_sb0[p + 1] - b.p_sb0[p];
+      }
+    }
+    int sr = nr, ss = ns;  // inclusive scan within the wave
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int ur = __shfl_up(sr, off), us = __shfl_up(ss, off);
+      if (lane >= off) {
+        sr += ur;
+        ss += us;
+      }
+    }
+    if (lane == kWave - 1) {
+      wsum[0][wave] = sr;
+      wsum[1][wave] = ss;
+    }
+    __syncthreads();
+    int pr = carry[0], pq = carry[1];
+    for (int w = 0; w < wave; ++w) {
+      pr += wsum[0][w];
+      pq += wsum[1][w];
+    }
+    s_nr[tid] = nr;
+    s_ns[tid] = ns;
+    s_or[tid] = pr + sr - nr;
+    s_os[tid] = pq + ss - ns;
+    __syncthreads();
+    if (tid < b.M) {
+      const int q = b.np_col[tid] - base;
+      if (b.np_col[tid] < b.P && q >= 0 && q < NT) {
+        b.list_cnt[tid] = s_or[q];
+        b.list_cnt[b.M + 1 + tid] = s_os[q];
+      }
+    }
+    const int pend = min(NT, b.P - base);
+    for (int q = wave; q < pend; q += NW) {
+      const int r0 = b.p_rb0[base + q], s0 = b.p_sb0[base + q];
+      for (int k = lane; k < s_nr[q]; k += kWave) b.rb_list[s_or[q] + k] = r0 + k;
+      for (int k = lane; k < s_ns[q]; k += kWave) b.sb_list[s_os[q] + k] = s0 + k;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 0; w < NW; ++w) {
+        carry[0] += wsum[0][w];
+        carry[1] += wsum[1][w];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < b.M && b.np_col[tid] >= b.P) {
+    b.list_cnt[tid] = carry[0];
+    b.list_cnt[b.M + 1 + tid] = carry[1];
+  }
+  if (tid == 0) {
+    b.list_cnt[b.M] = carry[0];
+    b.list_cnt[2 * b.M + 1] = carry[1];
+  }
 }
 
 // The basis of every problem that built one this cycle is now complete.
@@ -700,8 +810,9 @@ void batch_launch_init(const BatchBufs& b, hipStream_t s) {
 }
 
 void batch_launch_spmv(const BatchBufs& b, int j, int nrb, int grid, hipStream_t s) {
-  if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(grid), dim3(kBlock), 0, s, b, j, 0, nrb);
-  else hipLaunchKernelGGL(kb_spmv<double>, dim3(grid), dim3(kBlock), 0, s, b, j, 0, nrb);
+  const int* none = nullptr;
+  if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(grid), dim3(kBlock), 0, s, b, j, 0, nrb, none, none);
+  else hipLaunchKernelGGL(kb_spmv<double>, dim3(grid), dim3(kBlock), 0, s, b, j, 0, nrb, none, none);
 }
 
 void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int mcols, int gs_mode, double eta,
@@ -709,12 +820,17 @@ void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int mcols, in
   const int pg = (b.P + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(kb_restart_copy, dim3(b.nsb), dim3(kBlock), 0, s, b);
   hipLaunchKernelGGL(kb_cycle_begin, dim3(pg), dim3(kBlock), 0, s, b);
+  hipLaunchKernelGGL(kb_compact, dim3(1), dim3(1024), 0, s, b);
   for (int j = 0; j < mcols; ++j) {
-    // problems are ordered by decreasing m: column j concerns a prefix of them
-    const int np = plan.np[j], nsb = plan.nsb[j], nrb = plan.nrb[j];
+    // problems are ordered by decreasing m: column j concerns a prefix of
+    // them; the kernels walk this cycle's compacted blocks of that prefix
+    const int np = plan.np[j];
+    const int nsb = std::min(plan.nsb[j], kBatchSweepGrid);
     const int sg = plan.spmv_grid[j];
-    if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(sg), dim3(kBlock), 0, s, b, j, 0, nrb);
-    else hipLaunchKernelGGL(kb_spmv<double>, dim3(sg), dim3(kBlock), 0, s, b, j, 0, nrb);
+    const int* rl = b.rb_list;
+    const int* rc = b.list_cnt + j;
+    if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(sg), dim3(kBlock), 0, s, b, j, 0, 0, rl, rc);
+    else hipLaunchKernelGGL(kb_spmv<double>, dim3(sg), dim3(kBlock), 0, s, b, j, 0, 0, rl, rc);
 #define MEK_L(K) hipLaunchKernelGGL(kb_dots1<K>, dim3(nsb), dim3(kBlock), 0, s, b, j)
     MEK_SWEEP_BUCKETS(j + 1, MEK_L);
 #undef MEK_L
@@ -733,8 +849,10 @@ void batch_launch_cycle(const BatchBufs& b, const BatchPlan& plan, int mcols, in
   if (b.any_expokit) {
     // ||A v_{m+1}|| of every Expokit problem that built a full basis
     const int sg = plan.spmv_grid[0];
-    if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(sg), dim3(kBlock), 0, s, b, 0, 1, b.nrb);
-    else hipLaunchKernelGGL(kb_spmv<double>, dim3(sg), dim3(kBlock), 0, s, b, 0, 1, b.nrb);
+    const int* rl = b.rb_list;
+    const int* rc = b.list_cnt + b.M;
+    if (b.vals_f32) hipLaunchKernelGGL(kb_spmv<float>, dim3(sg), dim3(kBlock), 0, s, b, 0, 1, 0, rl, rc);
+    else hipLaunchKernelGGL(kb_spmv<double>, dim3(sg), dim3(kBlock), 0, s, b, 0, 1, 0, rl, rc);
     hipLaunchKernelGGL(kb_av_norm, dim3(b.nsb), dim3(kBlock), 0, s, b);
     hipLaunchKernelGGL(kb_fin_av, dim3(b.P), dim3(kBlock), 0, s, b);
   }

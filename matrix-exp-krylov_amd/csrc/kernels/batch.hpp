@@ -9,6 +9,7 @@ namespace mek {
 constexpr int kBatchNnzCap = 2048;  // nnz of a short-row SpMV block
 constexpr int kBatchMaxM = 62;      // largest Krylov dimension of a batched problem
 constexpr int kBatchRowsPerBlock = 4096;  // rows of one streaming block
+constexpr int kBatchSweepGrid = 2048;     // workgroups of a per-column sweep (grid-stride over its list)
 constexpr int kPScal = 16;          // per-problem double slots
 constexpr int kPInt = 24;           // per-problem int slots
 
@@ -46,6 +47,15 @@ struct BatchBufs {
   const int* sb_row0;       // nsb + 1 row starts
   int nsb;
   const int* p_sb0;         // P + 1: first streaming block of each problem
+  const int* p_rb0;         // P + 1: first SpMV row block of each problem
+  const int* np_col;        // M: problems taking part in column j (a prefix: decreasing m)
+  // Blocks of the problems that build a basis in this cycle, in storage
+  // order (kb_compact at the start of every cycle): rb_list / sb_list, and
+  // list_cnt[j] (j < M) row blocks of column j, list_cnt[M] all of them,
+  // list_cnt[M + 1 + j] streaming blocks of column j, list_cnt[2M + 1] all.
+  int* rb_list;
+  int* sb_list;
+  int* list_cnt;
   double* V;
   long long ld;
   double* w;
@@ -65,7 +75,8 @@ struct BatchBufs {
 
 void batch_prepare();
 void batch_launch_init(const BatchBufs& b, hipStream_t s);
-// The column-j SpMV over row blocks [0, nrb) with `grid` workgroups.
+// The column-j SpMV over row blocks [0, nrb) with `grid` workgroups (every
+// problem taken as active; timing and tuning at the start state).
 void batch_launch_spmv(const BatchBufs& b, int j, int nrb, int grid, hipStream_t s);
 // One cycle that builds mcols Arnoldi columns (the largest m of the problems
 // still active; those with a smaller m skip the later columns).

@@ -28,6 +28,7 @@ fact = ((m+1)/e)^(m+1) sqrt(2 pi (m+1)), then Expokit's step control.
 """
 from __future__ import annotations
 
+import json
 import math
 import os
 
@@ -91,9 +92,11 @@ def problem_specs(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
     return out
 
 
-def make_problem(spec, seed: int):
-    """(row_ptr, col_idx, values, t, kind) of one spec; values are rounded to
-    FP32 so FP32 storage is exact, and t makes ||tA||_1 = tau."""
+def make_problem(spec, seed: int, f32_exact: bool = True):
+    """(row_ptr, col_idx, values, t, kind) of one spec; t makes ||tA||_1 = tau.
+    With f32_exact the values are rounded to FP32 first (the labelled
+    training sets were built so); without it they stay as generated (random
+    rows: U[-1,1] in FP64)."""
     gen = _gen()
     i, kind, size, pe, tau = spec
     if kind == "lap2d":
@@ -102,16 +105,27 @@ def make_problem(spec, seed: int):
         rp, ci, va = gen.convdiff_3d(max(size, 5), pe=pe)
     else:
         rp, ci, va = gen.random_rows(size, 5, 30, seed=seed * 100003 + i)
-    va = va.astype(np.float32).astype(np.float64)  # FP32-exact values: lossless mixed storage
+    if f32_exact:
+        va = va.astype(np.float32).astype(np.float64)
     n = rp.size - 1
     return (rp, ci, va, tau / norm1(rp, ci, va, n), kind)
 
 
-def training_set(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000):
+def spec_rows(spec) -> int:
+    """Rows of the matrix a spec builds."""
+    _, kind, size, _, _ = spec
+    if kind == "lap2d":
+        return max(size, 8) ** 2
+    if kind == "cd3d":
+        return max(size, 5) ** 3
+    return size
+
+
+def training_set(count: int, seed: int, n_lo: int = 1000, n_hi: int = 30000, f32_exact: bool = True):
     """List of (row_ptr, col_idx, values, t, kind) with ||tA||_1 log-uniform in
     [1, 1e4] (2-D Laplacians, 3-D upwind convection-diffusion with Pe in
     [0.5, 100], random rows with 5-30 nonzeros)."""
-    return [make_problem(sp, seed) for sp in problem_specs(count, seed, n_lo, n_hi)]
+    return [make_problem(sp, seed, f32_exact) for sp in problem_specs(count, seed, n_lo, n_hi)]
 
 
 def modelled_bytes(m: int, s: int, n: int, nnz: int, value_bytes: int = 4, sweeps: int = 2) -> float:
@@ -144,10 +158,59 @@ def modelled_seconds(m: int, s: int, n: int, nnz: int, banded: bool, value_bytes
     return s * ((m + 1) * bspmv / bw + (gs + 8.0 * n * (m + 2)) / BW_SWEEP)
 
 
-def label_cost(m: int, cols: int, n: int, nnz: int, kind: str) -> float:
-    """Modelled seconds of a solve that built `cols` Krylov columns with an
-    m-column basis (cols / m sub-steps)."""
+COSTS = os.path.join(HERE, "selector_costs.json")
+
+
+def load_step_costs(path=COSTS):
+    """Measured step costs of the batched solver (train_selector.py
+    --calibrate), or None when the file is absent."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def step_seconds(costs, kind: str, m: int, n: int) -> float:
+    """Measured seconds of one m-column step of a problem of `kind` with n
+    rows: the per-row cost at the two calibrated sizes, linear in m between
+    grid points and in log n between the sizes (clamped outside both)."""
+    tab = costs[kind]
+    ms = sorted(int(k) for k in tab)
+    n0, n1 = costs["n"]
+    w = min(max((math.log(n) - math.log(n0)) / (math.log(n1) - math.log(n0)), 0.0), 1.0)
+    c0 = float(np.interp(m, ms, [tab[str(k)][0] for k in ms]))
+    c1 = float(np.interp(m, ms, [tab[str(k)][1] for k in ms]))
+    return ((1.0 - w) * c0 + w * c1) * n
+
+
+def label_cost(m: int, cols: int, n: int, nnz: int, kind: str, costs=None) -> float:
+    """Seconds of a solve that built `cols` Krylov columns with an m-column
+    basis (cols / m steps): each step at its measured cost when step costs
+    are given, else each kernel's bytes at its bandwidth (modelled_seconds)."""
+    if costs is not None:
+        return step_seconds(costs, kind, m, n) * cols / m
     return modelled_seconds(m, 1, n, nnz, kind != "rows") * cols / m
+
+
+def relabel(z, costs):
+    """New targets of a labelled set from its recorded candidates (m, first
+    sub-steps, columns built): the least label_cost under `costs`.  Returns
+    (T, m_b, s_b)."""
+    P = len(z["n"])
+    T = np.zeros((P, 2), np.float32)
+    m_b = np.zeros(P, np.int32)
+    s_b = np.zeros(P, np.int64)
+    for p in range(P):
+        best = None
+        for m, s, c in zip(z["cand_m"][p], z["cand_s"][p], z["cand_cols"][p]):
+            if m < 0:
+                continue
+            cost = label_cost(int(m), int(c), int(z["n"][p]), 0, str(z["kind"][p]), costs)
+            if best is None or cost < best[0]:
+                best = (cost, int(m), int(s))
+        _, m_b[p], s_b[p] = best
+        T[p] = encode(m_b[p], s_b[p])
+    return T, m_b, s_b
 
 
 DELTA = 1.2  # Expokit's acceptance safety factor: err_loc <= DELTA * dt * tol
@@ -263,6 +326,7 @@ def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
     its s_m.  The fewest-SpMV choice is recorded beside it, and so is every
     candidate (m, s_m, columns built)."""
     mek, gen = _mods()
+    costs = load_step_costs()
     X, T, meta = [], [], []
     for c0 in range(0, len(problems), chunk):
         part = problems[c0:c0 + chunk]
@@ -274,11 +338,11 @@ def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
             R = mek.Runner(A, m=min(M_MAX + 1, n - 1), t=t, tol=tol, device=device)
             H, k = R.arnoldi(v)
             R.close()
-            X.append(mek.features(A, t, tol, device=device))
             csrs.append(A)
             vs.append(v)
             cand.append(substeps_per_m(H, k, t, tol))
         ts = [p[3] for p in part]
+        X.extend(batch_features(csrs, ts, tol, device))
         cols = [dict() for _ in part]
         for m in M_GRID:
             idx = [i for i in range(len(part)) if m in cand[i]]
@@ -297,7 +361,7 @@ def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
             # breakdown-sized candidates (m not on the grid) are solved in one step
             for m in cand[i]:
                 cols[i].setdefault(m, m)
-            cost = {m: label_cost(m, c, n, nnz, kind) for m, c in cols[i].items()}
+            cost = {m: label_cost(m, c, n, nnz, kind, costs) for m, c in cols[i].items()}
             m_b = min(cost, key=lambda mm: cost[mm])
             m_s = min(cols[i], key=lambda mm: cols[i][mm])
             T.append(encode(m_b, cand[i][m_b]))
@@ -310,6 +374,18 @@ def build_dataset(problems, tol=1e-8, device=0, log=None, chunk=256):
         if log:
             log(f"labelled {min(c0 + chunk, len(problems))}/{len(problems)}")
     return np.array(X, np.float32), np.array(T, np.float32), meta
+
+
+def batch_features(csrs, ts, tol=1e-8, device=0):
+    """The 16 features of every (A_p, t_p) in one device pass (a batch of the
+    already-built CSR handles; the features do not depend on m)."""
+    mek, _ = _mods()
+    B = mek.Batch(csrs, ts, [min(30, c.info()["rows"] - 1) for c in csrs], tol, values_f32=True,
+                  device=device)
+    try:
+        return B.features()
+    finally:
+        B.close()
 
 
 def train(X, T, epochs=300, batch=128, lr=2e-3, seed=0, device=0):
