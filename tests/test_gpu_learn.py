@@ -72,54 +72,33 @@ def test_batch_features_equal_single(gpu):
 
 def test_trained_selector_gains_over_heuristic(gpu):
     """The committed selector (trained on the GPU by train_selector.py) against
-    the method without it, on problems it has not seen: the same batched
-    mixed-precision Expokit solve with (m = 30, Expokit's first step) and with
-    the predicted (m, first step).  The learned choice must move at least 10 %
-    fewer modelled HBM bytes (the quantity it was trained to minimise),
-    converge everywhere and keep the relative error within 1e-8 of a tight
-    FP64 solve."""
+    the method without it, on problems it has not seen (48 problems: 2-D
+    Laplacians, 3-D convection-diffusion and random rows, n log-uniform in
+    [2e4, 1e6], ||tA||_1 log-uniform in [1, 1e4], seed 77; about 15M rows):
+    the same batched Expokit solve with (m = 30, Expokit's first step) and with
+    the predicted (m, first step), through train_selector.evaluate_chunk.  The
+    learned form's measured wall time includes its selection (device features
+    of the resident matrices, the MLP forward pass, decoding) and must be at
+    least 1.05x faster; both forms must converge everywhere and stay within
+    1e-8 of a tight FP64 solve.  (On the 2,000-problem held-out set the
+    selector is 1.15x faster in the same measure.)"""
     import importlib.util
     import os
     here = os.path.join(os.path.dirname(__file__), "..", "matrix-exp-krylov_amd")
-    spec = importlib.util.spec_from_file_location("mek_selector", os.path.join(here, "selector.py"))
-    sel = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(sel)
+    spec = importlib.util.spec_from_file_location("mek_train_selector", os.path.join(here, "train_selector.py"))
+    ts = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ts)
+    sel = ts.sel
     S = sel.Selector.load()
-    tol = 1e-8
-    probs = sel.training_set(48, seed=77, n_lo=5000, n_hi=60000)
-    mats = [(rp, ci, va) for rp, ci, va, t, kind in probs]
-    ts = [t for *_x, t, kind in probs]
-    vs = [gen.normal_vector(rp.size - 1, seed=500 + i) for i, (rp, *_r) in enumerate(mats)]
-    csrs = [mek.CSR.from_arrays(*A) for A in mats]
-    F = np.array([mek.features(c, t, tol) for c, t in zip(csrs, ts)])
-    learned = S.predict_features(F)
-    refs = []
-    for c, t, v, (rp, *_r) in zip(csrs, ts, vs, mats):
-        R = mek.Runner(c, m=60, t=t, tol=1e-13, max_restarts=0, step_control="expokit_rel", gs_mode=1)
-        refs.append(R.expmv(v))
-        R.close()
-    res = {}
-    for name, choice in (("heuristic", [(30, 0)] * len(mats)), ("learned", learned)):
-        B = mek.Batch(csrs, ts, [m for m, s in choice], [tol / t for t in ts],
-                      substeps=[s for m, s in choice], values_f32=True, gs_mode=2,
-                      step_control="expokit_rel")
-        B.upload(vs)
-        B.solve()
-        ys = B.download()
-        cols = by = 0.0
-        errs = []
-        for p, ((rp, ci, va), (m, s)) in enumerate(zip(mats, choice)):
-            st = B.stats(p)
-            assert st["converged"] == 1, (name, p)
-            cols += st["spmv_count"]
-            by += sel.modelled_bytes(m, 1, rp.size - 1, int(rp[-1])) * st["spmv_count"] / m
-            r = refs[p]
-            if np.linalg.norm(r) >= 1e-3:
-                errs.append(np.linalg.norm(ys[p] - r) / np.linalg.norm(r))
-        res[name] = (cols, by, max(errs))
-        B.close()
-    print(f"[selector] heuristic cols={res['heuristic'][0]:.0f} bytes={res['heuristic'][1]:.3e} "
-          f"err={res['heuristic'][2]:.2e}; learned cols={res['learned'][0]:.0f} "
-          f"bytes={res['learned'][1]:.3e} err={res['learned'][2]:.2e}", flush=True)
-    assert res["learned"][1] < 0.9 * res["heuristic"][1]
-    assert res["learned"][2] <= 1e-8 and res["heuristic"][2] <= 1e-8
+    probs = sel.training_set(48, seed=77, n_lo=20000, n_hi=1000000, f32_exact=False)
+    print(f"[selector] {len(probs)} problems, {sum(p[0].size - 1 for p in probs)} rows", flush=True)
+    ev = ts.evaluate_chunk(S, probs, tol=1e-8)
+    h, le = ev["heuristic"], ev["learned"]
+    ratio = h["wall_s"] / le["wall_s"]
+    print(f"[selector] heuristic {h['wall_s']:.3f} s, {h['krylov_columns']} columns, "
+          f"err {max(h['rel_errs'], default=0.0):.2e}; learned {le['wall_s']:.3f} s (selection {ev['select_s'] * 1e3:.1f} ms), "
+          f"{le['krylov_columns']} columns, err {max(le['rel_errs'], default=0.0):.2e}; wall ratio {ratio:.3f}", flush=True)
+    assert h["converged"] == len(probs) and le["converged"] == len(probs)
+    assert max(le["rel_errs"], default=0.0) <= 1e-8 and max(h["rel_errs"], default=0.0) <= 1e-8
+    assert max(le["abs_errs"]) <= 1e-8 and max(h["abs_errs"]) <= 1e-8
+    assert ratio >= 1.05, ratio
