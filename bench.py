@@ -560,9 +560,10 @@ def run_single(args, mek, gen):
         return mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
                           gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=timing,
                           fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune,
-                          sstep=args.sstep, sstep_first=args.sstep_first)
+                          sstep=args.sstep, sstep_first=args.sstep_first, spmv_format=args.spmv_format)
 
     R = runner(False)
+    layout = R.spmv_format()
     f32_values = None
     if w["f32"]:
         c, mrel, bad = A.f32_inexact(nonfinite=True)
@@ -591,9 +592,10 @@ def run_single(args, mek, gen):
     span_steps = max(1, min(args.steps, 5))
     log(f"timing {span_steps} solves on the METRIC span (host v and y)")
     span = []
+    yh = np.ones(n)  # the caller's y, allocated and touched before the span as in main.cu:2281
     for _ in range(span_steps):
         t0 = time.perf_counter()
-        yh = R.expmv(v)
+        R.expmv(v, out=yh)
         span.append(time.perf_counter() - t0)
     assert np.array_equal(yh, y)
     tuning = R.tuning() if args.autotune else None
@@ -643,6 +645,11 @@ def run_single(args, mek, gen):
         roofline["traffic"] = tr[dominant]
     if spmv_roof is not None and "spmv" in tr:
         spmv_roof["traffic"] = tr["spmv"]
+    if spmv_roof is not None:
+        # the same launches priced at the CSR matrix's bytes (nnz (value + 4) + 4 (n + 1) + 16 n)
+        csr_bytes = nnz * ((4 if (w["f32"] and f32_values["values_f32_used"]) else 8) + 4) + 4.0 * (n + 1) + 16.0 * n
+        spmv_roof["layout"] = layout
+        spmv_roof["csr_equivalent_gbs"] = round(csr_bytes / (spmv_roof["avg_launch_us"] * 1e-6) / 1e9, 1)
 
     solves_per_s = 1000.0 / ms_per_step
     out = {
@@ -661,7 +668,8 @@ def run_single(args, mek, gen):
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "t": w["t"], "m": w["m"],
                    "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
                    "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "autotune": args.autotune,
-                   "sstep": args.sstep, "sstep_first": args.sstep_first, "seed": 0},
+                   "sstep": args.sstep, "sstep_first": args.sstep_first, "spmv_format": layout["format"],
+                   "seed": 0},
         "rel_err": rel,
         "values_f32": f32_values,
         "metric_span": {"solves_per_s": round(1.0 / statistics.median(span), 4),
@@ -707,6 +715,8 @@ def main():
     ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
     ap.add_argument("--sstep", type=int, default=12,
                     help="s-step blocks of this many basis vectors (0: column by column, as the reference)")
+    ap.add_argument("--spmv-format", default="auto", choices=["auto", "csr", "sell"],
+                    help="SpMV layout: sliced ELL where its padding is small (auto), CSR row blocks, or sliced ELL")
     ap.add_argument("--sstep-first", type=int, default=6,
                     help="columns of each cycle built one by one before the blocks (-1: --sstep)")
     args = ap.parse_args()

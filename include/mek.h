@@ -102,6 +102,8 @@ typedef struct {
                            0: column by column); needs m + 1 <= 64        */
     int sstep_first;    /* columns of each cycle built one by one before the
                            blocks (-1: sstep)                               */
+    int spmv_format;    /* 0 sliced ELL when its padding is small, else CSR
+                           row blocks; 1 CSR row blocks; 2 sliced ELL       */
 } mek_solver_options;
 
 void mek_solver_options_default(mek_solver_options* opt);
@@ -162,6 +164,9 @@ int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sw
 /* Workgroups of the s-step block sweeps (0 when the block path is off) and,
  * with autotune, the timed dots + update pair of the chosen grid (us). */
 int mek_runner_sstep_grid(const mek_runner* R, int* grid, double* us);
+/* SpMV layout in use: *format 1 CSR row blocks, 2 sliced ELL; for the
+ * latter the padded entry count and the widest row (else 0). */
+int mek_runner_spmv_format(const mek_runner* R, int* format, long long* entries, int* width);
 /* The runner's stream (hipStream_t). */
 int mek_runner_stream(const mek_runner* R, void** stream);
 void mek_runner_destroy(mek_runner* R);
@@ -243,6 +248,13 @@ int mek_spmv_plan_create(int n, long long nnz, const int* d_row_ptr, const int* 
                          mek_spmv_plan** out);
 /* d_y = A * d_x (beta = 0).  Replaces cusparseSpMV, main.cu:1743-1745. */
 int mek_spmv_execute(mek_spmv_plan* P, const double* d_x, double* d_y, void* stream);
+/* Switch the plan to a sliced-ELL copy of its matrix (64-row slices, one row
+ * per lane, each slice padded to its longest row), built on the device from
+ * the plan's CSR arrays; h_row_ptr as at create.  The same y bits as the CSR
+ * row blocks.  *entries and *width (may be null) receive the padded entry
+ * count and the widest row.  No reference counterpart (cuSPARSE chooses its
+ * own algorithm). */
+int mek_spmv_plan_use_sell(mek_spmv_plan* P, const int* h_row_ptr, long long* entries, int* width);
 void mek_spmv_plan_destroy(mek_spmv_plan* P);
 
 /* d_E = exp(d_A) for a dense k x k column-major matrix on the device (one

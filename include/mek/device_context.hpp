@@ -44,6 +44,15 @@ struct DeviceContext {
   int grid = 0;                   // blocks of the streaming kernels
   int spmv_grid = 1;              // blocks of the SpMV kernel
 
+  // sliced-ELL copy of the matrix (build_sell): when set, the SpMV reads it
+  // instead of the CSR row blocks
+  long long* d_sell_ptr = nullptr;  // n_slices + 1 entry offsets
+  int* d_sell_col = nullptr;        // padding: -1
+  void* d_sell_val = nullptr;       // double, or float when values_f32
+  int n_slices = 0;
+  long long sell_entries = 0;       // entries with padding
+  int sell_width = 0;               // widest slice (entries per row)
+
   // s-step blocks (allocated by alloc_sstep): state, flags, sweep partials
   double* d_ss_state = nullptr;
   int* d_ss_flags = nullptr;
@@ -67,6 +76,12 @@ struct DeviceContext {
   void create_descriptors(const CSRHost& part, int global_cols, const CSRHost::GhostMap& ghost_map,
                           int global_row_offset, int m);
   void destroy();
+  // Additive: the sliced-ELL copy of the device CSR (after alloc_from).
+  // format 0: built when its padding is at most mek::kSellMaxPad of nnz and
+  // no slice is wider than mek::kSellMaxWidth; 1: never (CSR row blocks);
+  // 2: always.  Returns whether the SpMV uses it.
+  bool build_sell(const CSRHost& part, int format);
+  void free_sell();
   // Additive: buffers of the s-step block path (sstep.hip), shifts uploaded.
   void alloc_sstep(const std::vector<double>& shifts);
   mek::SsArgs ss_args() const;
@@ -81,4 +96,11 @@ namespace mek {
 // (and at most row_cap rows); a row longer than nnz_cap is a block of its own.
 // Returns the n_blocks+1 block start rows.
 std::vector<int> build_row_blocks(const int* row_ptr, int n, int nnz_cap, int row_cap);
+
+// Sliced-ELL layout: slices of 64 consecutive rows, each padded to its
+// longest row.  Fills the n_slices + 1 entry offsets and returns the padded
+// entry count; *max_width receives the widest slice's row length.
+long long sell_layout(const int* row_ptr, int n, std::vector<long long>& ptr, int* max_width);
+constexpr double kSellMaxPad = 0.15;  // automatic choice: padding <= 15 % of nnz
+constexpr int kSellMaxWidth = 256;    // ... and no row longer than this
 }  // namespace mek

@@ -26,6 +26,7 @@ KERNEL_KINDS = {"spmv": 0, "update_dot": 1, "update_norm": 2, "dots": 3, "final"
 
 
 STEP_CONTROL = {"reference": 0, "expokit": 1, "expokit_rel": 2}
+SPMV_FORMAT = {"auto": 0, "csr": 1, "sell": 2}
 
 
 class MekError(RuntimeError):
@@ -42,7 +43,7 @@ class SolverOptions(ctypes.Structure):
                 ("dgks_eta", ctypes.c_double), ("values_f32", ctypes.c_int),
                 ("verbose", ctypes.c_int), ("timing", ctypes.c_int),
                 ("fuse_spmv_dot", ctypes.c_int), ("lanczos", ctypes.c_int), ("autotune", ctypes.c_int),
-                ("sstep", ctypes.c_int), ("sstep_first", ctypes.c_int)]
+                ("sstep", ctypes.c_int), ("sstep_first", ctypes.c_int), ("spmv_format", ctypes.c_int)]
 
 
 class ArnoldiParams(ctypes.Structure):
@@ -134,6 +135,8 @@ SIGNATURES = {
                                             ctypes.c_int, _ip, ctypes.POINTER(_vp)]),
     "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mek_spmv_plan_destroy": (None, [_vp]),
+    "mek_spmv_plan_use_sell": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
+    "mek_runner_spmv_format": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
     "mek_expm_dense_noalloc": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp]),
     "mek_runner_tuning": (ctypes.c_int, [_vp, _ip, _ip, _ip, _dp, _dp]),
@@ -314,7 +317,7 @@ class Runner:
     def __init__(self, A: CSR, m=30, t=1.0, tol=1e-8, max_restarts=10, reorthogonalize=True,
                  step_control="reference", gs_mode=-1, dgks_eta=0.1, values_f32=False,
                  verbose=False, timing=False, fuse_spmv_dot=False, lanczos=False, autotune=False, device=0,
-                 sstep=0, sstep_first=-1):
+                 sstep=0, sstep_first=-1, spmv_format="auto"):
         p = ArnoldiParams(int(m), float(t), float(tol), int(max_restarts), 1 if reorthogonalize else 0)
         o = SolverOptions()
         lib().mek_solver_options_default(ctypes.byref(o))
@@ -329,15 +332,25 @@ class Runner:
         o.autotune = 1 if autotune else 0
         o.sstep = int(sstep)
         o.sstep_first = int(sstep_first)
+        o.spmv_format = SPMV_FORMAT[spmv_format]
         h = _vp()
         _check(lib().mek_runner_create(A.h, ctypes.byref(p), ctypes.byref(o), int(device), ctypes.byref(h)))
         self.h = h
         self.n = A.info()["rows"]
         self.m = int(m)
 
-    def expmv(self, v):
+    def expmv(self, v, out=None):
+        """y = exp(t A) v for a host vector v.  `out` (a C-contiguous float64
+        array of n entries) receives y in place, as the reference's
+        compute_expmv writes into the caller's y vector."""
         v = _f64(v)
-        y = np.zeros(self.n, np.float64)
+        if out is None:
+            y = np.zeros(self.n, np.float64)
+        else:
+            y = out
+            if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous
+                    and y.flags.writeable and y.size == self.n):
+                raise ValueError(f"out must be a writeable C-contiguous float64 array of {self.n} entries")
         _check(lib().mek_runner_expmv(self.h, _pd(v), _pd(y)))
         return y
 
@@ -385,6 +398,14 @@ class Runner:
         _check(lib().mek_runner_sstep_grid(self.h, ctypes.byref(sg), ctypes.byref(su)))
         return {"nnz_cap": c.value, "spmv_grid": g.value, "sweep_grid": w.value,
                 "spmv_us": a.value, "sweep_us": b.value, "sstep_grid": sg.value, "sstep_us": su.value}
+
+    def spmv_format(self):
+        """The SpMV layout in use: {"format": "csr" | "sell", "entries": padded
+        entries (sliced ELL), "width": widest row (sliced ELL)}."""
+        f, w = ctypes.c_int(), ctypes.c_int()
+        e = ctypes.c_longlong()
+        _check(lib().mek_runner_spmv_format(self.h, ctypes.byref(f), ctypes.byref(e), ctypes.byref(w)))
+        return {"format": "sell" if f.value == 2 else "csr", "entries": e.value, "width": w.value}
 
     def arnoldi(self, v):
         v = _f64(v)
@@ -573,6 +594,14 @@ class SpmvPlan:
         _check(lib().mek_spmv_plan_create(self.n, self.nnz, self.d_rp.ptr, self.d_ci.ptr, self.d_va.ptr,
                                           1 if values_f32 else 0, _pi(rp), ctypes.byref(h)))
         self.h = h
+        self._rp = rp
+
+    def use_sell(self):
+        """Switch to the sliced-ELL copy of the matrix; returns (padded
+        entries, widest row)."""
+        e, w = ctypes.c_longlong(), ctypes.c_int()
+        _check(lib().mek_spmv_plan_use_sell(self.h, _pi(self._rp), ctypes.byref(e), ctypes.byref(w)))
+        return e.value, w.value
 
     def execute(self, d_x: DeviceBuffer, d_y: DeviceBuffer, stream=None):
         _check(lib().mek_spmv_execute(self.h, d_x.ptr, d_y.ptr, stream))

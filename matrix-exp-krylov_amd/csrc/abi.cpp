@@ -45,6 +45,9 @@ struct mek_spmv_plan {
   int* d_blocks = nullptr;
   double* d_partials = nullptr;
   int* d_iscal = nullptr;
+  long long* d_sell_ptr = nullptr;
+  int* d_sell_col = nullptr;
+  void* d_sell_val = nullptr;
 };
 
 namespace {
@@ -201,6 +204,7 @@ void mek_solver_options_default(mek_solver_options* o) {
   o->autotune = d.autotune;
   o->sstep = d.sstep;
   o->sstep_first = d.sstep_first;
+  o->spmv_format = d.spmv_format;
 }
 
 int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_solver_options* o,
@@ -230,6 +234,8 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       need(o->sstep_first >= -1, "sstep_first >= -1");
       so.sstep = o->sstep;
       so.sstep_first = o->sstep_first;
+      need(o->spmv_format >= 0 && o->spmv_format <= 2, "spmv_format in {0, 1, 2}");
+      so.spmv_format = o->spmv_format;
     }
     try {
       h->R->init_device(A->A, device_id);  // reads the matrix in place
@@ -331,6 +337,16 @@ int mek_runner_arnoldi(mek_runner* R, const double* v, double* H, int* k_used) {
     const int k = R->R->arnoldi_decomposition(vh, Hh);
     std::memcpy(H, Hh.data(), sizeof(double) * Hh.size());
     if (k_used) *k_used = k;
+  });
+}
+
+int mek_runner_spmv_format(const mek_runner* R, int* format, long long* entries, int* width) {
+  return guarded([&] {
+    need(R && R->R && format, "runner, format");
+    const DeviceContext& dc = R->R->device_contexts[0];
+    *format = dc.n_slices > 0 ? 2 : 1;
+    if (entries) *entries = dc.sell_entries;
+    if (width) *width = dc.sell_width;
   });
 }
 
@@ -579,8 +595,47 @@ int mek_spmv_execute(mek_spmv_plan* P, const double* d_x, double* d_y, void* str
   });
 }
 
+int mek_spmv_plan_use_sell(mek_spmv_plan* P, const int* h_row_ptr, long long* entries, int* width) {
+  return guarded([&] {
+    need(P && h_row_ptr, "plan, h_row_ptr");
+    mek::ArnoldiBufs& b = P->b;
+    need(P->d_sell_ptr == nullptr, "the plan already uses the sliced-ELL copy");
+    std::vector<long long> ptr;
+    int wmax = 0;
+    const long long ne = mek::sell_layout(h_row_ptr, b.n, ptr, &wmax);
+    const size_t nes = static_cast<size_t>(std::max<long long>(ne, 1));
+    const int ns = static_cast<int>(ptr.size()) - 1;
+    const int grid = std::max(1, std::min(mek::kMaxGrid, (ns + 15) / 16));
+    if (grid > b.spmv_grid) {  // partials for the larger of the two grids
+      double* d_part = nullptr;
+      MEK_HIP(hipMalloc(&d_part, sizeof(double) * grid));
+      if (P->d_partials) (void)hipFree(P->d_partials);
+      P->d_partials = d_part;
+      b.partials = d_part;
+    }
+    MEK_HIP(hipMalloc(&P->d_sell_ptr, sizeof(long long) * ptr.size()));
+    MEK_HIP(hipMalloc(&P->d_sell_col, sizeof(int) * nes));
+    MEK_HIP(hipMalloc(&P->d_sell_val, (b.vals_f32 ? sizeof(float) : sizeof(double)) * nes));
+    MEK_HIP(hipMemcpy(P->d_sell_ptr, ptr.data(), sizeof(long long) * ptr.size(), hipMemcpyHostToDevice));
+    mek::launch_sell_build(b.row_ptr, b.col_idx, b.vals, b.vals_f32, b.n, P->d_sell_ptr, P->d_sell_col,
+                           P->d_sell_val, nullptr);
+    MEK_HIP_LAUNCHED();
+    MEK_HIP(hipDeviceSynchronize());
+    b.sell_ptr = P->d_sell_ptr;
+    b.sell_col = P->d_sell_col;
+    b.sell_val = P->d_sell_val;
+    b.n_slices = ns;
+    b.spmv_grid = grid;
+    if (entries) *entries = ne;
+    if (width) *width = wmax;
+  });
+}
+
 void mek_spmv_plan_destroy(mek_spmv_plan* P) {
   if (!P) return;
+  if (P->d_sell_ptr) (void)hipFree(P->d_sell_ptr);
+  if (P->d_sell_col) (void)hipFree(P->d_sell_col);
+  if (P->d_sell_val) (void)hipFree(P->d_sell_val);
   if (P->d_blocks) (void)hipFree(P->d_blocks);
   if (P->d_partials) (void)hipFree(P->d_partials);
   if (P->d_iscal) (void)hipFree(P->d_iscal);

@@ -107,6 +107,17 @@ std::vector<double> sstep_shifts(const CSRHost& A, int s) {
   return out;
 }
 
+// Bytes one SpMV streams: the matrix in its layout (CSR: values, column
+// indices, row pointers; sliced ELL: padded values and column indices, slice
+// offsets), x once and w.  csr = true: the CSR figure whatever the layout.
+double spmv_bytes(const DeviceContext& dc, bool csr = false) {
+  const double vb = dc.values_f32 ? 4.0 : 8.0;
+  const double n = dc.local_rows;
+  if (dc.n_slices > 0 && !csr)
+    return static_cast<double>(dc.sell_entries) * (vb + 4.0) + 8.0 * (dc.n_slices + 1.0) + 16.0 * n;
+  return static_cast<double>(dc.local_nnz) * (vb + 4.0) + 4.0 * (n + 1.0) + 16.0 * n;
+}
+
 double round_step(double x) {
   const double s = std::pow(10.0, std::floor(std::log10(x)) - 1.0);
   return std::ceil(x / s) * s;
@@ -183,6 +194,9 @@ void ArnoldiRunner::init_one(const CSRHost& part, int device_id) {
   dc.alloc_from(part, global_cols, alloc_m, values_f32_used);
   static const CSRHost::GhostMap kEmpty;
   dc.create_descriptors(part, global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0, alloc_m);
+  if (options.spmv_format < 0 || options.spmv_format > 2)
+    throw mek::Error("ERROR: spmv_format must be 0, 1 or 2");
+  dc.build_sell(part, options.spmv_format);
   if (options.autotune) tuning = mek::autotune_single(dc, part, alloc_m);
   anorm_inf = part.norm_inf();
   if (options.sstep < 0 || options.sstep == 1 || options.sstep > mek::kSsMaxS)
@@ -329,8 +343,7 @@ void ArnoldiRunner::run_sstep_block(int k, int ss) {
   const mek::SsArgs a = dc.ss_args();
   hipStream_t s = dc.stream_compute;
   const double rows = static_cast<double>(a.rows);
-  const double bspmv = static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
-                       4.0 * (dc.local_rows + 1.0) + 16.0 * dc.local_rows;
+  const double bspmv = spmv_bytes(dc);
   const int gs_mode = (options.gs_mode >= 0) ? options.gs_mode : (params.reorthogonalize ? 1 : 0);
   const int force2 = gs_mode == 1 ? 1 : 0;
   for (int i = 0; i < ss; ++i) {
@@ -376,8 +389,8 @@ void ArnoldiRunner::run_columns_range(int j0, int j1) {
   const mek::ArnoldiBufs b = dc.bufs();
   hipStream_t s = dc.stream_compute;
   const double n = dc.local_rows;
-  const double bspmv = static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
-                       4.0 * (n + 1) + 16.0 * n;
+  const double bspmv = spmv_bytes(dc);
+  const double bspmv_csr = spmv_bytes(dc, true);  // the fused kernel reads the CSR row blocks
   const int gs_mode = (options.gs_mode >= 0) ? options.gs_mode : (params.reorthogonalize ? 1 : 0);
   // quantity slot for the SpMV's ||w||^2 partial when the dots kernel
   // computes the norm itself (partials then share one block stride)
@@ -387,7 +400,7 @@ void ArnoldiRunner::run_columns_range(int j0, int j1) {
     if (nq <= mek::kMaxFusedDots && options.fuse_spmv_dot) {
       begin_timed(mek::kKSpmv);
       mek::launch_spmv_dots(b, j, s);
-      end_timed(mek::kKSpmv, bspmv + 8.0 * n * nq);
+      end_timed(mek::kKSpmv, bspmv_csr + 8.0 * n * nq);
       mek::launch_fin_h1(b, nq, b.spmv_grid, s);
     } else {
       begin_timed(mek::kKSpmv);
@@ -433,8 +446,7 @@ void ArnoldiRunner::run_lanczos_columns(int m_cols) {
   const mek::ArnoldiBufs b = dc.bufs();
   hipStream_t s = dc.stream_compute;
   const double n = dc.local_rows;
-  const double bspmv = static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
-                       4.0 * (n + 1) + 16.0 * n;
+  const double bspmv = spmv_bytes(dc);
   for (int j = 0; j < m_cols; ++j) {
     begin_timed(mek::kKSpmv);
     mek::launch_spmv_lanczos(b, j, s);

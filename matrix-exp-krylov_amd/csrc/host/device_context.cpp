@@ -30,6 +30,21 @@ std::vector<int> build_row_blocks(const int* row_ptr, int n, int nnz_cap, int ro
   return starts;
 }
 
+long long sell_layout(const int* row_ptr, int n, std::vector<long long>& ptr, int* max_width) {
+  const long long ns = (static_cast<long long>(std::max(n, 0)) + kWave - 1) / kWave;
+  ptr.assign(static_cast<size_t>(ns) + 1, 0);
+  int wmax = 0;
+  for (long long s = 0; s < ns; ++s) {
+    int w = 0;
+    const long long r1 = std::min<long long>(n, (s + 1) * kWave);
+    for (long long r = s * kWave; r < r1; ++r) w = std::max(w, row_ptr[r + 1] - row_ptr[r]);
+    wmax = std::max(wmax, w);
+    ptr[s + 1] = ptr[s] + static_cast<long long>(w) * kWave;
+  }
+  if (max_width) *max_width = wmax;
+  return ptr[ns];
+}
+
 // Blocks of the streaming kernels: up to 4096 (16 per CU, measured to need
 // >= 2048 for full HBM bandwidth), at least 4 rows per thread, a multiple of
 // the 8 XCDs.
@@ -119,6 +134,48 @@ void DeviceContext::create_descriptors(const CSRHost& part, int /*global_cols*/,
   if (log) std::cout << "  GPU " << device_id << " SpMV buffer size: " << bytes << " bytes" << std::endl;
 }
 
+bool DeviceContext::build_sell(const CSRHost& part, int format) {
+  MEK_HIP(hipSetDevice(device_id));
+  free_sell();
+  if (format == 1 || local_rows <= 0) return false;
+  std::vector<long long> ptr;
+  int wmax = 0;
+  const long long ne = mek::sell_layout(part.row_ptr.data(), local_rows, ptr, &wmax);
+  if (format == 0 &&
+      (wmax > mek::kSellMaxWidth || static_cast<double>(ne) > (1.0 + mek::kSellMaxPad) * static_cast<double>(local_nnz)))
+    return false;
+  const size_t nes = static_cast<size_t>(std::max<long long>(ne, 1));
+  MEK_HIP(hipMalloc(&d_sell_ptr, sizeof(long long) * ptr.size()));
+  MEK_HIP(hipMalloc(&d_sell_col, sizeof(int) * nes));
+  MEK_HIP(hipMalloc(&d_sell_val, (values_f32 ? sizeof(float) : sizeof(double)) * nes));
+  MEK_HIP(hipMemcpy(d_sell_ptr, ptr.data(), sizeof(long long) * ptr.size(), hipMemcpyHostToDevice));
+  mek::launch_sell_build(d_row_ptr, d_col_idx, d_values, values_f32 ? 1 : 0, local_rows, d_sell_ptr, d_sell_col,
+                         d_sell_val, stream_compute);
+  MEK_HIP_LAUNCHED();
+  MEK_HIP(hipStreamSynchronize(stream_compute));
+  n_slices = static_cast<int>(ptr.size() - 1);
+  sell_entries = ne;
+  sell_width = wmax;
+  // four waves per workgroup, each a slice at a time; a multiple of the 8
+  // XCDs when there is work for that many
+  int g = std::min(mek::kMaxGrid, (n_slices + 15) / 16);
+  spmv_grid = g >= 8 ? (g + 7) / 8 * 8 : std::max(1, g);
+  spmv_grid = std::min(spmv_grid, mek::kMaxGrid);
+  return true;
+}
+
+void DeviceContext::free_sell() {
+  if (d_sell_ptr) (void)hipFree(d_sell_ptr);
+  if (d_sell_col) (void)hipFree(d_sell_col);
+  if (d_sell_val) (void)hipFree(d_sell_val);
+  d_sell_ptr = nullptr;
+  d_sell_col = nullptr;
+  d_sell_val = nullptr;
+  n_slices = 0;
+  sell_entries = 0;
+  sell_width = 0;
+}
+
 mek::ArnoldiBufs DeviceContext::bufs() const {
   ArnoldiBufs b{};
   const size_t M = static_cast<size_t>(allocated_m_max);
@@ -145,6 +202,10 @@ mek::ArnoldiBufs DeviceContext::bufs() const {
   b.partials = d_partials;
   b.grid = grid;
   b.spmv_grid = spmv_grid;
+  b.sell_ptr = d_sell_ptr;
+  b.sell_col = d_sell_col;
+  b.sell_val = d_sell_val;
+  b.n_slices = n_slices;
   return b;
 }
 
@@ -192,6 +253,7 @@ void DeviceContext::destroy() {
   fr(d_col_idx);
   fr(d_values);
   fr(d_row_blocks);
+  free_sell();
   fr(d_V_m);
   fr(d_w);
   fr(d_y);

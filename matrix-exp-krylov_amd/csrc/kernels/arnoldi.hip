@@ -25,6 +25,8 @@
 // a fixed-order in-block reduction (DPP), and per-block partials summed in
 // block order.  After a breakdown every later kernel of the attempt returns at
 // once, so the host can enqueue all m columns without reading back.
+#include <algorithm>
+
 #include "common.hpp"
 #include "launch.hpp"
 #include "sweeps.hpp"
@@ -126,6 +128,15 @@ __device__ __forceinline__ double sweep_combination(const double* __restrict__ V
 
 // ------------------------------------------------------------------- K1
 
+// The Newton shift of a row sum, a - theta (xs x[r]), with a separately
+// rounded product (no fused multiply-add), so that every SpMV form gives the
+// same bits.
+__device__ __forceinline__ double shift_row(double a, double theta, double xs, double xr) {
+#pragma clang fp contract(off)
+  const double p = theta * (xs * xr);
+  return a - p;
+}
+
 // w = A * (xs * x) with xs = *xs_ptr (1 when null); partial sums of
 // q_i . w for i < ndots (quantity i) and of ||w||^2 (quantity normq).  Each
 // workgroup owns a contiguous range of SpMV row blocks (XCD-aware), so x
@@ -226,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
         }
         double a = 0.0;
         for (int e = e0; e < e1; ++e) a += prod[e];
-        if (has_shift) a -= sh * (xs * xrr);
+        if (has_shift) a = shift_row(a, sh, xs, xrr);
         __builtin_nontemporal_store(a, w + r);
         acc[MAXJ] += a * a;
 #pragma unroll
@@ -245,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
       if (tid == 0) {
         double t = prod[0];
         for (int q = 1; q < kWavesPerBlock; ++q) t += prod[q];
-        if (has_shift) t -= sh * (xs * x[r0]);
+        if (has_shift) t = shift_row(t, sh, xs, x[r0]);
         __builtin_nontemporal_store(t, w + r0);
         acc[MAXJ] += t * t;
 #pragma unroll
@@ -264,6 +275,113 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(ArnoldiBufs b, const double
   __syncthreads();
   double nv[1] = {acc[MAXJ]};
   block_partials<1>(nv, 1, red, b.partials + static_cast<long long>(normq) * grid, grid, lb);
+}
+
+// K1 on the sliced-ELL copy (ArnoldiBufs::sell_*): the same operation and
+// arguments as k_spmv_dot with at most one dot (ndots <= 1).  One wave per
+// slice of 64 rows, one row per lane: each entry of the slice is one
+// coalesced 512-byte value load and 256-byte column load for the wave, with
+// no LDS and no barrier; the x gathers of neighbouring rows fall on the same
+// lines.  A row is summed entry by entry in CSR order from 0.0 with separately
+// rounded products, the additions of a sequential CSR loop, so w has the same
+// bits as k_spmv_dot's.  The waves stride over the slices together: wave v of
+// the grid (XCD-aware block order) takes slices v, v + waves, ..., so the
+// waves in flight read one contiguous stretch of the matrix and each XCD a
+// contiguous part of it (measured 215 us against 270-280 us when each
+// workgroup walks its own contiguous slice range, on the 4096^2 Laplacian).
+// Per-thread partials, then the block partials in block order as in
+// k_spmv_dot.
+constexpr int kSellU = 8;  // entries of a slice whose loads are issued together
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(ArnoldiBufs b, const double* x_in,
+                                                      const double* xs_ptr, double* wout, int ndots,
+                                                      int normq, int check_breakdown, int i0,
+                                                      const double* shift_ptr) {
+#pragma clang fp contract(off)
+  if (check_breakdown && (b.iscal[kBreakdown] || (check_breakdown == 2 && b.iscal[kSStepFail]))) return;
+  __shared__ double red[kWavesPerBlock * 2];
+  const bool has_shift = shift_ptr != nullptr;
+  const double sh = has_shift ? *shift_ptr : 0.0;
+  const int lb = xcd_block_id();
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const double* __restrict__ x = x_in;
+  const double xs = xs_ptr ? *xs_ptr : 1.0;
+  const double cs0 = ndots ? b.colscale[i0] : 0.0;
+  const double* __restrict__ Vd = b.V + static_cast<long long>(i0) * b.ld;
+  const long long* __restrict__ sp = b.sell_ptr;
+  const int* __restrict__ sc = b.sell_col;
+  double* __restrict__ w = wout;
+  const long long ns = b.n_slices;
+  const long long nw = static_cast<long long>(gridDim.x) * kWavesPerBlock;
+  double acc[2] = {0.0, 0.0};  // q_i0 . w, ||w||^2
+  for (long long s = static_cast<long long>(lb) * kWavesPerBlock + wave; s < ns; s += nw) {
+    const long long off = sp[s];
+    const int W = static_cast<int>((sp[s + 1] - off) / kWave);
+    const long long r = s * kWave + lane;
+    double xrr = 0.0;
+    if (has_shift && r < b.n) xrr = x[r];
+    double a = 0.0;
+    for (int k0 = 0; k0 < W; k0 += kSellU) {
+      int c[kSellU];
+      double v[kSellU];
+#pragma unroll
+      for (int u = 0; u < kSellU; ++u) {
+        const long long e = off + static_cast<long long>(k0 + u) * kWave + lane;
+        c[u] = -1;
+        v[u] = 0.0;
+        if (k0 + u < W) {
+          c[u] = load_idx_nt(sc, e);
+          v[u] = load_val_nt<VT>(b.sell_val, e);
+        }
+      }
+      double p[kSellU];
+#pragma unroll
+      for (int u = 0; u < kSellU; ++u) p[u] = (c[u] >= 0) ? v[u] * (xs * x[c[u]]) : 0.0;
+#pragma unroll
+      for (int u = 0; u < kSellU; ++u)
+        if (c[u] >= 0) a += p[u];
+    }
+    if (r < b.n) {
+      if (has_shift) a = shift_row(a, sh, xs, xrr);
+      __builtin_nontemporal_store(a, w + r);
+      acc[1] += a * a;
+      if (ndots) acc[0] += (cs0 * Vd[r]) * a;
+    }
+  }
+  block_partials<2>(acc, ndots, red, b.partials, gridDim.x, lb);
+  __syncthreads();
+  double nv[1] = {acc[1]};
+  block_partials<1>(nv, 1, red, b.partials + static_cast<long long>(normq) * gridDim.x, gridDim.x, lb);
+}
+
+// The sliced-ELL copy of a device CSR matrix: one thread per lane of every
+// slice (rows past n included) writes its row's entries at their slice
+// positions and pads the row to the slice width with column -1 and value 0.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_sell_build(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                       const VT* __restrict__ va, int n, long long nslices,
+                                                       const long long* __restrict__ sp, int* __restrict__ sc,
+                                                       VT* __restrict__ sv) {
+  for (long long r = gtid(); r < nslices * kWave; r += gstride()) {
+    const long long s = r / kWave;
+    const int lane = static_cast<int>(r % kWave);
+    const long long off = sp[s];
+    const int W = static_cast<int>((sp[s + 1] - off) / kWave);
+    const int k0 = (r < n) ? rp[r] : 0;
+    const int len = (r < n) ? rp[r + 1] - k0 : 0;
+    for (int e = 0; e < W; ++e) {
+      const long long d = off + static_cast<long long>(e) * kWave + lane;
+      if (e < len) {
+        sc[d] = ci[k0 + e];
+        sv[d] = va[k0 + e];
+      } else {
+        sc[d] = -1;
+        sv[d] = VT(0);
+      }
+    }
+  }
 }
 
 // K1 and K1b in one pass for column j (nq = j + 1 <= 64): w = A q_j as in
@@ -701,12 +819,36 @@ template <int MAXJ>
 static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const double* xs, double* w,
                               int ndots, int normq, int chk, hipStream_t s, int i0 = 0,
                               const double* shift = nullptr) {
+  if (b.sell_ptr && ndots <= 1) {
+    if (b.vals_f32)
+      hipLaunchKernelGGL(k_spmv_sell<float>, dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, ndots, normq,
+                         chk, i0, shift);
+    else
+      hipLaunchKernelGGL(k_spmv_sell<double>, dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, ndots,
+                         normq, chk, i0, shift);
+    return;
+  }
   if (b.vals_f32)
     hipLaunchKernelGGL((k_spmv_dot<float, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
                        ndots, normq, chk, i0, shift);
   else
     hipLaunchKernelGGL((k_spmv_dot<double, MAXJ>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w,
                        ndots, normq, chk, i0, shift);
+}
+
+void launch_sell_build(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32, int n,
+                       const long long* sell_ptr, int* sell_col, void* sell_val, hipStream_t s) {
+  if (n <= 0) return;
+  const long long nslices = (static_cast<long long>(n) + kWave - 1) / kWave;
+  const int grid = static_cast<int>(std::min<long long>((nslices * kWave + kBlock - 1) / kBlock, 16384));
+  if (vals_f32)
+    hipLaunchKernelGGL(k_sell_build<float>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx,
+                       static_cast<const float*>(vals), n, nslices, sell_ptr, sell_col,
+                       static_cast<float*>(sell_val));
+  else
+    hipLaunchKernelGGL(k_sell_build<double>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx,
+                       static_cast<const double*>(vals), n, nslices, sell_ptr, sell_col,
+                       static_cast<double*>(sell_val));
 }
 
 // s-step block vector i (columns k + i from k - 1 + i): y = (A - theta_i) p

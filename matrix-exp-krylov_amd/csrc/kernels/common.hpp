@@ -103,6 +103,15 @@ struct ArnoldiBufs {
   double* partials;      // per-block partial sums: quantity q of block g at q*grid + g
   int grid;              // blocks of the streaming kernels
   int spmv_grid;         // blocks of the SpMV kernel
+  // Sliced-ELL copy of the matrix (null when the SpMV uses the CSR row
+  // blocks): slices of 64 consecutive rows, one row per lane; slice s holds
+  // W_s = (sell_ptr[s+1] - sell_ptr[s]) / 64 entries per row, stored entry
+  // by entry (entry e of rows 0..63, then entry e+1, ...), each row's
+  // entries in CSR order; a row shorter than W_s is padded with column -1.
+  const long long* sell_ptr;  // n_slices + 1 entry offsets
+  const int* sell_col;
+  const void* sell_val;       // double, or float when vals_f32
+  int n_slices;
 };
 
 enum ScalarSlot : int {

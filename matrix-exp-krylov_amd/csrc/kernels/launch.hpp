@@ -42,6 +42,12 @@ void launch_lanczos_update(const ArnoldiBufs& b, int j, hipStream_t s);
 void launch_lanczos_fin(const ArnoldiBufs& b, int j, hipStream_t s);
 void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, double* y,
                        hipStream_t s);
+// Sliced-ELL copy (ArnoldiBufs::sell_*) of a device CSR matrix whose slice
+// offsets sell_ptr (n_slices + 1, from sell_layout) are on the device.  With
+// sell_ptr set in the buffers, every SpMV launcher above except the fused
+// spmv_dots uses it.
+void launch_sell_build(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32, int n,
+                       const long long* sell_ptr, int* sell_col, void* sell_val, hipStream_t s);
 
 // ------------------------------------------------------------ s-step blocks
 //

@@ -68,6 +68,51 @@ def test_spmv_f32_values_exact_stencil(gpu, oracle):
     np.testing.assert_array_equal(y, oracle.spmv(rp, ci, va, x))
 
 
+def _ragged_matrix(n=1000, seed=5):
+    """Rows of 0..9 nonzeros (a run of empty rows included), unsorted
+    columns within each row, n not a multiple of 64."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 10, n)
+    lens[100:170] = 0
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    ci = np.concatenate([rng.choice(n, int(k), replace=False) for k in lens]).astype(np.int32)
+    va = rng.uniform(-1, 1, int(rp[-1]))
+    return rp, ci, va
+
+
+@pytest.mark.parametrize("which", ["laplacian", "convdiff", "convdiff_f32", "ragged", "random", "tiny"])
+def test_spmv_sell_bitexact(gpu, oracle, which):
+    """The sliced-ELL copy gives the oracle's CSR-loop bits (and so the CSR
+    row blocks' bits), including padded rows, empty rows, unsorted columns,
+    FP32 values and a last slice of fewer than 64 rows."""
+    print(f"[spmv-sell] {which}", flush=True)
+    f32 = which == "convdiff_f32"
+    if which == "laplacian":
+        rp, ci, va = gen.laplacian_2d(77)
+    elif which.startswith("convdiff"):
+        rp, ci, va = gen.convdiff_3d(19)
+    elif which == "ragged":
+        rp, ci, va = _ragged_matrix()
+    elif which == "random":
+        rp, ci, va = gen.random_rows(3001, 5, 30, seed=2)
+    else:
+        rp, ci, va = np.array([0, 1, 1, 3], np.int32), np.array([2, 0, 2], np.int32), np.array([2.0, -1.0, 0.5])
+    x = np.random.default_rng(8).standard_normal(rp.size - 1)
+    plan = mek.SpmvPlan(rp, ci, va, values_f32=f32)
+    y_csr = plan(x)
+    entries, width = plan.use_sell()
+    y = plan(x)
+    n = rp.size - 1
+    lens = np.diff(rp)
+    slices = (n + 63) // 64
+    assert width == lens.max()
+    assert entries == sum(64 * int(lens[64 * s:64 * (s + 1)].max()) for s in range(slices))
+    np.testing.assert_array_equal(y, oracle.spmv(rp, ci, va, x))
+    np.testing.assert_array_equal(y, y_csr)
+    plan.close()
+
+
 @pytest.mark.parametrize("k", [1, 2, 5, 17, 40, 58, 62, 63, 80])
 def test_expm_dense(gpu, oracle, k):
     rng = np.random.default_rng(100 + k)

@@ -435,3 +435,34 @@ def test_odd_row_count(gpu, oracle, gs_mode):
     R.close()
     yref, st, _log = oracle.expmv(rp, ci, va, v, 0.5, 1e-10, 20, 10)
     assert rel_err(y, yref) <= 1e-8
+
+
+@pytest.mark.parametrize("sstep", [0, 12])
+def test_spmv_formats_agree(gpu, sstep):
+    """The automatic SpMV layout is the sliced-ELL copy for a stencil and the
+    CSR row blocks for rows of 5-30 random nonzeros (slices padded to their
+    longest row would carry ~70 % padding); forcing either layout gives the
+    same solve to rounding, within 1e-8 of the DST-I closed form."""
+    N, t = 80, 10.0
+    rp, ci, va = gen.laplacian_2d(N)
+    v = gen.normal_vector(N * N, seed=4)
+    kw = dict(m=40, t=t, tol=1e-10, max_restarts=0, step_control="expokit", gs_mode=2, sstep=sstep)
+    ys = {}
+    for fmt in ("auto", "csr", "sell"):
+        R = _runner(rp, ci, va, spmv_format=fmt, **kw)
+        info = R.spmv_format()
+        assert info["format"] == ("csr" if fmt == "csr" else "sell"), (fmt, info)
+        ys[fmt] = R.expmv(v)
+        R.close()
+    lens = np.diff(rp)
+    assert info["width"] == 5
+    assert info["entries"] == sum(64 * int(lens[64 * q:64 * (q + 1)].max()) for q in range((N * N + 63) // 64))
+    ref = laplacian_2d_expmv(N, t, v)
+    for fmt, y in ys.items():
+        assert rel_err(y, ref) <= TOL, fmt
+    assert rel_err(ys["sell"], ys["csr"]) <= 1e-12
+    np.testing.assert_array_equal(ys["auto"], ys["sell"])
+    rp, ci, va = gen.random_rows(4000, 5, 30, seed=3)
+    R = _runner(rp, ci, va, m=20, t=0.5, tol=1e-10, max_restarts=0, step_control="expokit")
+    assert R.spmv_format()["format"] == "csr"
+    R.close()
