@@ -623,16 +623,31 @@ def run_single(args, mek, gen):
     rel = float(np.linalg.norm((y - ref) / s) / np.linalg.norm(ref / s))
     log(f"relative error {rel:.3e}")
 
+    # The SpMV's bytes are those of the layout that runs (the runner's count:
+    # the diagonal copy reads n values per diagonal and no column index; the
+    # sliced-ELL copy its padded entries), the least the kernel must move; the
+    # CSR figure nnz (value + 4) + 4 (n + 1) + 8 n (x once) + 8 n (w) is kept
+    # beside it with the bandwidth it would imply.
+    f32_used = bool(w["f32"] and f32_values["values_f32_used"])
+    csr_bytes = nnz * ((4 if f32_used else 8) + 4) + 4.0 * (n + 1) + 16.0 * n
+
     def roof(k):
         kt = kern[k]
         if kt["launches"] == 0 or kt["ms"] <= 0:
             return None
-        achieved = kt["bytes"] / (kt["ms"] * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4),
-                "traffic": None, "kernel": k,
-                "bytes_per_launch": kt["bytes"] / kt["launches"],
-                "avg_launch_us": kt["ms"] * 1e3 / kt["launches"]}
+        per_launch = kt["bytes"] / kt["launches"]
+        avg_us = kt["ms"] * 1e3 / kt["launches"]
+        achieved = per_launch / (avg_us * 1e-6) / 1e9
+        r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+             "frac": round(achieved / PEAK_HBM_GBS, 4),
+             "traffic": None, "kernel": k,
+             "bytes_per_launch": per_launch,
+             "avg_launch_us": avg_us}
+        if k == "spmv":
+            r["layout"] = layout
+            r["csr_bytes_per_launch"] = csr_bytes
+            r["csr_equivalent_gbs"] = round(csr_bytes / (avg_us * 1e-6) / 1e9, 1)
+        return r
 
     streaming = [k for k in ("spmv", "update_dot", "update_norm", "dots", "lift") if kern[k]["launches"]]
     dominant = max(streaming, key=lambda k: kern[k]["ms"])
@@ -645,12 +660,6 @@ def run_single(args, mek, gen):
         roofline["traffic"] = tr[dominant]
     if spmv_roof is not None and "spmv" in tr:
         spmv_roof["traffic"] = tr["spmv"]
-    if spmv_roof is not None:
-        # the same launches priced at the CSR matrix's bytes (nnz (value + 4) + 4 (n + 1) + 16 n)
-        csr_bytes = nnz * ((4 if (w["f32"] and f32_values["values_f32_used"]) else 8) + 4) + 4.0 * (n + 1) + 16.0 * n
-        spmv_roof["layout"] = layout
-        spmv_roof["csr_equivalent_gbs"] = round(csr_bytes / (spmv_roof["avg_launch_us"] * 1e-6) / 1e9, 1)
-
     solves_per_s = 1000.0 / ms_per_step
     out = {
         "metric": METRIC,
@@ -715,8 +724,8 @@ def main():
     ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
     ap.add_argument("--sstep", type=int, default=12,
                     help="s-step blocks of this many basis vectors (0: column by column, as the reference)")
-    ap.add_argument("--spmv-format", default="auto", choices=["auto", "csr", "sell"],
-                    help="SpMV layout: sliced ELL where its padding is small (auto), CSR row blocks, or sliced ELL")
+    ap.add_argument("--spmv-format", default="auto", choices=["auto", "csr", "sell", "dia"],
+                    help="SpMV layout: the diagonal copy for stencils, else sliced ELL where its padding is small (auto); CSR row blocks; sliced ELL; diagonal copy")
     ap.add_argument("--sstep-first", type=int, default=6,
                     help="columns of each cycle built one by one before the blocks (-1: --sstep)")
     args = ap.parse_args()

@@ -102,8 +102,10 @@ typedef struct {
                            0: column by column); needs m + 1 <= 64        */
     int sstep_first;    /* columns of each cycle built one by one before the
                            blocks (-1: sstep)                               */
-    int spmv_format;    /* 0 sliced ELL when its padding is small, else CSR
-                           row blocks; 1 CSR row blocks; 2 sliced ELL       */
+    int spmv_format;    /* 0 diagonal copy for entries on <= 16 diagonals
+                           with little padding, else sliced ELL when its
+                           padding is small, else CSR row blocks; 1 CSR row
+                           blocks; 2 sliced ELL; 3 diagonal copy            */
 } mek_solver_options;
 
 void mek_solver_options_default(mek_solver_options* opt);
@@ -164,8 +166,9 @@ int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sw
 /* Workgroups of the s-step block sweeps (0 when the block path is off) and,
  * with autotune, the timed dots + update pair of the chosen grid (us). */
 int mek_runner_sstep_grid(const mek_runner* R, int* grid, double* us);
-/* SpMV layout in use: *format 1 CSR row blocks, 2 sliced ELL; for the
- * latter the padded entry count and the widest row (else 0). */
+/* SpMV layout in use: *format 1 CSR row blocks, 2 sliced ELL, 3 diagonal
+ * copy; for sliced ELL the padded entry count and the widest row, for the
+ * diagonal copy rows x diagonals and the diagonal count (else 0). */
 int mek_runner_spmv_format(const mek_runner* R, int* format, long long* entries, int* width);
 /* The runner's stream (hipStream_t). */
 int mek_runner_stream(const mek_runner* R, void** stream);
@@ -255,6 +258,16 @@ int mek_spmv_execute(mek_spmv_plan* P, const double* d_x, double* d_y, void* str
  * count and the widest row.  No reference counterpart (cuSPARSE chooses its
  * own algorithm). */
 int mek_spmv_plan_use_sell(mek_spmv_plan* P, const int* h_row_ptr, long long* entries, int* width);
+/* Switch the plan to a diagonal copy of its matrix (diagonal d holds the
+ * entries (r, r + offset_d); no column indices are read), built on the device
+ * from the plan's CSR arrays.  The matrix must have at most 16 distinct
+ * column - row offsets, strictly ascending columns within each row and no NaN
+ * value (else MEK_E_INVALID); h_row_ptr, h_col_idx, h_vals as at create.  The
+ * same y bits as the CSR row blocks.  *n_diags (may be null) receives the
+ * diagonal count.  No reference counterpart (cuSPARSE chooses its own
+ * algorithm). */
+int mek_spmv_plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx, const double* h_vals,
+                          int* n_diags);
 void mek_spmv_plan_destroy(mek_spmv_plan* P);
 
 /* d_E = exp(d_A) for a dense k x k column-major matrix on the device (one

@@ -86,9 +86,10 @@ struct SolverOptions {
   // rest of that cycle.
   int sstep = 0;
   int sstep_first = -1;
-  // SpMV layout: 0 the sliced-ELL copy when its padding is small (stencils,
-  // banded matrices), else the CSR row blocks; 1 CSR row blocks; 2 sliced
-  // ELL.  Both give the same bits.
+  // SpMV layout: 0 the diagonal copy when the entries lie on at most 16
+  // diagonals with little padding (stencils), else the sliced-ELL copy when
+  // its padding is small (banded matrices), else the CSR row blocks; 1 CSR
+  // row blocks; 2 sliced ELL; 3 diagonal.  All give the same bits.
   int spmv_format = 0;
 };
 

@@ -53,6 +53,13 @@ struct DeviceContext {
   long long sell_entries = 0;       // entries with padding
   int sell_width = 0;               // widest slice (entries per row)
 
+  // diagonal copy of the matrix (build_dia): when set, the SpMV reads it
+  // instead of the sliced-ELL copy or the CSR row blocks
+  void* d_dia_val = nullptr;        // n_diags x dia_ld, diagonal-major; absent entries hold the marker
+  int* d_dia_off = nullptr;         // column - row of each diagonal, ascending
+  int n_diags = 0;
+  long long dia_ld = 0;             // rows rounded up to 64
+
   // s-step blocks (allocated by alloc_sstep): state, flags, sweep partials
   double* d_ss_state = nullptr;
   int* d_ss_flags = nullptr;
@@ -82,6 +89,18 @@ struct DeviceContext {
   // 2: always.  Returns whether the SpMV uses it.
   bool build_sell(const CSRHost& part, int format);
   void free_sell();
+  // Additive: the diagonal copy of the device CSR (after alloc_from), for
+  // matrices whose entries lie on at most mek::kDiaMaxDiags diagonals and
+  // whose rows list their columns in ascending order (mek::dia_offsets).
+  // Returns whether it was built.
+  bool build_dia(const CSRHost& part, const std::vector<int>& offsets);
+  void free_dia();
+  // Additive: the SpMV layout.  format 0: the diagonal copy when the matrix
+  // has one with at most mek::kDiaMaxPad padding, else the sliced-ELL copy
+  // under build_sell's rule, else the CSR row blocks; 1 CSR row blocks; 2
+  // sliced ELL; 3 diagonal (an error when the matrix has no diagonal form).
+  // Returns the layout in use (1, 2 or 3).
+  int build_spmv_layout(const CSRHost& part, int format);
   // Additive: buffers of the s-step block path (sstep.hip), shifts uploaded.
   void alloc_sstep(const std::vector<double>& shifts);
   mek::SsArgs ss_args() const;
@@ -103,4 +122,13 @@ std::vector<int> build_row_blocks(const int* row_ptr, int n, int nnz_cap, int ro
 long long sell_layout(const int* row_ptr, int n, std::vector<long long>& ptr, int* max_width);
 constexpr double kSellMaxPad = 0.15;  // automatic choice: padding <= 15 % of nnz
 constexpr int kSellMaxWidth = 256;    // ... and no row longer than this
+
+// Diagonal layout: the distinct column - row offsets of a CSR matrix, in
+// ascending order, when there are at most kDiaMaxDiags of them, every row
+// lists its columns in strictly ascending order (so summing a row diagonal
+// by diagonal adds its products in CSR order) and no value is NaN (a NaN
+// marks the absent entries of the copy); empty otherwise.
+constexpr int kDiaMaxDiags = 16;
+constexpr double kDiaMaxPad = 0.2;    // automatic choice: n * diagonals <= 1.2 nnz
+std::vector<int> dia_offsets(const int* row_ptr, const int* col_idx, const double* values, int n);
 }  // namespace mek

@@ -26,7 +26,8 @@ KERNEL_KINDS = {"spmv": 0, "update_dot": 1, "update_norm": 2, "dots": 3, "final"
 
 
 STEP_CONTROL = {"reference": 0, "expokit": 1, "expokit_rel": 2}
-SPMV_FORMAT = {"auto": 0, "csr": 1, "sell": 2}
+SPMV_FORMAT = {"auto": 0, "csr": 1, "sell": 2, "dia": 3}
+SPMV_FORMAT_NAME = {1: "csr", 2: "sell", 3: "dia"}
 
 
 class MekError(RuntimeError):
@@ -136,6 +137,7 @@ SIGNATURES = {
     "mek_spmv_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mek_spmv_plan_destroy": (None, [_vp]),
     "mek_spmv_plan_use_sell": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
+    "mek_spmv_plan_use_dia": (ctypes.c_int, [_vp, _ip, _ip, _dp, _ip]),
     "mek_runner_spmv_format": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
     "mek_expm_dense_noalloc": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp]),
@@ -400,12 +402,13 @@ class Runner:
                 "spmv_us": a.value, "sweep_us": b.value, "sstep_grid": sg.value, "sstep_us": su.value}
 
     def spmv_format(self):
-        """The SpMV layout in use: {"format": "csr" | "sell", "entries": padded
-        entries (sliced ELL), "width": widest row (sliced ELL)}."""
+        """The SpMV layout in use: {"format": "csr" | "sell" | "dia", "entries":
+        padded entries (sliced ELL) or rows x diagonals (diagonal copy),
+        "width": widest row (sliced ELL) or diagonal count}."""
         f, w = ctypes.c_int(), ctypes.c_int()
         e = ctypes.c_longlong()
         _check(lib().mek_runner_spmv_format(self.h, ctypes.byref(f), ctypes.byref(e), ctypes.byref(w)))
-        return {"format": "sell" if f.value == 2 else "csr", "entries": e.value, "width": w.value}
+        return {"format": SPMV_FORMAT_NAME[f.value], "entries": e.value, "width": w.value}
 
     def arnoldi(self, v):
         v = _f64(v)
@@ -595,6 +598,15 @@ class SpmvPlan:
                                           1 if values_f32 else 0, _pi(rp), ctypes.byref(h)))
         self.h = h
         self._rp = rp
+        self._ci = _i32(col_idx)
+        self._va = _f64(values)
+
+    def use_dia(self):
+        """Switch to the diagonal copy of the matrix; returns the diagonal
+        count (MekError when the matrix has no diagonal form)."""
+        nd = ctypes.c_int()
+        _check(lib().mek_spmv_plan_use_dia(self.h, _pi(self._rp), _pi(self._ci), _pd(self._va), ctypes.byref(nd)))
+        return nd.value
 
     def use_sell(self):
         """Switch to the sliced-ELL copy of the matrix; returns (padded

@@ -48,6 +48,8 @@ struct mek_spmv_plan {
   long long* d_sell_ptr = nullptr;
   int* d_sell_col = nullptr;
   void* d_sell_val = nullptr;
+  void* d_dia_val = nullptr;
+  int* d_dia_off = nullptr;
 };
 
 namespace {
@@ -234,7 +236,7 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       need(o->sstep_first >= -1, "sstep_first >= -1");
       so.sstep = o->sstep;
       so.sstep_first = o->sstep_first;
-      need(o->spmv_format >= 0 && o->spmv_format <= 2, "spmv_format in {0, 1, 2}");
+      need(o->spmv_format >= 0 && o->spmv_format <= 3, "spmv_format in {0, 1, 2, 3}");
       so.spmv_format = o->spmv_format;
     }
     try {
@@ -344,6 +346,12 @@ int mek_runner_spmv_format(const mek_runner* R, int* format, long long* entries,
   return guarded([&] {
     need(R && R->R && format, "runner, format");
     const DeviceContext& dc = R->R->device_contexts[0];
+    if (dc.n_diags > 0) {
+      *format = 3;
+      if (entries) *entries = static_cast<long long>(dc.n_diags) * dc.local_rows;
+      if (width) *width = dc.n_diags;
+      return;
+    }
     *format = dc.n_slices > 0 ? 2 : 1;
     if (entries) *entries = dc.sell_entries;
     if (width) *width = dc.sell_width;
@@ -631,8 +639,47 @@ int mek_spmv_plan_use_sell(mek_spmv_plan* P, const int* h_row_ptr, long long* en
   });
 }
 
+int mek_spmv_plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx, const double* h_vals,
+                          int* n_diags) {
+  return guarded([&] {
+    need(P && h_row_ptr && (P->b.n == 0 || (h_col_idx && h_vals)), "plan, h_row_ptr, h_col_idx, h_vals");
+    mek::ArnoldiBufs& b = P->b;
+    need(P->d_dia_val == nullptr, "the plan already uses the diagonal copy");
+    const std::vector<int> offs = mek::dia_offsets(h_row_ptr, h_col_idx, h_vals, b.n);
+    if (offs.empty() || b.n <= 0)
+      throw InvalidArgument("the matrix has no diagonal form (more than 16 diagonals, columns not ascending "
+                                 "within a row, or NaN values)");
+    const int nd = static_cast<int>(offs.size());
+    const long long ldd = (static_cast<long long>(b.n) + mek::kWave - 1) / mek::kWave * mek::kWave;
+    const long long slices = ldd / mek::kWave;
+    const int grid = static_cast<int>(std::max<long long>(1, std::min<long long>(mek::kMaxGrid, (slices + 15) / 16)));
+    if (grid > b.spmv_grid) {  // partials for the larger of the two grids
+      double* d_part = nullptr;
+      MEK_HIP(hipMalloc(&d_part, sizeof(double) * grid));
+      if (P->d_partials) (void)hipFree(P->d_partials);
+      P->d_partials = d_part;
+      b.partials = d_part;
+    }
+    MEK_HIP(hipMalloc(&P->d_dia_val, (b.vals_f32 ? sizeof(float) : sizeof(double)) * static_cast<size_t>(ldd) * nd));
+    MEK_HIP(hipMalloc(&P->d_dia_off, sizeof(int) * nd));
+    MEK_HIP(hipMemcpy(P->d_dia_off, offs.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
+    mek::launch_dia_build(b.row_ptr, b.col_idx, b.vals, b.vals_f32, b.n, P->d_dia_off, nd, ldd, P->d_dia_val,
+                          nullptr);
+    MEK_HIP_LAUNCHED();
+    MEK_HIP(hipDeviceSynchronize());
+    b.dia_val = P->d_dia_val;
+    b.dia_off = P->d_dia_off;
+    b.n_diags = nd;
+    b.dia_ld = ldd;
+    b.spmv_grid = grid;
+    if (n_diags) *n_diags = nd;
+  });
+}
+
 void mek_spmv_plan_destroy(mek_spmv_plan* P) {
   if (!P) return;
+  if (P->d_dia_val) (void)hipFree(P->d_dia_val);
+  if (P->d_dia_off) (void)hipFree(P->d_dia_off);
   if (P->d_sell_ptr) (void)hipFree(P->d_sell_ptr);
   if (P->d_sell_col) (void)hipFree(P->d_sell_col);
   if (P->d_sell_val) (void)hipFree(P->d_sell_val);

@@ -109,10 +109,12 @@ std::vector<double> sstep_shifts(const CSRHost& A, int s) {
 
 // Bytes one SpMV streams: the matrix in its layout (CSR: values, column
 // indices, row pointers; sliced ELL: padded values and column indices, slice
-// offsets), x once and w.  csr = true: the CSR figure whatever the layout.
+// offsets; diagonal: n values per diagonal and the offsets), x once and w.  csr = true: the CSR figure whatever the layout.
 double spmv_bytes(const DeviceContext& dc, bool csr = false) {
   const double vb = dc.values_f32 ? 4.0 : 8.0;
   const double n = dc.local_rows;
+  if (dc.n_diags > 0 && !csr)
+    return static_cast<double>(dc.n_diags) * n * vb + 4.0 * dc.n_diags + 16.0 * n;
   if (dc.n_slices > 0 && !csr)
     return static_cast<double>(dc.sell_entries) * (vb + 4.0) + 8.0 * (dc.n_slices + 1.0) + 16.0 * n;
   return static_cast<double>(dc.local_nnz) * (vb + 4.0) + 4.0 * (n + 1.0) + 16.0 * n;
@@ -194,9 +196,7 @@ void ArnoldiRunner::init_one(const CSRHost& part, int device_id) {
   dc.alloc_from(part, global_cols, alloc_m, values_f32_used);
   static const CSRHost::GhostMap kEmpty;
   dc.create_descriptors(part, global_cols, ghost_maps.empty() ? kEmpty : ghost_maps[0], 0, alloc_m);
-  if (options.spmv_format < 0 || options.spmv_format > 2)
-    throw mek::Error("ERROR: spmv_format must be 0, 1 or 2");
-  dc.build_sell(part, options.spmv_format);
+  dc.build_spmv_layout(part, options.spmv_format);
   if (options.autotune) tuning = mek::autotune_single(dc, part, alloc_m);
   anorm_inf = part.norm_inf();
   if (options.sstep < 0 || options.sstep == 1 || options.sstep > mek::kSsMaxS)

@@ -51,7 +51,7 @@ int time_candidates(const std::vector<int>& candidates, const std::function<void
 
 namespace {
 
-using Key = std::tuple<int, int, long long, int, int, int>;  // device, rows, nnz, m, values_f32, sliced ELL
+using Key = std::tuple<int, int, long long, int, int, int>;  // device, rows, nnz, m, values_f32, layout
 std::mutex g_mu;
 std::map<Key, TuneChoice> g_cache;
 
@@ -66,8 +66,10 @@ void upload_blocks(DeviceContext& dc, const std::vector<int>& blocks) {
 }  // namespace
 
 TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
-  const bool sell = dc.n_slices > 0;
-  const Key key{dc.device_id, A.rows, A.nnz, m, dc.values_f32 ? 1 : 0, sell ? 1 : 0};
+  // sliced-ELL and diagonal copies: the SpMV grid only (no row blocks)
+  const int layout = dc.n_diags > 0 ? 3 : dc.n_slices > 0 ? 2 : 1;
+  const bool copy = layout != 1;
+  const Key key{dc.device_id, A.rows, A.nnz, m, dc.values_f32 ? 1 : 0, layout};
   std::ostringstream log;
   TuneChoice best;
   {
@@ -78,11 +80,12 @@ TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
   const bool hit = best.nnz_cap > 0;
   hipStream_t s = dc.stream_compute;
   MEK_HIP(hipSetDevice(dc.device_id));
-  if (!hit && sell) {
-    // sliced-ELL SpMV: the grid only (the row blocks are not used)
+  if (!hit && copy) {
+    // sliced-ELL or diagonal SpMV: the grid only (the row blocks are not used)
     std::vector<int> grids;
+    const long long slices = (static_cast<long long>(dc.local_rows) + kWave - 1) / kWave;
     for (int g : {1024, 2048, kMaxGrid})
-      if (g <= std::max(8, (dc.n_slices / 4 + 7) / 8 * 8)) grids.push_back(g);
+      if (g <= std::max<long long>(8, (slices / 4 + 7) / 8 * 8)) grids.push_back(g);
     if (grids.empty()) grids.push_back(dc.spmv_grid);
     ArnoldiBufs b = dc.bufs();
     std::vector<double> us;
@@ -93,7 +96,8 @@ TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
           launch_spmv_plain(b, b.V, nullptr, b.w, s);
         },
         s, 5, &us);
-    for (size_t c = 0; c < grids.size(); ++c) log << "sell spmv grid=" << grids[c] << " " << us[c] << "us; ";
+    for (size_t c = 0; c < grids.size(); ++c)
+      log << (layout == 3 ? "dia" : "sell") << " spmv grid=" << grids[c] << " " << us[c] << "us; ";
     best.nnz_cap = kSpmvNnzCap;
     best.spmv_grid = grids[i];
     best.spmv_us = us[i];
@@ -103,7 +107,7 @@ TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
     // most the partial-sum capacity)
     const int caps[] = {512, 1024, kSpmvNnzCap};
     double best_spmv = 1e300;
-    for (int cap : (sell ? std::vector<int>{} : std::vector<int>(std::begin(caps), std::end(caps)))) {
+    for (int cap : (copy ? std::vector<int>{} : std::vector<int>(std::begin(caps), std::end(caps)))) {
       const std::vector<int> blocks = build_row_blocks(A.row_ptr.data(), A.rows, cap, kSpmvRowCap);
       upload_blocks(dc, blocks);
       std::vector<int> grids;
@@ -126,7 +130,7 @@ TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
         best.spmv_grid = grids[i];
       }
     }
-    if (!sell) best.spmv_us = best_spmv;
+    if (!copy) best.spmv_us = best_spmv;
     // sweeps: the update sweep at a mid-size basis
     {
       ArnoldiBufs b = dc.bufs();
@@ -152,7 +156,7 @@ TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
   }
   // apply
   upload_blocks(dc, build_row_blocks(A.row_ptr.data(), A.rows, best.nnz_cap, kSpmvRowCap));
-  dc.spmv_grid = sell ? best.spmv_grid : std::min(best.spmv_grid, std::max(8, (dc.n_row_blocks + 7) / 8 * 8));
+  dc.spmv_grid = copy ? best.spmv_grid : std::min(best.spmv_grid, std::max(8, (dc.n_row_blocks + 7) / 8 * 8));
   dc.grid = best.sweep_grid;
   // the timing runs wrote into the basis and the status block: start clean
   MEK_HIP(hipMemsetAsync(dc.d_small, 0, sizeof(double) * dc.small_doubles(), s));

@@ -2,6 +2,7 @@
 #include "mek/device_context.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <iostream>
 
@@ -43,6 +44,27 @@ long long sell_layout(const int* row_ptr, int n, std::vector<long long>& ptr, in
   }
   if (max_width) *max_width = wmax;
   return ptr[ns];
+}
+
+std::vector<int> dia_offsets(const int* row_ptr, const int* col_idx, const double* values, int n) {
+  static_assert(kDiaMaxDiags == kDiaDiagCap, "the diagonal-copy SpMV's capacity");
+  std::vector<int> offs;
+  for (int r = 0; r < n; ++r) {
+    const int e0 = row_ptr[r], e1 = row_ptr[r + 1];
+    for (int e = e0; e < e1; ++e) {
+      if (e > e0 && col_idx[e] <= col_idx[e - 1]) return {};
+      if (std::isnan(values[e])) return {};
+      const long long o = static_cast<long long>(col_idx[e]) - r;
+      // the k-th entry of a row mostly has the k-th offset of the rows before
+      const size_t guess = static_cast<size_t>(e - e0);
+      if (guess < offs.size() && offs[guess] == o) continue;
+      if (std::find(offs.begin(), offs.end(), o) != offs.end()) continue;
+      if (offs.size() == static_cast<size_t>(kDiaMaxDiags)) return {};
+      offs.push_back(static_cast<int>(o));
+    }
+  }
+  std::sort(offs.begin(), offs.end());
+  return offs;
 }
 
 // Blocks of the streaming kernels: up to 4096 (16 per CU, measured to need
@@ -164,6 +186,63 @@ bool DeviceContext::build_sell(const CSRHost& part, int format) {
   return true;
 }
 
+bool DeviceContext::build_dia(const CSRHost& part, const std::vector<int>& offsets) {
+  MEK_HIP(hipSetDevice(device_id));
+  free_dia();
+  if (offsets.empty() || local_rows <= 0) return false;
+  const int nd = static_cast<int>(offsets.size());
+  if (nd > mek::kDiaMaxDiags) throw mek::Error("ERROR: more than 16 diagonals");
+  const long long ldd = (static_cast<long long>(local_rows) + mek::kWave - 1) / mek::kWave * mek::kWave;
+  const size_t vb = values_f32 ? sizeof(float) : sizeof(double);
+  MEK_HIP(hipMalloc(&d_dia_val, vb * static_cast<size_t>(ldd) * nd));
+  MEK_HIP(hipMalloc(&d_dia_off, sizeof(int) * nd));
+  MEK_HIP(hipMemcpy(d_dia_off, offsets.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
+  (void)part;
+  mek::launch_dia_build(d_row_ptr, d_col_idx, d_values, values_f32 ? 1 : 0, local_rows, d_dia_off, nd, ldd,
+                        d_dia_val, stream_compute);
+  MEK_HIP_LAUNCHED();
+  MEK_HIP(hipStreamSynchronize(stream_compute));
+  n_diags = nd;
+  dia_ld = ldd;
+  // four waves per workgroup, each 64 rows (or 128) at a time; a multiple of
+  // the 8 XCDs when there is work for that many
+  const long long slices = (static_cast<long long>(local_rows) + mek::kWave - 1) / mek::kWave;
+  int g = static_cast<int>(std::min<long long>(mek::kMaxGrid, (slices + 15) / 16));
+  spmv_grid = g >= 8 ? (g + 7) / 8 * 8 : std::max(1, g);
+  spmv_grid = std::min(spmv_grid, mek::kMaxGrid);
+  return true;
+}
+
+void DeviceContext::free_dia() {
+  if (d_dia_val) (void)hipFree(d_dia_val);
+  if (d_dia_off) (void)hipFree(d_dia_off);
+  d_dia_val = nullptr;
+  d_dia_off = nullptr;
+  n_diags = 0;
+  dia_ld = 0;
+}
+
+int DeviceContext::build_spmv_layout(const CSRHost& part, int format) {
+  if (format < 0 || format > 3) throw mek::Error("ERROR: spmv_format must be 0, 1, 2 or 3");
+  free_dia();
+  free_sell();
+  if (format == 1) return 1;
+  if (format == 0 || format == 3) {
+    const std::vector<int> offs =
+        mek::dia_offsets(part.row_ptr.data(), part.col_idx.data(), part.values.data(), local_rows);
+    if (format == 3 && (offs.empty() || local_rows <= 0))
+      throw mek::Error("ERROR: the matrix has no diagonal form (more than 16 diagonals, columns not "
+                       "ascending within a row, or NaN values)");
+    const bool small_pad = !offs.empty() && static_cast<double>(offs.size()) * local_rows <=
+                                                (1.0 + mek::kDiaMaxPad) * static_cast<double>(local_nnz);
+    if (format == 3 || small_pad) {
+      build_dia(part, offs);
+      return 3;
+    }
+  }
+  return build_sell(part, format == 2 ? 2 : 0) ? 2 : 1;
+}
+
 void DeviceContext::free_sell() {
   if (d_sell_ptr) (void)hipFree(d_sell_ptr);
   if (d_sell_col) (void)hipFree(d_sell_col);
@@ -206,6 +285,10 @@ mek::ArnoldiBufs DeviceContext::bufs() const {
   b.sell_col = d_sell_col;
   b.sell_val = d_sell_val;
   b.n_slices = n_slices;
+  b.dia_val = d_dia_val;
+  b.dia_off = d_dia_off;
+  b.n_diags = n_diags;
+  b.dia_ld = dia_ld;
   return b;
 }
 
@@ -254,6 +337,7 @@ void DeviceContext::destroy() {
   fr(d_values);
   fr(d_row_blocks);
   free_sell();
+  free_dia();
   fr(d_V_m);
   fr(d_w);
   fr(d_y);

@@ -384,6 +384,191 @@ __global__ __launch_bounds__(kBlock) void k_sell_build(const int* __restrict__ r
   }
 }
 
+// K1 on the diagonal copy (ArnoldiBufs::dia_*): the same operation and
+// arguments as k_spmv_sell.  No column index is read: row r of diagonal d is
+// (r, r + off_d).  Each lane takes P consecutive rows (a wave 64 P rows), so
+// for each diagonal a lane loads P consecutive values (one 16-byte load for
+// P = 2 FP64 values, non-temporal) and the P consecutive x entries
+// x[r + off_d] (one 16-byte load whatever the offset's parity), the rows' x
+// reads overlapping across diagonals and served by the caches; all loads of a
+// step are issued together.  The waves stride over the 64 P-row groups
+// together in XCD-aware block order, as in k_spmv_sell.  Each row is summed
+// diagonal by diagonal in ascending offset order, which is its CSR order (the
+// copy exists only for rows with ascending columns), from 0.0 with separately
+// rounded products, absent entries skipped: w has the bits of the CSR loop.
+// P = 2 stores w in 16-byte pairs and needs wout 16-byte aligned (measured on
+// the 4096^2 Laplacian in isolation: 139 us against 156-163 us for one row per
+// lane).
+template <typename VT>
+__device__ __forceinline__ bool dia_absent(VT v);
+template <>
+__device__ __forceinline__ bool dia_absent<double>(double v) {
+  return static_cast<unsigned long long>(__double_as_longlong(v)) == kDiaAbsent64;
+}
+template <>
+__device__ __forceinline__ bool dia_absent<float>(float v) {
+  return __float_as_uint(v) == kDiaAbsent32;
+}
+template <typename VT>
+__device__ __forceinline__ VT dia_marker();
+template <>
+__device__ __forceinline__ double dia_marker<double>() {
+  return __longlong_as_double(static_cast<long long>(kDiaAbsent64));
+}
+template <>
+__device__ __forceinline__ float dia_marker<float>() {
+  return __uint_as_float(kDiaAbsent32);
+}
+
+template <typename VT, int P>
+struct DiaVec {
+  typedef VT type __attribute__((ext_vector_type(P)));
+};
+template <typename VT>
+struct DiaVec<VT, 1> {
+  typedef VT type;
+};
+
+template <typename VT, int ND, int P>
+__global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double* x_in, const double* xs_ptr,
+                                                     double* wout, int ndots, int normq, int check_breakdown,
+                                                     int i0, const double* shift_ptr) {
+#pragma clang fp contract(off)
+  if (check_breakdown && (b.iscal[kBreakdown] || (check_breakdown == 2 && b.iscal[kSStepFail]))) return;
+  __shared__ double red[kWavesPerBlock * 2];
+  typedef typename DiaVec<VT, P>::type VTP;
+  typedef typename DiaVec<double, P>::type DP;
+  const bool has_shift = shift_ptr != nullptr;
+  const double sh = has_shift ? *shift_ptr : 0.0;
+  const int lb = xcd_block_id();
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const double* __restrict__ x = x_in;
+  const double xs = xs_ptr ? *xs_ptr : 1.0;
+  const double cs0 = ndots ? b.colscale[i0] : 0.0;
+  const double* __restrict__ Vd = b.V + static_cast<long long>(i0) * b.ld;
+  const VT* __restrict__ dv = static_cast<const VT*>(b.dia_val);
+  const long long n = b.n;
+  const long long ld = b.dia_ld;
+  const int nd = b.n_diags;
+  int off[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) off[d] = (d < nd) ? b.dia_off[d] : 0;
+  double* __restrict__ w = wout;
+  const long long ngroups = (n + kWave * P - 1) / (kWave * P);
+  const long long nw = static_cast<long long>(gridDim.x) * kWavesPerBlock;
+  double acc[2] = {0.0, 0.0};  // q_i0 . w, ||w||^2
+  for (long long g = static_cast<long long>(lb) * kWavesPerBlock + wave; g < ngroups; g += nw) {
+    const long long r0 = (g * kWave + lane) * P;
+    const bool full = r0 + P <= n;  // all P rows exist
+    VT v[ND][P];
+    double xv[ND][P];
+    double xrr[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) xrr[k] = 0.0;
+    if (has_shift) {
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (r0 + k < n) xrr[k] = x[r0 + k];
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const long long c0 = r0 + off[d];
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        v[d][k] = dia_marker<VT>();
+        xv[d][k] = 0.0;
+      }
+      if (d < nd) {
+        if (full && c0 >= 0 && c0 + P <= n) {
+          const VTP vv = __builtin_nontemporal_load(reinterpret_cast<const VTP*>(dv + static_cast<long long>(d) * ld + r0));
+          DP xx;
+          __builtin_memcpy(&xx, x + c0, sizeof(DP));
+          if constexpr (P == 1) {
+            v[d][0] = vv;
+            xv[d][0] = xx;
+          } else {
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+              v[d][k] = vv[k];
+              xv[d][k] = xx[k];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            const long long r = r0 + k, c = c0 + k;
+            if (r < n && c >= 0 && c < n) {
+              v[d][k] = dv[static_cast<long long>(d) * ld + r];
+              xv[d][k] = x[c];
+            }
+          }
+        }
+      }
+    }
+    double a[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      a[k] = 0.0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d)
+        if (!dia_absent<VT>(v[d][k])) a[k] += static_cast<double>(v[d][k]) * (xs * xv[d][k]);
+      if (has_shift) a[k] = shift_row(a[k], sh, xs, xrr[k]);
+    }
+    if (full) {
+      if constexpr (P == 1) {
+        __builtin_nontemporal_store(a[0], w + r0);
+      } else {
+        DP o;
+#pragma unroll
+        for (int k = 0; k < P; ++k) o[k] = a[k];
+        __builtin_nontemporal_store(o, reinterpret_cast<DP*>(w + r0));
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (r0 + k < n) w[r0 + k] = a[k];
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      if (r0 + k < n) {
+        acc[1] += a[k] * a[k];
+        if (ndots) acc[0] += (cs0 * Vd[r0 + k]) * a[k];
+      }
+    }
+  }
+  block_partials<2>(acc, ndots, red, b.partials, gridDim.x, lb);
+  __syncthreads();
+  double nv[1] = {acc[1]};
+  block_partials<1>(nv, 1, red, b.partials + static_cast<long long>(normq) * gridDim.x, gridDim.x, lb);
+}
+
+// The diagonal copy of a device CSR matrix: every entry set to the absent
+// marker, then one thread per row writes each of its entries into the
+// diagonal of its offset.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_dia_fill(VT* __restrict__ dv, long long count) {
+  const VT mk = dia_marker<VT>();
+  for (long long i = gtid(); i < count; i += gstride()) dv[i] = mk;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_dia_build(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                      const VT* __restrict__ va, int n,
+                                                      const int* __restrict__ offs, int nd, long long ld,
+                                                      VT* __restrict__ dv) {
+  for (long long r = gtid(); r < n; r += gstride()) {
+    for (int e = rp[r]; e < rp[r + 1]; ++e) {
+      const long long o = static_cast<long long>(ci[e]) - r;
+      for (int d = 0; d < nd; ++d)
+        if (offs[d] == o) {
+          dv[static_cast<long long>(d) * ld + r] = va[e];
+          break;
+        }
+    }
+  }
+}
+
 // K1 and K1b in one pass for column j (nq = j + 1 <= 64): w = A q_j as in
 // k_spmv_dot, and, while each row block's w is still in LDS, the partials of
 // q_i . w for i < nq over that block's rows, the basis columns split over the
@@ -819,6 +1004,33 @@ template <int MAXJ>
 static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const double* xs, double* w,
                               int ndots, int normq, int chk, hipStream_t s, int i0 = 0,
                               const double* shift = nullptr) {
+  if (b.dia_val && ndots <= 1) {
+#define MEK_DIA(VT, ND, R)                                                                          \
+  hipLaunchKernelGGL((k_spmv_dia<VT, ND, R>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, ndots, \
+                     normq, chk, i0, shift)
+    // two rows per lane when w can take 16-byte stores (every buffer of the
+    // runner; a caller's output of the plain SpMV may not)
+    const bool pair = (reinterpret_cast<uintptr_t>(w) & 15) == 0;
+    if (b.n_diags <= 8) {
+      if (pair) {
+        if (b.vals_f32) MEK_DIA(float, 8, 2);
+        else MEK_DIA(double, 8, 2);
+      } else {
+        if (b.vals_f32) MEK_DIA(float, 8, 1);
+        else MEK_DIA(double, 8, 1);
+      }
+    } else {
+      if (pair) {
+        if (b.vals_f32) MEK_DIA(float, kDiaDiagCap, 2);
+        else MEK_DIA(double, kDiaDiagCap, 2);
+      } else {
+        if (b.vals_f32) MEK_DIA(float, kDiaDiagCap, 1);
+        else MEK_DIA(double, kDiaDiagCap, 1);
+      }
+    }
+#undef MEK_DIA
+    return;
+  }
   if (b.sell_ptr && ndots <= 1) {
     if (b.vals_f32)
       hipLaunchKernelGGL(k_spmv_sell<float>, dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, ndots, normq,
@@ -849,6 +1061,23 @@ void launch_sell_build(const int* row_ptr, const int* col_idx, const void* vals,
     hipLaunchKernelGGL(k_sell_build<double>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx,
                        static_cast<const double*>(vals), n, nslices, sell_ptr, sell_col,
                        static_cast<double*>(sell_val));
+}
+
+void launch_dia_build(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32, int n,
+                      const int* offsets, int n_diags, long long ld, void* dia_val, hipStream_t s) {
+  if (n <= 0 || n_diags <= 0) return;
+  const long long count = static_cast<long long>(n_diags) * ld;
+  const int fgrid = static_cast<int>(std::min<long long>((count + kBlock - 1) / kBlock, 16384));
+  const int grid = static_cast<int>(std::min<long long>((static_cast<long long>(n) + kBlock - 1) / kBlock, 16384));
+  if (vals_f32) {
+    hipLaunchKernelGGL(k_dia_fill<float>, dim3(fgrid), dim3(kBlock), 0, s, static_cast<float*>(dia_val), count);
+    hipLaunchKernelGGL(k_dia_build<float>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx,
+                       static_cast<const float*>(vals), n, offsets, n_diags, ld, static_cast<float*>(dia_val));
+  } else {
+    hipLaunchKernelGGL(k_dia_fill<double>, dim3(fgrid), dim3(kBlock), 0, s, static_cast<double*>(dia_val), count);
+    hipLaunchKernelGGL(k_dia_build<double>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx,
+                       static_cast<const double*>(vals), n, offsets, n_diags, ld, static_cast<double*>(dia_val));
+  }
 }
 
 // s-step block vector i (columns k + i from k - 1 + i): y = (A - theta_i) p

@@ -112,7 +112,21 @@ struct ArnoldiBufs {
   const int* sell_col;
   const void* sell_val;       // double, or float when vals_f32
   int n_slices;
+  // Diagonal copy of the matrix (null when unused): diagonal d holds the
+  // entries (r, r + dia_off[d]) at dia_val[d * dia_ld + r], offsets in
+  // ascending order; an entry the CSR does not hold is a NaN marker
+  // (kDiaAbsent64 / kDiaAbsent32 bits), which the SpMV skips.
+  const void* dia_val;        // double, or float when vals_f32
+  const int* dia_off;
+  int n_diags;
+  long long dia_ld;
 };
+
+// Marker bits of an absent entry in the diagonal copy (quiet NaNs; the copy
+// is only built for matrices without NaN values).
+constexpr unsigned long long kDiaAbsent64 = 0x7FF8DEADBEEF0001ULL;
+constexpr unsigned int kDiaAbsent32 = 0x7FC0BEEFU;
+constexpr int kDiaDiagCap = 16;  // most diagonals the diagonal-copy SpMV takes
 
 enum ScalarSlot : int {
   kVNorm = 0,     // beta = ||start vector||

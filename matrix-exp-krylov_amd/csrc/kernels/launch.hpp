@@ -48,6 +48,12 @@ void launch_spmv_plain(const ArnoldiBufs& b, const double* x, const double* xs, 
 // spmv_dots uses it.
 void launch_sell_build(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32, int n,
                        const long long* sell_ptr, int* sell_col, void* sell_val, hipStream_t s);
+// Diagonal copy (ArnoldiBufs::dia_*) of a device CSR matrix: n_diags x ld
+// entries (ld >= n), offsets (ascending, on the device) from dia_offsets.
+// With dia_val set in the buffers, every SpMV launcher above except the
+// fused spmv_dots uses it (before the sliced-ELL copy).
+void launch_dia_build(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32, int n,
+                      const int* offsets, int n_diags, long long ld, void* dia_val, hipStream_t s);
 
 // ------------------------------------------------------------ s-step blocks
 //

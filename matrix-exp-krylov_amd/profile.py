@@ -9,12 +9,16 @@ Two modes:
     three launches), then one warm-up and one timed solve of the workload.
 
 ``python matrix-exp-krylov_amd/profile.py summarize --stats DIR --fetch DIR
---write DIR --workload W --out-md FILE --out-json FILE [--command TEXT ...]``
+--write DIR [--cache DIR] --workload W --out-md FILE --out-json FILE
+[--command TEXT ...]``
     Reads the rocprofv3 CSV outputs (kernel-trace statistics, and the two
     counter runs), scales each counter by what it reads for the calibration
     kernel of known size, and writes a markdown summary plus a JSON map
     ``{kernel kind: HBM bytes per launch}`` that bench.py attaches to its
-    roofline object as ``traffic``.
+    roofline object as ``traffic``.  With ``--cache``, a third counter run
+    (``--pmc TCC_REQ_sum TCC_HIT_sum TCC_MISS_sum TCP_TCC_READ_REQ_sum``) adds
+    the L2 requests, hits and misses and the L1-to-L2 read requests per
+    launch of each kind.
 """
 from __future__ import annotations
 
@@ -94,9 +98,11 @@ WORKLOADS = {
 
 
 def batch_workload(args, mek, gen):
-    """256 problems of the batch benchmark in one Expokit-controlled lockstep
-    solve with FP32 values (one warm-up solve, one measured)."""
-    probs = gen.batch_problems(P=256, seed=0, f32_exact=True)
+    """256 problems of the batch benchmark (values U[-1,1] FP64, as
+    configured) in one Expokit-controlled lockstep solve; the FP32-value
+    request falls back to FP64 storage on these values, as in the bench (one
+    warm-up solve, one measured)."""
+    probs = gen.batch_problems(P=256, seed=0, f32_exact=False)
     mats = [(q["rp"], q["ci"], q["va"]) for q in probs]
     ms = [min(q["m"], q["rp"].size - 2) for q in probs]
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
@@ -314,7 +320,49 @@ def summarize(args):
                 "|---|---|---|---|\n")
         for k, v in sorted(traffic.items(), key=lambda kv: -kv[1]):
             f.write(f"| {k} | {working[k]} | {idle[k]} | {v / 1e6:.1f} |\n")
+        if args.cache:
+            cache = cache_per_kind(args.cache)
+            f.write("\n## L2 (TCC) and L1-to-L2 requests per launch\n\n"
+                    f"Averages over the launches with at least {CACHE_MIN_REQ} L2 requests.\n\n"
+                    "| kind | launches | TCC_REQ | TCC_HIT | TCC_MISS | L2 hit rate | TCP_TCC_READ_REQ |\n"
+                    "|---|---|---|---|---|---|---|\n")
+            for k, c in sorted(cache.items(), key=lambda kv: -kv[1]["TCC_REQ_sum"]):
+                f.write(f"| {k} | {c['launches']} | {c['TCC_REQ_sum'] / 1e6:.2f}M | {c['TCC_HIT_sum'] / 1e6:.2f}M | "
+                        f"{c['TCC_MISS_sum'] / 1e6:.2f}M | {c['hit_rate']:.3f} | "
+                        f"{c['TCP_TCC_READ_REQ_sum'] / 1e6:.2f}M |\n")
+            out["cache_per_launch"] = cache
+            if args.out_json:
+                with open(args.out_json, "w") as g:
+                    json.dump(out, g, indent=1)
     print(json.dumps(out))
+
+
+CACHE_COUNTERS = ("TCC_REQ_sum", "TCC_HIT_sum", "TCC_MISS_sum", "TCP_TCC_READ_REQ_sum")
+CACHE_MIN_REQ = 8192  # L2 requests of a launch that did work (about 1 MiB)
+
+
+def cache_per_kind(d):
+    """Per kind: mean of each cache counter over the working launches."""
+    per = {c: read_counter(d, c) for c in CACHE_COUNTERS}
+    n = len(per["TCC_REQ_sum"])
+    if any(len(v) != n for v in per.values()):
+        raise SystemExit("cache counters: dispatch counts differ between counters")
+    acc = defaultdict(lambda: defaultdict(float))
+    for i in range(n):
+        name, req = per["TCC_REQ_sum"][i]
+        if "neg" in name.lower() or req < CACHE_MIN_REQ:
+            continue
+        a = acc[kind_of(name)]
+        a["launches"] += 1
+        for c in CACHE_COUNTERS:
+            a[c] += per[c][i][1]
+    out = {}
+    for k, a in acc.items():
+        L = a["launches"]
+        out[k] = {c: a[c] / L for c in CACHE_COUNTERS}
+        out[k]["launches"] = int(L)
+        out[k]["hit_rate"] = a["TCC_HIT_sum"] / max(a["TCC_REQ_sum"], 1.0)
+    return out
 
 
 def main():
@@ -329,6 +377,7 @@ def main():
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch", default=None)
     s.add_argument("--write", default=None)
+    s.add_argument("--cache", default=None, help="directory of a TCC_REQ/HIT/MISS + TCP_TCC_READ_REQ run")
     s.add_argument("--workload", required=True)
     s.add_argument("--out-md", required=True)
     s.add_argument("--out-json", default=None)

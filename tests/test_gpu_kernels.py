@@ -7,6 +7,7 @@ expm: against the oracle's Pade restatement (Higham 2005).
 Gram-Schmidt: the Hessenberg matrix of one Arnoldi decomposition (CGS2 on the
 device) against the oracle's MGS2.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -110,6 +111,107 @@ def test_spmv_sell_bitexact(gpu, oracle, which):
     assert entries == sum(64 * int(lens[64 * s:64 * (s + 1)].max()) for s in range(slices))
     np.testing.assert_array_equal(y, oracle.spmv(rp, ci, va, x))
     np.testing.assert_array_equal(y, y_csr)
+    plan.close()
+
+
+def _banded_sparse(n, offsets, keep, seed):
+    """Entries on the given diagonals, each kept with probability `keep`
+    (absent entries inside the band), explicit zeros and negative zeros among
+    the values, columns ascending within each row."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for o in offsets:
+        r = np.arange(max(0, -o), min(n, n - o))
+        r = r[rng.random(r.size) < keep]
+        rows.append(r)
+        cols.append(r + o)
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    order = np.lexsort((c, r))
+    r, c = r[order], c[order]
+    va = rng.uniform(-1, 1, r.size)
+    va[rng.random(r.size) < 0.05] = 0.0
+    va[rng.random(r.size) < 0.05] = -0.0
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    return np.cumsum(rp).astype(np.int32), c.astype(np.int32), va
+
+
+@pytest.mark.parametrize("which", ["laplacian", "convdiff", "convdiff_f32", "band_holes", "nine", "sixteen",
+                                   "n65", "one_row", "zero_x"])
+def test_spmv_dia_bitexact(gpu, oracle, which):
+    """The diagonal copy gives the oracle's CSR-loop bits: stencils with their
+    absent boundary entries, FP32 values, bands with holes, explicit (and
+    negative) zeros, 9 and 16 diagonals (the two kernel shapes), row counts
+    that are not a multiple of 64 or 128, one row, and an x with zeros (where
+    an added 0.0 product would turn a -0.0 row sum into +0.0)."""
+    print(f"[spmv-dia] {which}", flush=True)
+    f32 = which == "convdiff_f32"
+    n_diags = None
+    if which == "laplacian":
+        rp, ci, va = gen.laplacian_2d(77)
+        n_diags = 5
+    elif which.startswith("convdiff"):
+        rp, ci, va = gen.convdiff_3d(19)
+        n_diags = 7
+    elif which == "band_holes":
+        rp, ci, va = _banded_sparse(1000, [-300, -7, -3, 0, 2, 5, 11, 640], 0.6, seed=3)
+    elif which == "nine":
+        rp, ci, va = _banded_sparse(2000, [-40, -9, -4, -1, 0, 1, 4, 9, 40], 0.9, seed=4)
+        n_diags = 9
+    elif which == "sixteen":
+        rp, ci, va = _banded_sparse(3001, list(range(-8, 8)), 1.0, seed=5)
+        n_diags = 16
+    elif which == "n65":
+        rp, ci, va = _banded_sparse(65, [-1, 0, 1, 64], 0.8, seed=6)
+    elif which == "one_row":
+        rp, ci, va = np.array([0, 1], np.int32), np.array([0], np.int32), np.array([-2.5])
+        n_diags = 1
+    else:
+        rp, ci, va = _banded_sparse(500, [-2, -1, 0, 1, 2], 1.0, seed=7)
+        va = -np.abs(va)
+    n = rp.size - 1
+    x = np.random.default_rng(9).standard_normal(n)
+    if which == "zero_x":
+        x[::3] = 0.0
+    plan = mek.SpmvPlan(rp, ci, va, values_f32=f32)
+    y_csr = plan(x)
+    nd = plan.use_dia()
+    if n_diags is not None:
+        assert nd == n_diags
+    assert nd == np.unique(ci.astype(np.int64) - np.repeat(np.arange(n), np.diff(rp))).size
+    y = plan(x)
+    yo = oracle.spmv(rp, ci, va.astype(np.float32) if f32 else va, x)
+    np.testing.assert_array_equal(y.view(np.int64), yo.view(np.int64))
+    np.testing.assert_array_equal(y.view(np.int64), y_csr.view(np.int64))
+    # an output that is 8- but not 16-byte aligned takes the one-row-per-lane form
+    d_x = mek.DeviceBuffer.from_array(x)
+    d_y = mek.DeviceBuffer(8 * (n + 1))
+    mek._check(mek.lib().mek_spmv_execute(plan.h, d_x.ptr, ctypes.c_void_p(d_y.ptr.value + 8), None))
+    mek.synchronize()
+    y1 = d_y.to_array()[1:]
+    np.testing.assert_array_equal(y1.view(np.int64), yo.view(np.int64))
+    plan.close()
+
+
+@pytest.mark.parametrize("which", ["unsorted", "seventeen", "nan"])
+def test_spmv_dia_refused(gpu, which):
+    """No diagonal copy for rows with unsorted columns (the diagonal order would
+    not be the CSR order), more than 16 diagonals, or NaN values (NaN marks
+    absent entries)."""
+    if which == "unsorted":
+        rp = np.array([0, 2, 3], np.int32)
+        ci = np.array([1, 0, 1], np.int32)
+        va = np.array([1.0, 2.0, 3.0])
+    elif which == "seventeen":
+        rp, ci, va = _banded_sparse(200, list(range(-8, 9)), 1.0, seed=8)
+    else:
+        rp, ci, va = gen.laplacian_2d(8)
+        va = va.copy()
+        va[5] = np.nan
+    plan = mek.SpmvPlan(rp, ci, va)
+    with pytest.raises(mek.MekError):
+        plan.use_dia()
     plan.close()
 
 

@@ -439,30 +439,64 @@ def test_odd_row_count(gpu, oracle, gs_mode):
 
 @pytest.mark.parametrize("sstep", [0, 12])
 def test_spmv_formats_agree(gpu, sstep):
-    """The automatic SpMV layout is the sliced-ELL copy for a stencil and the
-    CSR row blocks for rows of 5-30 random nonzeros (slices padded to their
-    longest row would carry ~70 % padding); forcing either layout gives the
-    same solve to rounding, within 1e-8 of the DST-I closed form."""
+    """The automatic SpMV layout is the diagonal copy for a stencil, the
+    sliced-ELL copy for a band of more than 16 diagonals, and the CSR row
+    blocks for rows of 5-30 random nonzeros (slices padded to their longest
+    row would carry ~70 % padding); forcing any layout gives the same solve to
+    rounding, within 1e-8 of the DST-I closed form."""
     N, t = 80, 10.0
     rp, ci, va = gen.laplacian_2d(N)
     v = gen.normal_vector(N * N, seed=4)
     kw = dict(m=40, t=t, tol=1e-10, max_restarts=0, step_control="expokit", gs_mode=2, sstep=sstep)
-    ys = {}
-    for fmt in ("auto", "csr", "sell"):
+    ys, infos = {}, {}
+    for fmt in ("auto", "csr", "sell", "dia"):
         R = _runner(rp, ci, va, spmv_format=fmt, **kw)
-        info = R.spmv_format()
-        assert info["format"] == ("csr" if fmt == "csr" else "sell"), (fmt, info)
+        infos[fmt] = R.spmv_format()
         ys[fmt] = R.expmv(v)
         R.close()
+    assert infos["auto"]["format"] == "dia" and infos["csr"]["format"] == "csr"
+    assert infos["sell"]["format"] == "sell" and infos["dia"]["format"] == "dia"
     lens = np.diff(rp)
-    assert info["width"] == 5
-    assert info["entries"] == sum(64 * int(lens[64 * q:64 * (q + 1)].max()) for q in range((N * N + 63) // 64))
+    assert infos["sell"]["width"] == 5
+    assert infos["sell"]["entries"] == sum(64 * int(lens[64 * q:64 * (q + 1)].max())
+                                           for q in range((N * N + 63) // 64))
+    assert infos["dia"]["width"] == 5 and infos["dia"]["entries"] == 5 * N * N
     ref = laplacian_2d_expmv(N, t, v)
     for fmt, y in ys.items():
         assert rel_err(y, ref) <= TOL, fmt
     assert rel_err(ys["sell"], ys["csr"]) <= 1e-12
-    np.testing.assert_array_equal(ys["auto"], ys["sell"])
+    assert rel_err(ys["dia"], ys["csr"]) <= 1e-12
+    np.testing.assert_array_equal(ys["auto"], ys["dia"])
     rp, ci, va = gen.random_rows(4000, 5, 30, seed=3)
     R = _runner(rp, ci, va, m=20, t=0.5, tol=1e-10, max_restarts=0, step_control="expokit")
     assert R.spmv_format()["format"] == "csr"
     R.close()
+    with pytest.raises(mek.MekError):
+        _runner(rp, ci, va, m=20, t=0.5, tol=1e-10, max_restarts=0, spmv_format="dia")
+    rp, ci, va = _band(3000, list(range(-10, 11)))
+    R = _runner(rp, ci, va, m=20, t=0.5, tol=1e-10, max_restarts=0, step_control="expokit")
+    assert R.spmv_format()["format"] == "sell"
+    R.close()
+
+
+def _band(n, offsets, seed=6):
+    """A full band: row r holds the columns r + o (o in offsets, in range),
+    values U[-1,1] with a Gershgorin-negative diagonal."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for o in sorted(offsets):
+        r = np.arange(max(0, -o), min(n, n - o))
+        rows.append(r)
+        cols.append(r + o)
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    order = np.lexsort((c, r))
+    r, c = r[order], c[order]
+    va = rng.uniform(-1, 1, r.size)
+    diag = r == c
+    off = np.zeros(n)
+    np.add.at(off, r[~diag], np.abs(va[~diag]))
+    va[diag] = -1.0 - off[r[diag]]
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, r + 1, 1)
+    return np.cumsum(rp).astype(np.int32), c.astype(np.int32), va
