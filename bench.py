@@ -371,6 +371,7 @@ def batch_comparison(args, mek, gen, reps, cpu_seconds):
         log("batch: CPU baseline (oracle, one thread, bounded sample)")
         cpu = cpu_baseline_batch(mats, ts, ms, vs, tol, cpu_seconds)
     speed = head["problems_per_s"] / res["plain_fp64"]["problems_per_s"]
+    stencil = stencil_batch_comparison(args, mek, gen, reps)
     return {
         "workload": ("batch256: 256 problems, n_i log-uniform in [1e3, 2e5], random rows with 5-30 "
                      "nonzeros, values U[-1,1] FP64, Gershgorin-negative diagonal, t_i ~ U[0.1, 5], "
@@ -386,7 +387,81 @@ def batch_comparison(args, mek, gen, reps, cpu_seconds):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "forms": res,
+        "stencil_batch": stencil,
     }
+
+
+def stencil_batch_comparison(args, mek, gen, reps):
+    """Where FP32 value storage is lossless: 64 independent stencil problems
+    (32 2-D Laplacians on N x N grids, N log-uniform in [40, 600], t ~ U[1, 10];
+    32 3-D convection-diffusion operators on N^3 grids, N log-uniform in
+    [16, 60], t ~ U[0.05, 0.5]; v ~ N(0,1) normalised; m = 30), solved as one
+    batched FP64 solve and one with FP32 values (exact here: the stencil
+    coefficients are FP32 numbers), each against its closed form (2-D DST-I,
+    Kronecker product of 1-D exponentials) at 1e-8, with the batched SpMV
+    timed over all problems in both storages."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from closed_forms import kron3_expmv, laplacian_2d_expmv
+    rng = np.random.default_rng(11)
+    tol = 1e-8
+    mats, ts, vs, refs = [], [], [], []
+    (lx, dx, ux), (ly, dy, uy), (lz, dz, uz) = gen.convdiff_3d_coeffs()
+    for i in range(64):
+        if i % 2 == 0:
+            N = int(round(math.exp(rng.uniform(math.log(40), math.log(600)))))
+            t = float(rng.uniform(1.0, 10.0))
+            A = gen.laplacian_2d(N)
+            v = gen.normal_vector(N * N, seed=1000 + i)
+            refs.append(laplacian_2d_expmv(N, t, v))
+        else:
+            N = int(round(math.exp(rng.uniform(math.log(16), math.log(60)))))
+            t = float(rng.uniform(0.05, 0.5))
+            A = gen.convdiff_3d(N)
+            v = gen.normal_vector(N ** 3, seed=1000 + i)
+            refs.append(kron3_expmv(gen.tridiag_1d(N, lz, dz, uz), gen.tridiag_1d(N, ly, dy, uy),
+                                    gen.tridiag_1d(N, lx, dx, ux), t, v))
+        mats.append(A)
+        ts.append(t)
+        vs.append(v)
+    csrs = [mek.CSR.from_arrays(*A) for A in mats]
+    inexact = int(sum(c.f32_inexact(nonfinite=True)[0] for c in csrs))
+    rows = [A[0].size - 1 for A in mats]
+    nnz = int(sum(int(A[0][-1]) for A in mats))
+    gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
+    out = {"workload": ("64 stencil problems: 32 2-D Laplacians (N log-uniform in [40, 600], t ~ U[1, 10]) "
+                        "and 32 3-D convection-diffusion operators (N^3, N log-uniform in [16, 60], "
+                        "t ~ U[0.05, 0.5]), m = 30, tol 1e-8, seed 11"),
+           "problems": len(mats), "rows": int(sum(rows)), "nnz": nnz, "values_f32_inexact": inexact}
+    for name, f32 in (("batched_fp64", False), ("batched_fp32_values", True)):
+        log(f"stencil batch: {name}")
+        B = mek.Batch(csrs, ts, [30] * len(mats), [tol / t for t in ts], values_f32=f32, gs_mode=gs_mode,
+                      step_control="expokit_rel", autotune=args.autotune)
+        B.upload(vs)
+        B.solve()
+        mek.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            B.solve()
+        mek.synchronize()
+        el = (time.perf_counter() - t0) / reps
+        ys = B.download()
+        err = max(float(np.linalg.norm(y - r) / np.linalg.norm(r)) for y, r in zip(ys, refs))
+        st = [B.stats(p) for p in range(len(mats))]
+        vb = 4 if B.values_f32_used() else 8
+        spmv_us = B.time_spmv(20)
+        spmv_bytes = nnz * (vb + 4) + 4.0 * (sum(rows) + 1) + 16.0 * sum(rows)
+        out[name] = {"problems_per_s": len(mats) / el, "solve_s": el, "values_f32_used": B.values_f32_used(),
+                     "krylov_columns": int(sum(x["spmv_count"] for x in st)),
+                     "converged": int(sum(x["converged"] for x in st)), "max_rel_err": err,
+                     "spmv_us": spmv_us, "spmv_gbs": round(spmv_bytes / (spmv_us * 1e-6) / 1e9, 1)}
+        B.close()
+    for c in csrs:
+        c.close()
+    out["speedup_fp32_vs_fp64"] = round(out["batched_fp32_values"]["problems_per_s"] /
+                                        out["batched_fp64"]["problems_per_s"], 3)
+    out["spmv_speedup_fp32_vs_fp64"] = round(out["batched_fp64"]["spmv_us"] /
+                                             out["batched_fp32_values"]["spmv_us"], 3)
+    return out
 
 
 def run_batch(args, mek, gen):
@@ -413,6 +488,7 @@ def run_batch(args, mek, gen):
         "roofline": b["roofline"],
         "cpu_baseline": b["cpu_baseline"],
         "forms": b["forms"],
+        "stencil_batch": b["stencil_batch"],
     }
     print(json.dumps(out), flush=True)
 

@@ -59,6 +59,8 @@ KINDS = [
     ("k_ss_update", "sstep_update"),
     ("k_ss_", "finalise"),
     ("k_spmv_dot", "spmv"),
+    ("k_spmv_dia", "spmv"),
+    ("k_spmv_sell", "spmv"),
     ("k_update_dot", "update_dot"),
     ("k_update_norm", "update_norm"),
     ("k_dots", "dots"),
