@@ -396,9 +396,11 @@ __global__ __launch_bounds__(kBlock) void k_sell_build(const int* __restrict__ r
 // diagonal by diagonal in ascending offset order, which is its CSR order (the
 // copy exists only for rows with ascending columns), from 0.0 with separately
 // rounded products, absent entries skipped: w has the bits of the CSR loop.
-// P = 2 stores w in 16-byte pairs and needs wout 16-byte aligned (measured on
-// the 4096^2 Laplacian in isolation: 139 us against 156-163 us for one row per
-// lane).
+// P > 1 stores w in 8P-byte groups and needs wout 16-byte aligned (measured
+// on the 4096^2 Laplacian in isolation: 139 us against 156-163 us for one row
+// per lane; with FP32 values on the 256^3 convection-diffusion matrix P = 4,
+// whose value loads are 16 bytes too, took 167 us in the solve against 159 us
+// for P = 2).
 template <typename VT>
 __device__ __forceinline__ bool dia_absent(VT v);
 template <>

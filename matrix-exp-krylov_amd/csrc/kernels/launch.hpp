@@ -84,6 +84,7 @@ enum SsFlag : int {
   kSsBlocks,       // blocks accepted (statistics, kept across solves)
   kSsSecond,       // blocks that took a second update or Cholesky pass (statistics)
   kSsFailK,        // k of the block that failed (iscal[kSStepFail] set)
+  kSsFused,        // the first update wrote Q_new = (P - Q_k C1) R^-1 (R from the dots)
   kSsNumFlags = 8
 };
 
@@ -109,8 +110,10 @@ struct SsArgs {
 //   for i < s: launch_ss_spmv(i), launch_ss_sigma(i)      p_i, sigma_i
 //   launch_ss_dots                                        Q^T P over [0, k+s)
 //   launch_ss_reduce, launch_ss_plan(kSsPhaseA)
-//   launch_ss_update(window [0, k+s), cond -1)            P' = P - Q_k C
-//   launch_ss_reduce, launch_ss_plan(kSsPhaseB, call 0)
+//   launch_ss_update(window [0, k+s), cond -1)            P' = P - Q_k C, or
+//                                                         Q_new = P' R^-1 at once
+//   launch_ss_reduce, launch_ss_plan(kSsPhaseB, call 0)   (accepts a Q_new that
+//                                                         passes both checks)
 //   the same three with cond kSsPass2 and call 1         second update if due
 //   launch_ss_update(window [k, k+s), cond kSsNormReady)  Q_new = P' R^-1
 //   launch_ss_reduce, launch_ss_plan(kSsPhaseD, call 0)

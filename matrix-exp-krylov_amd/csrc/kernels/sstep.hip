@@ -12,6 +12,13 @@
 //   Q_new = P' R^-1, R^T R = P'^T P' (Cholesky QR), with Q_new^T Q_new on the
 //    side (Q_new R2^-1 once more when that Gram matrix is off I by > 1e-12)
 //
+// When every column of P keeps at least a tenth of its norm in P' (the
+// DGKS test), P'^T P' follows from the dots by Pythagoras, P^T P - C1^T C1,
+// and the update writes Q_new = (P - Q_k C1) R^-1 at once; its side dots are
+// then Q_k^T Q_new and Q_new^T Q_new, and a Q_new that passes both checks
+// (1e-12) is accepted without the normalising sweep.  One that does not goes
+// on as a P' would: a second update, or a second Cholesky QR sweep.
+//
 // and recovers the new Hessenberg columns from the change of basis: with
 // [p_0 .. p_s] = [Q_k Q_new] T and A [p_0 .. p_{s-1}] = [p_0 .. p_s] B (B
 // bidiagonal: theta_i on the diagonal, sigma_i below it), the columns k-1 ..
@@ -255,6 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_ss_sigma(SsArgs a, int k, int i, int
       a.fl[kSsNormReady] = 0;
       a.fl[kSsNorm2] = 0;
       a.fl[kSsPasses] = 1;
+      a.fl[kSsFused] = 0;
     }
   }
 }
@@ -332,6 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
         if (c < k) {
           const double c1 = sD[c][j] * sCs[c] * sCs[k + j];
           Ct[e] = c1;
+          sC[c][j] = c1;
           mv = -c1 * sCs[c];
           if (!isfinite(c1)) {
             a.iscal[kSStepFail] = 1;
@@ -343,13 +352,69 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
       }
       M[e] = mv;
     }
+    if (force2) return;
+    // P'^T P' = P^T P - C1^T C1 (true columns) and the DGKS test per column
+    __syncthreads();
+    if (tid < kSsMaxS * kSsMaxS) {
+      const int l = tid / kSsMaxS, j = tid % kSsMaxS;
+      double g = 0.0;
+      if (l < s && j < s) {
+        g = sD[k + l][j] * sCs[k + l] * sCs[k + j];
+        for (int c = 0; c < k; ++c) g -= sC[c][l] * sC[c][j];
+      }
+      G[l][j] = g;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      bool ok = true;
+      for (int j = 0; j < s && ok; ++j) {
+        const double p2 = sD[k + j][j] * sCs[k + j] * sCs[k + j];
+        ok = G[j][j] >= 0.01 * p2;
+      }
+      ok = ok && chol_inv(G, s, R, Ri);
+      if (ok) {
+        double lo = R[0][0], hi = R[0][0];
+        for (int j = 1; j < s; ++j) {
+          lo = fmin(lo, R[j][j]);
+          hi = fmax(hi, R[j][j]);
+        }
+        ok = hi <= 1e7 * lo;
+      }
+      state = ok ? 1 : 0;
+    }
+    __syncthreads();
+    if (state != 1) return;
+    // Q_new = (P - Q_k C1) R^-1: coefficients of the raw window columns
+    for (int e = tid; e < kSsMaxCols * kSsMaxS; e += kBlock) {
+      const int c = e / kSsMaxS, j = e % kSsMaxS;
+      double mv = 0.0;
+      if (j < s && c < k + s) {
+        if (c < k) {
+          double x = 0.0;
+          for (int l = 0; l <= j; ++l) x += sC[c][l] * Ri[l][j];
+          mv = -x * sCs[c];
+        } else if (c - k <= j) {
+          mv = Ri[c - k][j] * sCs[c];
+        }
+      }
+      M[e] = mv;
+    }
+    if (tid < kSsMaxS * kSsMaxS) {
+      const int l = tid / kSsMaxS, j = tid % kSsMaxS;
+      Rt[tid] = R[l][j];
+    }
+    if (tid == 0) a.fl[kSsFused] = 1;
     return;
   }
   if (phase == kSsPhaseB) {
-    // side dots of P' (stored true, column scale 1 from here on)
+    // side dots of P' (stored true, column scale 1 from here on); after a
+    // fused first update (call 0) they are those of Q_new = P' R^-1, and R
+    // (R_tot so far) is staged in sX
+    const bool fused = a.fl[kSsFused] != 0;
     if (tid < kSsMaxS * kSsMaxS) {
       const int l = tid / kSsMaxS, j = tid % kSsMaxS;
       G[l][j] = (l < s && j < s) ? sD[k + l][j] : 0.0;
+      sX[l][j] = Rt[tid];
     }
     if (tid < s) {
       // second update due when ||Q_k^T p'_j|| > 1e-12 ||p'_j|| (also on NaN)
@@ -368,6 +433,17 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
       for (int j = 0; j < s; ++j) need = need || need_col[j];
       if (need) {
         state = (call == 0) ? 1 : 2;
+      } else if (fused && call == 0) {
+        // Q_new orthogonal to Q_k: accept it when Q_new^T Q_new is I to
+        // 1e-12, else one Cholesky QR sweep more
+        double err = 0.0;
+        for (int l = 0; l < s; ++l)
+          for (int j = 0; j < s; ++j) {
+            const double e = fabs(G[l][j] - (l == j ? 1.0 : 0.0));
+            err = (e > err || !(e == e)) ? e : err;
+          }
+        if (err <= 1e-12) state = 3;
+        else state = (err <= 1e-2 && chol_inv(G, s, R, Ri)) ? 0 : 2;
       } else {
         state = chol_inv(G, s, R, Ri) ? 0 : 2;
         if (state == 0) {
@@ -390,15 +466,23 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
       }
       return;
     }
+    if (state == 3) goto accepted;
     if (state == 1) {
-      // second update: P'' = P' - Q_k C2; C_tot += C2
+      // second update: P'' = P' - Q_k C2; C_tot += C2 (after a fused
+      // update the stored columns are Q_new = P' R^-1, so C_tot += C2 R)
       for (int e = tid; e < kSsMaxCols * kSsMaxS; e += kBlock) {
         const int c = e / kSsMaxS, j = e % kSsMaxS;
         double mv = 0.0;
         if (j < s && c < k + s) {
           if (c < k) {
             const double c2 = sD[c][j] * sCs[c];
-            Ct[e] += c2;
+            if (fused) {
+              double x = 0.0;
+              for (int l = 0; l <= j; ++l) x += sD[c][l] * sCs[c] * sX[l][j];
+              Ct[e] += x;
+            } else {
+              Ct[e] += c2;
+            }
             mv = -c2 * sCs[c];
           } else {
             mv = (c - k == j) ? 1.0 : 0.0;
@@ -412,18 +496,25 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
       }
       return;
     }
-    // normalising sweep over the window [k, k+s): Q_new = P' R^-1
+    // normalising sweep over the window [k, k+s): Q_new = P' R^-1 (after a
+    // fused update, a second Cholesky QR of its Q_new); R_tot = R R_tot then
     for (int e = tid; e < kSsMaxCols * kSsMaxS; e += kBlock) {
       const int l = e / kSsMaxS, j = e % kSsMaxS;
       M[e] = (l < s && j < s) ? Ri[l][j] : 0.0;
     }
     if (tid < kSsMaxS * kSsMaxS) {
       const int l = tid / kSsMaxS, j = tid % kSsMaxS;
-      Rt[tid] = R[l][j];
+      double x = R[l][j];
+      if (fused) {
+        x = 0.0;
+        for (int q = l; q <= j && q < s; ++q) x += R[l][q] * sX[q][j];
+      }
+      Rt[tid] = x;
     }
     if (tid == 0) {
       a.fl[kSsNormReady] = 1;
       a.fl[kSsPass2] = 0;
+      if (fused && call == 0) a.fl[kSsPasses] = 2;
     }
     return;
   }
@@ -477,6 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_ss_plan(SsArgs a, int phase, int cal
   }
   // accepted: Hessenberg columns k-1 .. k+s-2 = (T B - H_old U) W^-1, with
   // [p_0 .. p_s] = [Q_k Q_new] T
+accepted:
   const double* th = a.st + kSsTheta;
   const double* sg = a.st + kSsSigma;
   const int ldh = a.ldh;
