@@ -5,7 +5,7 @@ device-resident v and y).  Default workload: 2-D 5-point Laplacian on a
 4096 x 4096 grid (n = 16,777,216, nnz = 83,869,696, FP64 values, i32
 indices), v ~ N(0,1) normalised (seed 0), t = 10, m = 40, tol 1e-8, sub-steps
 by the Expokit heuristic.  FP64 Arnoldi with the basis grown by s-step
-blocks of 12 vectors after the first 6 columns of each cycle (--sstep 0 gives
+blocks of 16 vectors after the first 8 columns of each cycle (--sstep 0 gives
 the reference's column-by-column Gram-Schmidt).
 
 Prints one JSON line with
@@ -632,11 +632,13 @@ def run_single(args, mek, gen):
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
     A = mek.CSR.from_arrays(rp, ci, va)
 
+    sstep, sstep_first = mek.sstep_plan(w["m"], args.sstep, args.sstep_first)
+
     def runner(timing):
         return mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control=args.step_control,
                           gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=timing,
                           fuse_spmv_dot=args.fuse, lanczos=args.lanczos, autotune=args.autotune,
-                          sstep=args.sstep, sstep_first=args.sstep_first, spmv_format=args.spmv_format)
+                          sstep=sstep, sstep_first=sstep_first, spmv_format=args.spmv_format)
 
     R = runner(False)
     layout = R.spmv_format()
@@ -753,7 +755,7 @@ def run_single(args, mek, gen):
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "t": w["t"], "m": w["m"],
                    "tol": w["tol"], "step_control": args.step_control, "gs": args.gs,
                    "fused_spmv_dot": args.fuse, "lanczos": args.lanczos, "autotune": args.autotune,
-                   "sstep": args.sstep, "sstep_first": args.sstep_first, "spmv_format": layout["format"],
+                   "sstep": sstep, "sstep_first": sstep_first, "spmv_format": layout["format"],
                    "seed": 0},
         "rel_err": rel,
         "values_f32": f32_values,
@@ -798,12 +800,14 @@ def main():
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="leave the batched comparison out of the default line")
     ap.add_argument("--block-m", type=int, default=0, help="block Arnoldi steps (block2048; 0: 16)")
-    ap.add_argument("--sstep", type=int, default=12,
-                    help="s-step blocks of this many basis vectors (0: column by column, as the reference)")
+    ap.add_argument("--sstep", type=int, default=-1,
+                    help="s-step blocks of this many basis vectors (0: column by column, as the reference; "
+                         "-1: equal blocks of at most 16 after the first 8 columns)")
     ap.add_argument("--spmv-format", default="auto", choices=["auto", "csr", "sell", "dia"],
                     help="SpMV layout: the diagonal copy for stencils, else sliced ELL where its padding is small (auto); CSR row blocks; sliced ELL; diagonal copy")
-    ap.add_argument("--sstep-first", type=int, default=6,
-                    help="columns of each cycle built one by one before the blocks (-1: --sstep)")
+    ap.add_argument("--sstep-first", type=int, default=-1,
+                    help="columns of each cycle built one by one before the blocks (-1: 8 with --sstep -1, "
+                         "else --sstep)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the solver runs on one GPU; use --gpus 1 (each problem's rows would be split "

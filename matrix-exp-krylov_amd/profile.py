@@ -163,9 +163,10 @@ def workload(args):
     gs_mode = {"always": 1, "never": 0, "dgks": 2}[args.gs]
     A = mek.CSR.from_arrays(rp, ci, va)
     # the bench's runner: autotuned launch shapes, s-step blocks as given
+    sstep, sstep_first = mek.sstep_plan(w["m"], args.sstep, args.sstep_first)
     R = mek.Runner(A, m=w["m"], t=w["t"], tol=w["tol"], max_restarts=0, step_control="expokit",
                    gs_mode=gs_mode, values_f32=w["f32"], verbose=False, timing=False, autotune=True,
-                   sstep=args.sstep, sstep_first=args.sstep_first)
+                   sstep=sstep, sstep_first=sstep_first)
     A.close()
     d_v = mek.DeviceBuffer.from_array(v)
     d_y = mek.DeviceBuffer(8 * v.size)
@@ -373,8 +374,9 @@ def main():
     a = sub.add_parser("workload")
     a.add_argument("--workload", default="laplacian4096", choices=sorted(WORKLOADS) + ["batch256", "block2048"])
     a.add_argument("--gs", default="dgks", choices=["always", "dgks", "never"])
-    a.add_argument("--sstep", type=int, default=12, help="s-step block size (0: column by column)")
-    a.add_argument("--sstep-first", type=int, default=6)
+    a.add_argument("--sstep", type=int, default=-1,
+                   help="s-step block size (0: column by column; -1: the bench's shape, mek.sstep_plan)")
+    a.add_argument("--sstep-first", type=int, default=-1)
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch", default=None)

@@ -1,11 +1,13 @@
 """Full-size parity on one MI355X against size-independent closed forms.
 
 * 2-D Laplacian on a 4096 x 4096 grid (n = 16.8M), t = 10, m = 40, tol 1e-8,
-  Expokit step control, column by column and with s-step blocks of 12:
-  y against the DST-I closed form.
+  Expokit step control, column by column, with s-step blocks of 12 after 6
+  columns, and with the bench's shape (16 after 8): y against the DST-I
+  closed form.
 * 3-D convection-diffusion on 256^3 (n = 16.8M, Pe = 50), t = 0.5, m = 50,
-  FP32-stored and FP64 values, and FP64 with s-step blocks of 12: y against
-  the Kronecker closed form.
+  FP32-stored and FP64 values, and with s-step blocks (12 after 6; the
+  bench's 14 after 8, FP64 and FP32 values): y against the Kronecker closed
+  form.
 """
 import numpy as np
 import pytest
@@ -16,14 +18,15 @@ from conftest import gen, mek
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("sstep", [0, 12])
-def test_laplacian_4096_expokit(gpu, sstep):
+@pytest.mark.parametrize("sstep,first", [(0, -1), (12, 6), (-1, -1)])
+def test_laplacian_4096_expokit(gpu, sstep, first):
     N, t = 4096, 10.0
-    print(f"[scale] laplacian 4096^2 build (sstep {sstep})", flush=True)
+    sstep, first = mek.sstep_plan(40, sstep, first)
+    print(f"[scale] laplacian 4096^2 build (sstep {sstep}, first {first})", flush=True)
     rp, ci, va = gen.laplacian_2d(N)
     v = gen.normal_vector(N * N, seed=0)
     R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=40, t=t, tol=1e-8, max_restarts=0,
-                   step_control="expokit", gs_mode=2 if sstep else -1, sstep=sstep, sstep_first=6)
+                   step_control="expokit", gs_mode=2 if sstep else -1, sstep=sstep, sstep_first=first)
     del rp, ci, va
     print("[scale] laplacian 4096^2 solve", flush=True)
     y = R.expmv(v)
@@ -35,16 +38,18 @@ def test_laplacian_4096_expokit(gpu, sstep):
     R.close()
 
 
-@pytest.mark.parametrize("f32,sstep", [(True, 0), (False, 0), (False, 12)])
-def test_convdiff_256(gpu, f32, sstep):
+@pytest.mark.parametrize("f32,sstep,first", [(True, 0, -1), (False, 0, -1), (False, 12, 6), (False, -1, -1),
+                                             (True, -1, -1)])
+def test_convdiff_256(gpu, f32, sstep, first):
     N, t = 256, 0.5
-    print(f"[scale] convdiff 256^3 f32={f32} build", flush=True)
+    sstep, first = mek.sstep_plan(50, sstep, first)
+    print(f"[scale] convdiff 256^3 f32={f32} build (sstep {sstep}, first {first})", flush=True)
     rp, ci, va = gen.convdiff_3d(N)
     (lx, dx, ux), (ly, dy, uy), (lz, dz, uz) = gen.convdiff_3d_coeffs()
     v = gen.normal_vector(N ** 3, seed=0)
     R = mek.Runner(mek.CSR.from_arrays(rp, ci, va), m=50, t=t, tol=1e-8, max_restarts=0,
                    step_control="expokit", values_f32=f32, gs_mode=2 if sstep else -1, sstep=sstep,
-                   sstep_first=6)
+                   sstep_first=first)
     del rp, ci, va
     print("[scale] convdiff solve", flush=True)
     y = R.expmv(v)
