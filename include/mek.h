@@ -126,6 +126,8 @@ typedef struct {
     int sstep_blocks;     /* s-step blocks accepted in the last solve        */
     int sstep_second;     /* of them, with a second update or Cholesky pass  */
     int sstep_fallbacks;  /* cycles rebuilt column by column                 */
+    int sstep_fused;      /* blocks done by the fused update alone (Cholesky
+                             QR from the dots, no normalising sweep)        */
 } mek_solve_stats;
 
 typedef struct mek_runner mek_runner;

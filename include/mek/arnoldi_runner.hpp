@@ -121,6 +121,7 @@ struct SolveStats {
   int sstep_blocks = 0;     // s-step blocks accepted
   int sstep_second = 0;     // of them, blocks that took a second update or Cholesky pass
   int sstep_fallbacks = 0;  // cycles rebuilt column by column after a failed block
+  int sstep_fused = 0;      // blocks done by the fused update alone (no normalising sweep)
   std::vector<StepRecord> steps;
   KernelTiming kernels[kNumKinds];
 };
@@ -189,7 +190,7 @@ struct ArnoldiRunner {
 
   bool ss_active_ = false;
   std::vector<double> ss_shifts_;
-  int ss_blocks_before_ = 0, ss_second_before_ = 0;
+  int ss_blocks_before_ = 0, ss_second_before_ = 0, ss_fused_before_ = 0;
   struct EventPair { hipEvent_t a, b; int kind; double bytes; };
   std::vector<EventPair> ev_pool_;
   size_t ev_used_ = 0;

@@ -60,7 +60,7 @@ class SolveStats(ctypes.Structure):
                 ("extra_spmv", ctypes.c_int), ("reorth_columns", ctypes.c_int),
                 ("aborted", ctypes.c_int), ("solve_ms", ctypes.c_double),
                 ("sstep_blocks", ctypes.c_int), ("sstep_second", ctypes.c_int),
-                ("sstep_fallbacks", ctypes.c_int)]
+                ("sstep_fallbacks", ctypes.c_int), ("sstep_fused", ctypes.c_int)]
 
 
 _lib = None

@@ -298,6 +298,7 @@ int mek_runner_stats(const mek_runner* R, mek_solve_stats* o) {
     o->sstep_blocks = r.stats.sstep_blocks;
     o->sstep_second = r.stats.sstep_second;
     o->sstep_fallbacks = r.stats.sstep_fallbacks;
+    o->sstep_fused = r.stats.sstep_fused;
   });
 }
 

@@ -727,8 +727,10 @@ void ArnoldiRunner::solve_on_device(const double* d_v, const double* h_v) {
     MEK_HIP(hipMemcpy(fl, dc.d_ss_flags, sizeof(fl), hipMemcpyDeviceToHost));
     stats.sstep_blocks = fl[mek::kSsBlocks] - ss_blocks_before_;
     stats.sstep_second = fl[mek::kSsSecond] - ss_second_before_;
+    stats.sstep_fused = fl[mek::kSsFusedBlocks] - ss_fused_before_;
     ss_blocks_before_ = fl[mek::kSsBlocks];
     ss_second_before_ = fl[mek::kSsSecond];
+    ss_fused_before_ = fl[mek::kSsFusedBlocks];
   }
 }
 

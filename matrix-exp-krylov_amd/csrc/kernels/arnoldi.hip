@@ -454,8 +454,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
   const long long ld = b.dia_ld;
   const int nd = b.n_diags;
   int off[ND];
+  int d0 = -1;  // the main diagonal's slot: its x loads give the shift's x[r]
 #pragma unroll
-  for (int d = 0; d < ND; ++d) off[d] = (d < nd) ? b.dia_off[d] : 0;
+  for (int d = 0; d < ND; ++d) {
+    off[d] = (d < nd) ? b.dia_off[d] : 0;
+    if (d < nd && off[d] == 0) d0 = d;
+  }
   double* __restrict__ w = wout;
   const long long ngroups = (n + kWave * P - 1) / (kWave * P);
   const long long nw = static_cast<long long>(gridDim.x) * kWavesPerBlock;
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
     double xrr[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) xrr[k] = 0.0;
-    if (has_shift) {
+    if (has_shift && !(full && d0 >= 0)) {
 #pragma unroll
       for (int k = 0; k < P; ++k)
         if (r0 + k < n) xrr[k] = x[r0 + k];
@@ -507,6 +511,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
           }
         }
       }
+    }
+    if (has_shift && full && d0 >= 0) {
+#pragma unroll
+      for (int d = 0; d < ND; ++d)
+        if (d == d0) {
+#pragma unroll
+          for (int k = 0; k < P; ++k) xrr[k] = xv[d][k];
+        }
     }
     double a[P];
 #pragma unroll

@@ -85,7 +85,8 @@ enum SsFlag : int {
   kSsSecond,       // blocks that took a second update or Cholesky pass (statistics)
   kSsFailK,        // k of the block that failed (iscal[kSStepFail] set)
   kSsFused,        // the first update wrote Q_new = (P - Q_k C1) R^-1 (R from the dots)
-  kSsNumFlags = 8
+  kSsFusedBlocks,  // blocks accepted after the fused update alone (statistics, kept across solves)
+  kSsNumFlags = 16
 };
 
 struct SsArgs {

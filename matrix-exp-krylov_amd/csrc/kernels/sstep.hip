@@ -611,6 +611,7 @@ accepted:
     a.fl[kSsNorm2] = 0;
     a.fl[kSsBlocks] += 1;
     if (a.fl[kSsPasses] > 1 || call == 1) a.fl[kSsSecond] += 1;
+    if (phase == kSsPhaseB) a.fl[kSsFusedBlocks] += 1;
   }
 }
 

@@ -143,3 +143,52 @@ def test_sstep_breakdown_in_the_first_columns(gpu):
                    sstep=8)
     assert rel_err(y, np.exp(t * lam) * v) <= 1e-12
     assert st["sstep_blocks"] == 0 and st["sstep_fallbacks"] == 0
+
+
+@pytest.mark.parametrize("which", ["laplacian", "convdiff", "random"])
+def test_fused_cholesky_qr_matches_unfused(gpu, oracle, which):
+    """The fused Cholesky QR (R from the dots by Pythagoras, Q_new written by
+    the update sweep, no normalising sweep) against the unfused blocks
+    (gs_mode 1: second update and normalising sweep every time) and the
+    column path: the same y to 1e-10 (1e-8 on random rows) and the reference
+    to 1e-8, with the fused path taken in some blocks on the stencils (DGKS
+    mode) and in none with gs_mode 1."""
+    if which == "laplacian":
+        N, t, m = 96, 10.0, 40
+        rp, ci, va = gen.laplacian_2d(N)
+        v = gen.normal_vector(N * N, seed=3)
+        ref = laplacian_2d_expmv(N, t, v)
+    elif which == "convdiff":
+        N, t, m = 24, 0.5, 50
+        rp, ci, va = gen.convdiff_3d(N)
+        (lx, dx, ux), (ly, dy, uy), (lz, dz, uz) = gen.convdiff_3d_coeffs()
+        v = gen.normal_vector(N ** 3, seed=4)
+        ref = kron3_expmv(gen.tridiag_1d(N, lz, dz, uz), gen.tridiag_1d(N, ly, dy, uy),
+                          gen.tridiag_1d(N, lx, dx, ux), t, v)
+    else:
+        t, m = 0.5, 40
+        rp, ci, va = gen.random_rows(6000, 5, 30, seed=5)
+        v = gen.normal_vector(6000, seed=5)
+        ref, _st, _log = oracle.expmv(rp, ci, va, v, t, 1e-12, 40, 0)
+    # the bench's shape on the stencils; blocks of 8 on random rows, whose
+    # Newton blocks of 16 are too ill-conditioned to pass the Cholesky test
+    s, first = mek.sstep_plan(m) if which != "random" else (8, 8)
+    kw = dict(m=m, t=t, tol=1e-10, max_restarts=0, step_control="expokit", sstep=s, sstep_first=first)
+    yf, stf = _solve(rp, ci, va, v, gs_mode=2, **kw)
+    yu, stu = _solve(rp, ci, va, v, gs_mode=1, **kw)
+    yc, _ = _solve(rp, ci, va, v, m=m, t=t, tol=1e-10, max_restarts=0, step_control="expokit", gs_mode=2)
+    print(f"[sstep] {which} fused: blocks={stf['sstep_blocks']} fused={stf['sstep_fused']} "
+          f"second={stf['sstep_second']}; unfused: blocks={stu['sstep_blocks']} fused={stu['sstep_fused']}",
+          flush=True)
+    if which != "random":
+        assert stf["sstep_fallbacks"] == 0 and stu["sstep_fallbacks"] == 0
+    assert stu["sstep_fused"] == 0 and stu["sstep_blocks"] > 0 and stf["sstep_blocks"] > 0
+    if which != "random":
+        assert stf["sstep_fused"] > 0
+    assert stf["sstep_fused"] <= stf["sstep_blocks"]
+    # random rows give the worst-conditioned blocks: agreement with the column
+    # path to 1e-8 there (as for H in test_sstep_hessenberg_matches_column_path)
+    agree = 1e-8 if which == "random" else 1e-10
+    for y in (yf, yu):
+        assert rel_err(y, ref) <= 1e-8
+        assert rel_err(y, yc) <= agree
