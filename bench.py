@@ -794,10 +794,15 @@ def run_single(args, mek, gen):
         "solve": {"substeps": stats["accepted"], "expm_attempts": stats["attempts"],
                   "krylov_columns": stats["spmv_count"], "extra_spmv": stats["extra_spmv"],
                   "converged": bool(stats["converged"]), "sstep_blocks": stats["sstep_blocks"],
-                  "sstep_second": stats["sstep_second"], "sstep_fallbacks": stats["sstep_fallbacks"]},
+                  "sstep_second": stats["sstep_second"], "sstep_fallbacks": stats["sstep_fallbacks"],
+                  "sstep_fused": stats["sstep_fused"]},
         "roofline": roofline,
         "spmv": spmv_roof,
         "kernels_ms_per_solve": {k: round(kern[k]["ms"] / tsteps, 3) for k in kern if kern[k]["launches"]},
+        # layout bytes over event time, per kind (a kind whose launches can return at once, such as
+        # the second passes, counts no bytes)
+        "kernels_gbs": {k: round(kern[k]["bytes"] / (kern[k]["ms"] * 1e-3) / 1e9, 1)
+                        for k in kern if kern[k]["launches"] and kern[k]["ms"] > 0},
         # algorithmic bytes of every timed kernel of a solve / the solve's wall time
         "solve_effective_gbs": round(sum(kern[k]["bytes"] for k in kern) / tsteps / (ms_per_step * 1e-3) / 1e9, 1),
         "launch_shapes": tuning,

@@ -68,6 +68,12 @@ int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values)
  * beyond FP32's range; they are included in *count.  No reference
  * counterpart: the check the solvers make for values_f32 = 1. */
 int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel, long long* nonfinite);
+/* The diagonals of A for the diagonal-copy SpMV: *count (0..16) distinct
+ * column - row offsets in ascending order into offsets[16], or *count = 0
+ * when A has more than 16 of them, a row whose columns are not strictly
+ * ascending, or a NaN value.  Host only.  No reference counterpart: the test
+ * the runner makes for spmv_format 0 and 3. */
+int mek_csr_diagonals(const mek_csr* A, int* offsets, int* count);
 void mek_csr_destroy(mek_csr* A);
 
 /* ---------------------------------------------------------------- solver */

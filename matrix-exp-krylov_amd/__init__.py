@@ -86,6 +86,7 @@ SIGNATURES = {
     "mek_csr_load_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "mek_csr_info": (ctypes.c_int, [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_csr_export": (ctypes.c_int, [_vp, _ip, _ip, _dp]),
+    "mek_csr_diagonals": (ctypes.c_int, [_vp, _ip, _ip]),
     "mek_csr_f32_inexact": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _dp,
                                            ctypes.POINTER(ctypes.c_longlong)]),
     "mek_csr_destroy": (None, [_vp]),
@@ -307,6 +308,15 @@ class CSR:
         c, m, b = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_longlong()
         _check(lib().mek_csr_f32_inexact(self.h, ctypes.byref(c), ctypes.byref(m), ctypes.byref(b)))
         return (c.value, m.value, b.value) if nonfinite else (c.value, m.value)
+
+    def diagonals(self):
+        """Ascending column - row offsets of the diagonal-copy SpMV, or []
+        when the matrix has no diagonal form (> 16 diagonals, columns not
+        ascending within a row, or a NaN value)."""
+        offs = np.zeros(16, np.int32)
+        n = ctypes.c_int()
+        _check(lib().mek_csr_diagonals(self.h, _pi(offs), ctypes.byref(n)))
+        return [int(o) for o in offs[:n.value]]
 
     def arrays(self):
         i = self.info()

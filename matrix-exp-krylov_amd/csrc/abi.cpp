@@ -179,6 +179,16 @@ int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel, lon
   });
 }
 
+int mek_csr_diagonals(const mek_csr* A, int* offsets, int* count) {
+  return guarded([&] {
+    need(A && offsets && count, "matrix, outputs");
+    const std::vector<int> offs =
+        mek::dia_offsets(A->A.row_ptr.data(), A->A.col_idx.data(), A->A.values.data(), A->A.rows);
+    *count = static_cast<int>(offs.size());
+    for (size_t i = 0; i < offs.size(); ++i) offsets[i] = offs[i];
+  });
+}
+
 int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values) {
   return guarded([&] {
     need(A, "csr");
