@@ -368,10 +368,11 @@ void ArnoldiRunner::run_sstep_block(int k, int ss) {
   end_timed(mek::kKUpdateNorm, 0.0);
   mek::launch_ss_reduce(a, k + ss, ss, mek::kSsPass2, s);
   mek::launch_ss_plan(a, mek::kSsPhaseB, 1, k, ss, force2, mek::kSsPass2, s);
-  // normalising sweep over the block's own columns
+  // normalising sweep over the block's own columns, only when the fused
+  // update did not already write Q_new (no bytes counted: it may return at once)
   begin_timed(mek::kKLift);
   mek::launch_ss_update(a, k, ss, k, ss, mek::kSsNormReady, s);
-  end_timed(mek::kKLift, 8.0 * rows * 2 * ss);
+  end_timed(mek::kKLift, 0.0);
   mek::launch_ss_reduce(a, ss, ss, mek::kSsNormReady, s);
   mek::launch_ss_plan(a, mek::kSsPhaseD, 0, k, ss, force2, mek::kSsNormReady, s);
   begin_timed(mek::kKUpdateNorm);
