@@ -166,6 +166,10 @@ struct ArnoldiRunner {
 
   // Additive: device-resident input and output (device pointers of n doubles).
   void compute_expmv_device(const double* d_v, double* d_y);
+  // Additive: compute_expmv on host arrays of n doubles (no vector copies).
+  // v and y cross PCIe through a pinned staging buffer of n doubles (kept by
+  // the runner), filled and emptied by several host threads.
+  void compute_expmv_host(const double* v_host, double* y_host);
   // Additive: one Arnoldi decomposition from v (no time stepping): fills
   // H ((m+1) x m, column-major) and returns the columns built.
   int arnoldi_decomposition(const std::vector<double>& v_host, std::vector<double>& H_out);
@@ -191,6 +195,9 @@ struct ArnoldiRunner {
   bool ss_active_ = false;
   std::vector<double> ss_shifts_;
   int ss_blocks_before_ = 0, ss_second_before_ = 0, ss_fused_before_ = 0;
+  void stage_copy(double* dst, const double* src, size_t n);
+  double* h_stage_ = nullptr;  // pinned staging buffer for host v and y
+  size_t h_stage_n_ = 0;
   struct EventPair { hipEvent_t a, b; int kind; double bytes; };
   std::vector<EventPair> ev_pool_;
   size_t ev_used_ = 0;

@@ -262,10 +262,7 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
 int mek_runner_expmv(mek_runner* R, const double* v, double* y) {
   return guarded([&] {
     need(R && v && y, "runner, v, y");
-    const int n = R->R->global_rows;
-    std::vector<double> vh(v, v + n), yh(n, 0.0);
-    R->R->compute_expmv(vh, yh);
-    std::memcpy(y, yh.data(), sizeof(double) * n);
+    R->R->compute_expmv_host(v, y);
   });
 }
 

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <iostream>
 #include <limits>
+#include <thread>
 
 #include <rocprofiler-sdk-roctx/roctx.h>
 
@@ -735,15 +736,48 @@ void ArnoldiRunner::solve_on_device(const double* d_v, const double* h_v) {
   }
 }
 
+// Host copies to and from the pinned staging buffer: one thread per 16 MB
+// (at most 8), so the copy runs near the host's memory bandwidth instead of
+// one core's.
+void ArnoldiRunner::stage_copy(double* dst, const double* src, size_t n) {
+  const size_t per = size_t(2) << 20;  // doubles per thread at least
+  const size_t nt = std::min<size_t>(8, std::max<size_t>(1, n / per));
+  if (nt == 1) {
+    std::copy(src, src + n, dst);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t a = n * t / nt, b = n * (t + 1) / nt;
+    th.emplace_back([=] { std::copy(src + a, src + b, dst + a); });
+  }
+  for (auto& x : th) x.join();
+}
+
 void ArnoldiRunner::compute_expmv(const std::vector<double>& v_host, std::vector<double>& y_host) {
-  const double t0 = now_ms();
   DeviceContext& dc = device_contexts[0];
   if (static_cast<int>(v_host.size()) < dc.local_rows || static_cast<int>(y_host.size()) < dc.local_rows)
     throw mek::Error("ERROR: v and y must hold " + std::to_string(dc.local_rows) + " entries");
-  solve_on_device(nullptr, v_host.data());
-  MEK_HIP(hipMemcpyAsync(y_host.data(), dc.d_y, sizeof(double) * dc.local_rows, hipMemcpyDeviceToHost,
-                         dc.stream_compute));
+  compute_expmv_host(v_host.data(), y_host.data());
+}
+
+void ArnoldiRunner::compute_expmv_host(const double* v_host, double* y_host) {
+  const double t0 = now_ms();
+  DeviceContext& dc = device_contexts[0];
+  const size_t n = static_cast<size_t>(dc.local_rows);
+  MEK_HIP(hipSetDevice(dc.device_id));
+  if (h_stage_n_ < n) {
+    if (h_stage_) (void)hipHostFree(h_stage_);
+    h_stage_ = nullptr;
+    h_stage_n_ = 0;
+    MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_stage_), sizeof(double) * std::max<size_t>(n, 1)));
+    h_stage_n_ = n;
+  }
+  stage_copy(h_stage_, v_host, n);
+  solve_on_device(nullptr, h_stage_);
+  MEK_HIP(hipMemcpyAsync(h_stage_, dc.d_y, sizeof(double) * n, hipMemcpyDeviceToHost, dc.stream_compute));
   MEK_HIP(hipStreamSynchronize(dc.stream_compute));
+  stage_copy(y_host, h_stage_, n);
   if (options.verbose) {
     // reference's analytic FLOP model (main.cu:1538-1565)
     const double N = global_rows, nnz = static_cast<double>(global_nnz), mv = params.m,
@@ -790,6 +824,9 @@ int ArnoldiRunner::arnoldi_decomposition(const std::vector<double>& v_host, std:
 }
 
 void ArnoldiRunner::destroy() {
+  if (h_stage_) (void)hipHostFree(h_stage_);
+  h_stage_ = nullptr;
+  h_stage_n_ = 0;
   for (auto& p : ev_pool_) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
