@@ -27,6 +27,7 @@ struct TuneChoice {
   double spmv_us = 0.0;
   double sweep_us = 0.0;
   int sstep_grid = 0;   // s-step block sweep workgroups (when s-step is on)
+  int sstep_grid_small = 0;  // ... for a cycle's first block when its window is at most 32 columns
   double sstep_us = 0.0;
   bool cached = false;
   std::string log;      // every candidate and its time

@@ -65,6 +65,7 @@ struct DeviceContext {
   int* d_ss_flags = nullptr;
   double* d_ss_partials = nullptr;
   int ss_grid = 0;
+  int ss_grid_small = 0;          // for block windows of up to 32 columns (0: ss_grid)
 
   int local_rows = 0;
   long long local_nnz = 0;

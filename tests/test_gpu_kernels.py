@@ -138,7 +138,7 @@ def _banded_sparse(n, offsets, keep, seed):
 
 
 @pytest.mark.parametrize("which", ["laplacian", "convdiff", "convdiff_f32", "band_holes", "nine", "sixteen",
-                                   "n65", "one_row", "zero_x"])
+                                   "sixteen_f32", "n65", "one_row", "zero_x"])
 def test_spmv_dia_bitexact(gpu, oracle, which):
     """The diagonal copy gives the oracle's CSR-loop bits: stencils with their
     absent boundary entries, FP32 values, bands with holes, explicit (and
@@ -146,7 +146,7 @@ def test_spmv_dia_bitexact(gpu, oracle, which):
     that are not a multiple of 64 or 128, one row, and an x with zeros (where
     an added 0.0 product would turn a -0.0 row sum into +0.0)."""
     print(f"[spmv-dia] {which}", flush=True)
-    f32 = which == "convdiff_f32"
+    f32 = which.endswith("_f32")
     n_diags = None
     if which == "laplacian":
         rp, ci, va = gen.laplacian_2d(77)
@@ -159,8 +159,10 @@ def test_spmv_dia_bitexact(gpu, oracle, which):
     elif which == "nine":
         rp, ci, va = _banded_sparse(2000, [-40, -9, -4, -1, 0, 1, 4, 9, 40], 0.9, seed=4)
         n_diags = 9
-    elif which == "sixteen":
+    elif which.startswith("sixteen"):
         rp, ci, va = _banded_sparse(3001, list(range(-8, 8)), 1.0, seed=5)
+        if f32:
+            va = va.astype(np.float32).astype(np.float64)
         n_diags = 16
     elif which == "n65":
         rp, ci, va = _banded_sparse(65, [-1, 0, 1, 64], 0.8, seed=6)
