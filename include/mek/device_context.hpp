@@ -94,7 +94,7 @@ struct DeviceContext {
   // matrices whose entries lie on at most mek::kDiaMaxDiags diagonals and
   // whose rows list their columns in ascending order (mek::dia_offsets).
   // Returns whether it was built.
-  bool build_dia(const CSRHost& part, const std::vector<int>& offsets);
+  bool build_dia(const std::vector<int>& offsets);
   void free_dia();
   // Additive: the SpMV layout.  format 0: the diagonal copy when the matrix
   // has one with at most mek::kDiaMaxPad padding, else the sliced-ELL copy

@@ -186,7 +186,7 @@ bool DeviceContext::build_sell(const CSRHost& part, int format) {
   return true;
 }
 
-bool DeviceContext::build_dia(const CSRHost& part, const std::vector<int>& offsets) {
+bool DeviceContext::build_dia(const std::vector<int>& offsets) {
   MEK_HIP(hipSetDevice(device_id));
   free_dia();
   if (offsets.empty() || local_rows <= 0) return false;
@@ -197,7 +197,6 @@ bool DeviceContext::build_dia(const CSRHost& part, const std::vector<int>& offse
   MEK_HIP(hipMalloc(&d_dia_val, vb * static_cast<size_t>(ldd) * nd));
   MEK_HIP(hipMalloc(&d_dia_off, sizeof(int) * nd));
   MEK_HIP(hipMemcpy(d_dia_off, offsets.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
-  (void)part;
   mek::launch_dia_build(d_row_ptr, d_col_idx, d_values, values_f32 ? 1 : 0, local_rows, d_dia_off, nd, ldd,
                         d_dia_val, stream_compute);
   MEK_HIP_LAUNCHED();
@@ -236,7 +235,7 @@ int DeviceContext::build_spmv_layout(const CSRHost& part, int format) {
     const bool small_pad = !offs.empty() && static_cast<double>(offs.size()) * local_rows <=
                                                 (1.0 + mek::kDiaMaxPad) * static_cast<double>(local_nnz);
     if (format == 3 || small_pad) {
-      build_dia(part, offs);
+      build_dia(offs);
       return 3;
     }
   }
