@@ -730,7 +730,8 @@ def run_single(args, mek, gen):
     log(f"relative error {rel:.3e}")
 
     # The SpMV's bytes are those of the layout that runs (the runner's count:
-    # the diagonal copy reads n values per diagonal and no column index; the
+    # the diagonal copy reads n values per diagonal and no column index, the
+    # constant-diagonal form a presence mask per row and no value; the
     # sliced-ELL copy its padded entries), the least the kernel must move; the
     # CSR figure nnz (value + 4) + 4 (n + 1) + 8 n (x once) + 8 n (w) is kept
     # beside it with the bandwidth it would imply.
@@ -836,8 +837,10 @@ def main():
     ap.add_argument("--sstep", type=int, default=-1,
                     help="s-step blocks of this many basis vectors (0: column by column, as the reference; "
                          "-1: equal blocks of at most 16 after the first 8 columns)")
-    ap.add_argument("--spmv-format", default="auto", choices=["auto", "csr", "sell", "dia"],
-                    help="SpMV layout: the diagonal copy for stencils, else sliced ELL where its padding is small (auto); CSR row blocks; sliced ELL; diagonal copy")
+    ap.add_argument("--spmv-format", default="auto", choices=["auto", "csr", "sell", "dia", "dia_const"],
+                    help="SpMV layout: the diagonal layout for stencils (constant diagonals when each holds one "
+                         "value), else sliced ELL where its padding is small (auto); CSR row blocks; sliced ELL; "
+                         "diagonal copy with per-entry values; constant diagonals")
     ap.add_argument("--sstep-first", type=int, default=-1,
                     help="columns of each cycle built one by one before the blocks (-1: 8 with --sstep -1, "
                          "else --sstep)")

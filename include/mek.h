@@ -74,6 +74,11 @@ int mek_csr_f32_inexact(const mek_csr* A, long long* count, double* max_rel, lon
  * ascending, or a NaN value.  Host only.  No reference counterpart: the test
  * the runner makes for spmv_format 0 and 3. */
 int mek_csr_diagonals(const mek_csr* A, int* offsets, int* count);
+/* The value of each of those diagonals into values[16] (offset order), when
+ * every entry on a diagonal has the same bits (after rounding to FP32 when
+ * f32 != 0), else *count = 0; the test the runner makes for spmv_format 0 and
+ * 4.  Host only.  No reference counterpart. */
+int mek_csr_diagonal_values(const mek_csr* A, int f32, double* values, int* count);
 void mek_csr_destroy(mek_csr* A);
 
 /* ---------------------------------------------------------------- solver */
@@ -108,10 +113,13 @@ typedef struct {
                            0: column by column); needs m + 1 <= 64        */
     int sstep_first;    /* columns of each cycle built one by one before the
                            blocks (-1: sstep)                               */
-    int spmv_format;    /* 0 diagonal copy for entries on <= 16 diagonals
-                           with little padding, else sliced ELL when its
+    int spmv_format;    /* 0 diagonal layout for entries on <= 16 diagonals
+                           with little padding (constant diagonals when each
+                           holds one value), else sliced ELL when its
                            padding is small, else CSR row blocks; 1 CSR row
-                           blocks; 2 sliced ELL; 3 diagonal copy            */
+                           blocks; 2 sliced ELL; 3 diagonal copy with
+                           per-entry values; 4 constant diagonals: one value
+                           per diagonal and a presence mask per row          */
 } mek_solver_options;
 
 void mek_solver_options_default(mek_solver_options* opt);
@@ -175,8 +183,9 @@ int mek_runner_tuning(const mek_runner* R, int* nnz_cap, int* spmv_grid, int* sw
  * with autotune, the timed dots + update pair of the chosen grid (us). */
 int mek_runner_sstep_grid(const mek_runner* R, int* grid, double* us);
 /* SpMV layout in use: *format 1 CSR row blocks, 2 sliced ELL, 3 diagonal
- * copy; for sliced ELL the padded entry count and the widest row, for the
- * diagonal copy rows x diagonals and the diagonal count (else 0). */
+ * copy, 4 constant diagonals; for sliced ELL the padded entry count and the
+ * widest row, for the diagonal layouts rows x diagonals and the diagonal
+ * count (else 0). */
 int mek_runner_spmv_format(const mek_runner* R, int* format, long long* entries, int* width);
 /* The runner's stream (hipStream_t). */
 int mek_runner_stream(const mek_runner* R, void** stream);
@@ -276,6 +285,11 @@ int mek_spmv_plan_use_sell(mek_spmv_plan* P, const int* h_row_ptr, long long* en
  * algorithm). */
 int mek_spmv_plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx, const double* h_vals,
                           int* n_diags);
+/* The same with the constant-diagonal form (one value per diagonal, a
+ * presence mask per row; no per-entry values are read): refused when a
+ * diagonal holds two values.  Same y bits.  No reference counterpart. */
+int mek_spmv_plan_use_dia_const(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx,
+                                const double* h_vals, int* n_diags);
 void mek_spmv_plan_destroy(mek_spmv_plan* P);
 
 /* d_E = exp(d_A) for a dense k x k column-major matrix on the device (one

@@ -59,6 +59,10 @@ struct DeviceContext {
   int* d_dia_off = nullptr;         // column - row of each diagonal, ascending
   int n_diags = 0;
   long long dia_ld = 0;             // rows rounded up to 64
+  // constant-diagonal form (build_dia with values; d_dia_val stays null):
+  // one value per diagonal and a presence mask per row
+  void* d_dia_mask = nullptr;       // dia_ld entries: uint8 (n_diags <= 8) or uint16, bit d = diagonal d
+  double* d_dia_cval = nullptr;     // n_diags values
 
   // s-step blocks (allocated by alloc_sstep): state, flags, sweep partials
   double* d_ss_state = nullptr;
@@ -93,14 +97,18 @@ struct DeviceContext {
   // Additive: the diagonal copy of the device CSR (after alloc_from), for
   // matrices whose entries lie on at most mek::kDiaMaxDiags diagonals and
   // whose rows list their columns in ascending order (mek::dia_offsets).
-  // Returns whether it was built.
-  bool build_dia(const std::vector<int>& offsets);
+  // With cvals (one value per diagonal, from mek::dia_constants) it builds
+  // the constant-diagonal form instead: presence masks and the values, no
+  // per-entry copy.  Returns whether it was built.
+  bool build_dia(const std::vector<int>& offsets, const std::vector<double>& cvals = {});
   void free_dia();
-  // Additive: the SpMV layout.  format 0: the diagonal copy when the matrix
-  // has one with at most mek::kDiaMaxPad padding, else the sliced-ELL copy
-  // under build_sell's rule, else the CSR row blocks; 1 CSR row blocks; 2
-  // sliced ELL; 3 diagonal (an error when the matrix has no diagonal form).
-  // Returns the layout in use (1, 2 or 3).
+  // Additive: the SpMV layout.  format 0: the diagonal layout when the
+  // matrix has one with at most mek::kDiaMaxPad padding (its constant form
+  // when every diagonal holds one value), else the sliced-ELL copy under
+  // build_sell's rule, else the CSR row blocks; 1 CSR row blocks; 2 sliced
+  // ELL; 3 diagonal copy with per-entry values; 4 constant diagonals (3 and
+  // 4 are errors when the matrix has no such form).  Returns the layout in
+  // use (1, 2, 3 or 4).
   int build_spmv_layout(const CSRHost& part, int format);
   // Additive: buffers of the s-step block path (sstep.hip), shifts uploaded.
   void alloc_sstep(const std::vector<double>& shifts);
@@ -132,4 +140,10 @@ constexpr int kSellMaxWidth = 256;    // ... and no row longer than this
 constexpr int kDiaMaxDiags = 16;
 constexpr double kDiaMaxPad = 0.2;    // automatic choice: n * diagonals <= 1.2 nnz
 std::vector<int> dia_offsets(const int* row_ptr, const int* col_idx, const double* values, int n);
+// Constant-diagonal form: the value of each diagonal of `offsets` (from
+// dia_offsets) when all entries on it have the same bits as stored on the
+// device (rounded to FP32 first when f32), else empty.  The SpMV multiplies
+// by that value, so the products are those of the per-entry copy.
+std::vector<double> dia_constants(const int* row_ptr, const int* col_idx, const double* values, int n,
+                                  const std::vector<int>& offsets, bool f32);
 }  // namespace mek

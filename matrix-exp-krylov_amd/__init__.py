@@ -26,8 +26,8 @@ KERNEL_KINDS = {"spmv": 0, "update_dot": 1, "update_norm": 2, "dots": 3, "final"
 
 
 STEP_CONTROL = {"reference": 0, "expokit": 1, "expokit_rel": 2}
-SPMV_FORMAT = {"auto": 0, "csr": 1, "sell": 2, "dia": 3}
-SPMV_FORMAT_NAME = {1: "csr", 2: "sell", 3: "dia"}
+SPMV_FORMAT = {"auto": 0, "csr": 1, "sell": 2, "dia": 3, "dia_const": 4}
+SPMV_FORMAT_NAME = {1: "csr", 2: "sell", 3: "dia", 4: "dia_const"}
 
 
 class MekError(RuntimeError):
@@ -87,6 +87,7 @@ SIGNATURES = {
     "mek_csr_info": (ctypes.c_int, [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_csr_export": (ctypes.c_int, [_vp, _ip, _ip, _dp]),
     "mek_csr_diagonals": (ctypes.c_int, [_vp, _ip, _ip]),
+    "mek_csr_diagonal_values": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _ip]),
     "mek_csr_f32_inexact": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _dp,
                                            ctypes.POINTER(ctypes.c_longlong)]),
     "mek_csr_destroy": (None, [_vp]),
@@ -139,6 +140,7 @@ SIGNATURES = {
     "mek_spmv_plan_destroy": (None, [_vp]),
     "mek_spmv_plan_use_sell": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_spmv_plan_use_dia": (ctypes.c_int, [_vp, _ip, _ip, _dp, _ip]),
+    "mek_spmv_plan_use_dia_const": (ctypes.c_int, [_vp, _ip, _ip, _dp, _ip]),
     "mek_runner_spmv_format": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
     "mek_expm_dense": (ctypes.c_int, [ctypes.c_int, _vp, _vp]),
     "mek_expm_dense_noalloc": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp]),
@@ -317,6 +319,16 @@ class CSR:
         n = ctypes.c_int()
         _check(lib().mek_csr_diagonals(self.h, _pi(offs), ctypes.byref(n)))
         return [int(o) for o in offs[:n.value]]
+
+    def diagonal_values(self, f32=False):
+        """The one value of each diagonal (in diagonals() order) of the
+        constant-diagonal SpMV, or [] when some diagonal holds two values (bit
+        patterns, after rounding to FP32 with f32=True) or the matrix has no
+        diagonal form."""
+        vals = np.zeros(16, np.float64)
+        n = ctypes.c_int()
+        _check(lib().mek_csr_diagonal_values(self.h, 1 if f32 else 0, _pd(vals), ctypes.byref(n)))
+        return [float(v) for v in vals[:n.value]]
 
     def arrays(self):
         i = self.info()
@@ -633,6 +645,15 @@ class SpmvPlan:
         count (MekError when the matrix has no diagonal form)."""
         nd = ctypes.c_int()
         _check(lib().mek_spmv_plan_use_dia(self.h, _pi(self._rp), _pi(self._ci), _pd(self._va), ctypes.byref(nd)))
+        return nd.value
+
+    def use_dia_const(self):
+        """Switch to the constant-diagonal form (one value per diagonal, a
+        presence mask per row); returns the diagonal count (MekError when a
+        diagonal holds two values or the matrix has no diagonal form)."""
+        nd = ctypes.c_int()
+        _check(lib().mek_spmv_plan_use_dia_const(self.h, _pi(self._rp), _pi(self._ci), _pd(self._va),
+                                                 ctypes.byref(nd)))
         return nd.value
 
     def use_sell(self):

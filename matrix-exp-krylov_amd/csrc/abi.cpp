@@ -50,6 +50,8 @@ struct mek_spmv_plan {
   void* d_sell_val = nullptr;
   void* d_dia_val = nullptr;
   int* d_dia_off = nullptr;
+  void* d_dia_mask = nullptr;
+  double* d_dia_cval = nullptr;
 };
 
 namespace {
@@ -189,6 +191,18 @@ int mek_csr_diagonals(const mek_csr* A, int* offsets, int* count) {
   });
 }
 
+int mek_csr_diagonal_values(const mek_csr* A, int f32, double* values, int* count) {
+  return guarded([&] {
+    need(A && values && count, "matrix, outputs");
+    const std::vector<int> offs =
+        mek::dia_offsets(A->A.row_ptr.data(), A->A.col_idx.data(), A->A.values.data(), A->A.rows);
+    const std::vector<double> cv =
+        mek::dia_constants(A->A.row_ptr.data(), A->A.col_idx.data(), A->A.values.data(), A->A.rows, offs, f32 != 0);
+    *count = static_cast<int>(cv.size());
+    for (size_t i = 0; i < cv.size(); ++i) values[i] = cv[i];
+  });
+}
+
 int mek_csr_export(const mek_csr* A, int* row_ptr, int* col_idx, double* values) {
   return guarded([&] {
     need(A, "csr");
@@ -246,7 +260,7 @@ int mek_runner_create(const mek_csr* A, const mek_arnoldi_params* p, const mek_s
       need(o->sstep_first >= -1, "sstep_first >= -1");
       so.sstep = o->sstep;
       so.sstep_first = o->sstep_first;
-      need(o->spmv_format >= 0 && o->spmv_format <= 3, "spmv_format in {0, 1, 2, 3}");
+      need(o->spmv_format >= 0 && o->spmv_format <= 4, "spmv_format in {0, 1, 2, 3, 4}");
       so.spmv_format = o->spmv_format;
     }
     try {
@@ -355,7 +369,7 @@ int mek_runner_spmv_format(const mek_runner* R, int* format, long long* entries,
     need(R && R->R && format, "runner, format");
     const DeviceContext& dc = R->R->device_contexts[0];
     if (dc.n_diags > 0) {
-      *format = 3;
+      *format = dc.d_dia_mask ? 4 : 3;
       if (entries) *entries = static_cast<long long>(dc.n_diags) * dc.local_rows;
       if (width) *width = dc.n_diags;
       return;
@@ -647,16 +661,21 @@ int mek_spmv_plan_use_sell(mek_spmv_plan* P, const int* h_row_ptr, long long* en
   });
 }
 
-int mek_spmv_plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx, const double* h_vals,
-                          int* n_diags) {
+static int plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx, const double* h_vals,
+                        int* n_diags, bool constant) {
   return guarded([&] {
     need(P && h_row_ptr && (P->b.n == 0 || (h_col_idx && h_vals)), "plan, h_row_ptr, h_col_idx, h_vals");
     mek::ArnoldiBufs& b = P->b;
-    need(P->d_dia_val == nullptr, "the plan already uses the diagonal copy");
+    need(b.n_diags == 0, "the plan already uses a diagonal layout");
     const std::vector<int> offs = mek::dia_offsets(h_row_ptr, h_col_idx, h_vals, b.n);
     if (offs.empty() || b.n <= 0)
       throw InvalidArgument("the matrix has no diagonal form (more than 16 diagonals, columns not ascending "
                                  "within a row, or NaN values)");
+    std::vector<double> cv;
+    if (constant) {
+      cv = mek::dia_constants(h_row_ptr, h_col_idx, h_vals, b.n, offs, b.vals_f32 != 0);
+      if (cv.empty()) throw InvalidArgument("the matrix has no constant-diagonal form (a diagonal holds two values)");
+    }
     const int nd = static_cast<int>(offs.size());
     const long long ldd = (static_cast<long long>(b.n) + mek::kWave - 1) / mek::kWave * mek::kWave;
     const long long slices = ldd / mek::kWave;
@@ -668,15 +687,25 @@ int mek_spmv_plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_c
       P->d_partials = d_part;
       b.partials = d_part;
     }
-    MEK_HIP(hipMalloc(&P->d_dia_val, (b.vals_f32 ? sizeof(float) : sizeof(double)) * static_cast<size_t>(ldd) * nd));
     MEK_HIP(hipMalloc(&P->d_dia_off, sizeof(int) * nd));
     MEK_HIP(hipMemcpy(P->d_dia_off, offs.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
-    mek::launch_dia_build(b.row_ptr, b.col_idx, b.vals, b.vals_f32, b.n, P->d_dia_off, nd, ldd, P->d_dia_val,
-                          nullptr);
+    if (constant) {
+      MEK_HIP(hipMalloc(&P->d_dia_mask, (nd <= 8 ? 1 : 2) * static_cast<size_t>(ldd)));
+      MEK_HIP(hipMalloc(&P->d_dia_cval, sizeof(double) * nd));
+      MEK_HIP(hipMemcpy(P->d_dia_cval, cv.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+      mek::launch_dia_mask_build(b.row_ptr, b.col_idx, b.n, P->d_dia_off, nd, ldd, P->d_dia_mask, nullptr);
+    } else {
+      MEK_HIP(hipMalloc(&P->d_dia_val,
+                        (b.vals_f32 ? sizeof(float) : sizeof(double)) * static_cast<size_t>(ldd) * nd));
+      mek::launch_dia_build(b.row_ptr, b.col_idx, b.vals, b.vals_f32, b.n, P->d_dia_off, nd, ldd, P->d_dia_val,
+                            nullptr);
+    }
     MEK_HIP_LAUNCHED();
     MEK_HIP(hipDeviceSynchronize());
     b.dia_val = P->d_dia_val;
     b.dia_off = P->d_dia_off;
+    b.dia_mask = P->d_dia_mask;
+    b.dia_cval = P->d_dia_cval;
     b.n_diags = nd;
     b.dia_ld = ldd;
     b.spmv_grid = grid;
@@ -684,10 +713,22 @@ int mek_spmv_plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_c
   });
 }
 
+int mek_spmv_plan_use_dia(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx, const double* h_vals,
+                          int* n_diags) {
+  return plan_use_dia(P, h_row_ptr, h_col_idx, h_vals, n_diags, false);
+}
+
+int mek_spmv_plan_use_dia_const(mek_spmv_plan* P, const int* h_row_ptr, const int* h_col_idx,
+                                const double* h_vals, int* n_diags) {
+  return plan_use_dia(P, h_row_ptr, h_col_idx, h_vals, n_diags, true);
+}
+
 void mek_spmv_plan_destroy(mek_spmv_plan* P) {
   if (!P) return;
   if (P->d_dia_val) (void)hipFree(P->d_dia_val);
   if (P->d_dia_off) (void)hipFree(P->d_dia_off);
+  if (P->d_dia_mask) (void)hipFree(P->d_dia_mask);
+  if (P->d_dia_cval) (void)hipFree(P->d_dia_cval);
   if (P->d_sell_ptr) (void)hipFree(P->d_sell_ptr);
   if (P->d_sell_col) (void)hipFree(P->d_sell_col);
   if (P->d_sell_val) (void)hipFree(P->d_sell_val);

@@ -110,10 +110,14 @@ std::vector<double> sstep_shifts(const CSRHost& A, int s) {
 
 // Bytes one SpMV streams: the matrix in its layout (CSR: values, column
 // indices, row pointers; sliced ELL: padded values and column indices, slice
-// offsets; diagonal: n values per diagonal and the offsets), x once and w.  csr = true: the CSR figure whatever the layout.
+// offsets; diagonal: n values per diagonal and the offsets; constant
+// diagonals: one presence mask per row, the offsets and values), x once and
+// w.  csr = true: the CSR figure whatever the layout.
 double spmv_bytes(const DeviceContext& dc, bool csr = false) {
   const double vb = dc.values_f32 ? 4.0 : 8.0;
   const double n = dc.local_rows;
+  if (dc.n_diags > 0 && !csr && dc.d_dia_mask)
+    return (dc.n_diags <= 8 ? 1.0 : 2.0) * n + 12.0 * dc.n_diags + 16.0 * n;
   if (dc.n_diags > 0 && !csr)
     return static_cast<double>(dc.n_diags) * n * vb + 4.0 * dc.n_diags + 16.0 * n;
   if (dc.n_slices > 0 && !csr)
@@ -643,8 +647,7 @@ void ArnoldiRunner::expokit_controller() {
       begin_timed(mek::kKSpmv);
       mek::launch_spmv_plain(b, dc.d_V_m + static_cast<long long>(m) * dc.ld, b.colscale + m, dc.d_w,
                              dc.stream_compute);
-      end_timed(mek::kKSpmv, static_cast<double>(dc.local_nnz) * ((dc.values_f32 ? 4 : 8) + 4) +
-                                 4.0 * (dc.local_rows + 1) + 16.0 * dc.local_rows);
+      end_timed(mek::kKSpmv, spmv_bytes(dc));
       mek::launch_fin_norm(b, mek::kAvNorm, b.spmv_grid, dc.stream_compute);
       MEK_HIP_LAUNCHED();
       stats.extra_spmv++;

@@ -67,7 +67,7 @@ void upload_blocks(DeviceContext& dc, const std::vector<int>& blocks) {
 
 TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
   // sliced-ELL and diagonal copies: the SpMV grid only (no row blocks)
-  const int layout = dc.n_diags > 0 ? 3 : dc.n_slices > 0 ? 2 : 1;
+  const int layout = dc.n_diags > 0 ? (dc.d_dia_mask ? 4 : 3) : dc.n_slices > 0 ? 2 : 1;
   const bool copy = layout != 1;
   const Key key{dc.device_id, A.rows, A.nnz, m, dc.values_f32 ? 1 : 0, layout};
   std::ostringstream log;
@@ -97,7 +97,8 @@ TuneChoice autotune_single(DeviceContext& dc, const CSRHost& A, int m) {
         },
         s, 5, &us);
     for (size_t c = 0; c < grids.size(); ++c)
-      log << (layout == 3 ? "dia" : "sell") << " spmv grid=" << grids[c] << " " << us[c] << "us; ";
+      log << (layout == 4 ? "dia_const" : layout == 3 ? "dia" : "sell") << " spmv grid=" << grids[c] << " "
+          << us[c] << "us; ";
     best.nnz_cap = kSpmvNnzCap;
     best.spmv_grid = grids[i];
     best.spmv_us = us[i];

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <iostream>
 
@@ -65,6 +66,36 @@ std::vector<int> dia_offsets(const int* row_ptr, const int* col_idx, const doubl
   }
   std::sort(offs.begin(), offs.end());
   return offs;
+}
+
+std::vector<double> dia_constants(const int* row_ptr, const int* col_idx, const double* values, int n,
+                                  const std::vector<int>& offsets, bool f32) {
+  const int nd = static_cast<int>(offsets.size());
+  if (nd == 0 || nd > kDiaMaxDiags) return {};
+  std::vector<uint64_t> bits(nd, 0);
+  std::vector<char> seen(nd, 0);
+  for (int r = 0; r < n; ++r) {
+    for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+      const long long o = static_cast<long long>(col_idx[e]) - r;
+      // rows list the diagonals in ascending order: start at the entry's rank
+      int d = std::min(e - row_ptr[r], nd - 1);
+      while (d > 0 && offsets[d] > o) --d;
+      while (d < nd - 1 && offsets[d] < o) ++d;
+      if (offsets[d] != o) return {};
+      const double v = f32 ? static_cast<double>(static_cast<float>(values[e])) : values[e];
+      uint64_t b;
+      std::memcpy(&b, &v, sizeof b);
+      if (!seen[d]) {
+        seen[d] = 1;
+        bits[d] = b;
+      } else if (bits[d] != b) {
+        return {};
+      }
+    }
+  }
+  std::vector<double> out(nd);
+  for (int d = 0; d < nd; ++d) std::memcpy(&out[d], &bits[d], sizeof(double));  // unseen: +0.0, never read
+  return out;
 }
 
 // Blocks of the streaming kernels: up to 4096 (16 per CU, measured to need
@@ -186,19 +217,29 @@ bool DeviceContext::build_sell(const CSRHost& part, int format) {
   return true;
 }
 
-bool DeviceContext::build_dia(const std::vector<int>& offsets) {
+bool DeviceContext::build_dia(const std::vector<int>& offsets, const std::vector<double>& cvals) {
   MEK_HIP(hipSetDevice(device_id));
   free_dia();
   if (offsets.empty() || local_rows <= 0) return false;
   const int nd = static_cast<int>(offsets.size());
   if (nd > mek::kDiaMaxDiags) throw mek::Error("ERROR: more than 16 diagonals");
+  if (!cvals.empty() && cvals.size() != offsets.size())
+    throw mek::Error("ERROR: one constant per diagonal");
   const long long ldd = (static_cast<long long>(local_rows) + mek::kWave - 1) / mek::kWave * mek::kWave;
-  const size_t vb = values_f32 ? sizeof(float) : sizeof(double);
-  MEK_HIP(hipMalloc(&d_dia_val, vb * static_cast<size_t>(ldd) * nd));
   MEK_HIP(hipMalloc(&d_dia_off, sizeof(int) * nd));
   MEK_HIP(hipMemcpy(d_dia_off, offsets.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
-  mek::launch_dia_build(d_row_ptr, d_col_idx, d_values, values_f32 ? 1 : 0, local_rows, d_dia_off, nd, ldd,
-                        d_dia_val, stream_compute);
+  if (cvals.empty()) {
+    const size_t vb = values_f32 ? sizeof(float) : sizeof(double);
+    MEK_HIP(hipMalloc(&d_dia_val, vb * static_cast<size_t>(ldd) * nd));
+    mek::launch_dia_build(d_row_ptr, d_col_idx, d_values, values_f32 ? 1 : 0, local_rows, d_dia_off, nd, ldd,
+                          d_dia_val, stream_compute);
+  } else {
+    const size_t mb = nd <= 8 ? 1 : 2;
+    MEK_HIP(hipMalloc(&d_dia_mask, mb * static_cast<size_t>(ldd)));
+    MEK_HIP(hipMalloc(&d_dia_cval, sizeof(double) * nd));
+    MEK_HIP(hipMemcpy(d_dia_cval, cvals.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+    mek::launch_dia_mask_build(d_row_ptr, d_col_idx, local_rows, d_dia_off, nd, ldd, d_dia_mask, stream_compute);
+  }
   MEK_HIP_LAUNCHED();
   MEK_HIP(hipStreamSynchronize(stream_compute));
   n_diags = nd;
@@ -215,28 +256,38 @@ bool DeviceContext::build_dia(const std::vector<int>& offsets) {
 void DeviceContext::free_dia() {
   if (d_dia_val) (void)hipFree(d_dia_val);
   if (d_dia_off) (void)hipFree(d_dia_off);
+  if (d_dia_mask) (void)hipFree(d_dia_mask);
+  if (d_dia_cval) (void)hipFree(d_dia_cval);
   d_dia_val = nullptr;
   d_dia_off = nullptr;
+  d_dia_mask = nullptr;
+  d_dia_cval = nullptr;
   n_diags = 0;
   dia_ld = 0;
 }
 
 int DeviceContext::build_spmv_layout(const CSRHost& part, int format) {
-  if (format < 0 || format > 3) throw mek::Error("ERROR: spmv_format must be 0, 1, 2 or 3");
+  if (format < 0 || format > 4) throw mek::Error("ERROR: spmv_format must be 0, 1, 2, 3 or 4");
   free_dia();
   free_sell();
   if (format == 1) return 1;
-  if (format == 0 || format == 3) {
+  if (format == 0 || format == 3 || format == 4) {
     const std::vector<int> offs =
         mek::dia_offsets(part.row_ptr.data(), part.col_idx.data(), part.values.data(), local_rows);
-    if (format == 3 && (offs.empty() || local_rows <= 0))
+    if (format >= 3 && (offs.empty() || local_rows <= 0))
       throw mek::Error("ERROR: the matrix has no diagonal form (more than 16 diagonals, columns not "
                        "ascending within a row, or NaN values)");
     const bool small_pad = !offs.empty() && static_cast<double>(offs.size()) * local_rows <=
                                                 (1.0 + mek::kDiaMaxPad) * static_cast<double>(local_nnz);
-    if (format == 3 || small_pad) {
-      build_dia(offs);
-      return 3;
+    if (format >= 3 || small_pad) {
+      std::vector<double> cv;
+      if (format != 3)
+        cv = mek::dia_constants(part.row_ptr.data(), part.col_idx.data(), part.values.data(), local_rows, offs,
+                                values_f32);
+      if (format == 4 && cv.empty())
+        throw mek::Error("ERROR: the matrix has no constant-diagonal form (a diagonal holds two values)");
+      build_dia(offs, cv);
+      return cv.empty() ? 3 : 4;
     }
   }
   return build_sell(part, format == 2 ? 2 : 0) ? 2 : 1;
@@ -288,6 +339,8 @@ mek::ArnoldiBufs DeviceContext::bufs() const {
   b.dia_off = d_dia_off;
   b.n_diags = n_diags;
   b.dia_ld = dia_ld;
+  b.dia_mask = d_dia_mask;
+  b.dia_cval = d_dia_cval;
   return b;
 }
 

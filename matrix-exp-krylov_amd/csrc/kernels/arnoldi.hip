@@ -26,6 +26,7 @@
 // block order.  After a breakdown every later kernel of the attempt returns at
 // once, so the host can enqueue all m columns without reading back.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "launch.hpp"
@@ -390,17 +391,29 @@ __global__ __launch_bounds__(kBlock) void k_sell_build(const int* __restrict__ r
 // for each diagonal a lane loads P consecutive values (one 16-byte load for
 // P = 2 FP64 values, non-temporal) and the P consecutive x entries
 // x[r + off_d] (one 16-byte load whatever the offset's parity), the rows' x
-// reads overlapping across diagonals and served by the caches; all loads of a
-// step are issued together.  The waves stride over the 64 P-row groups
-// together in XCD-aware block order, as in k_spmv_sell.  Each row is summed
-// diagonal by diagonal in ascending offset order, which is its CSR order (the
-// copy exists only for rows with ascending columns), from 0.0 with separately
-// rounded products, absent entries skipped: w has the bits of the CSR loop.
+// reads overlapping across diagonals and served by the caches.  The waves
+// stride over the 64 P-row groups together in XCD-aware block order, as in
+// k_spmv_sell.  Each row is summed diagonal by diagonal in ascending offset
+// order, which is its CSR order (the copy exists only for rows with
+// ascending columns), from 0.0 with separately rounded products, absent
+// entries skipped: w has the bits of the CSR loop.
+// The kernel is bound by the latency of a group's loads more than by bytes
+// (on the 4096^2 Laplacian a one-diagonal matrix took as long as the five
+// diagonals): so a group's loads are issued before the previous group's
+// store (a wait for loads also waits for every older store), the interior
+// groups load without a branch, and the 3-, 5- and 7-diagonal matrices get
+// kernels of their exact count (EX), whose fewer registers keep more waves
+// resident (the constant 5-diagonal form 77 -> 66 us per launch).
 // P > 1 stores w in 8P-byte groups and needs wout 16-byte aligned (measured
 // on the 4096^2 Laplacian in isolation: 139 us against 156-163 us for one row
 // per lane; with FP32 values on the 256^3 convection-diffusion matrix P = 4,
 // whose value loads are 16 bytes too, took 167 us in the solve against 159 us
-// for P = 2).
+// for P = 2; P = 4 for the constant form: 76 against 77 us).
+// CV: the constant-diagonal form (ArnoldiBufs::dia_mask / dia_cval).  No
+// value is read: a lane loads its P rows' presence masks (P bytes, or 2P)
+// and multiplies by the diagonal's one value, so a row costs its mask, its w
+// and its share of x instead of 8 (or 4) bytes per diagonal more; the
+// products and their order are those above, so w keeps the CSR loop's bits.
 template <typename VT>
 __device__ __forceinline__ bool dia_absent(VT v);
 template <>
@@ -430,8 +443,27 @@ template <typename VT>
 struct DiaVec<VT, 1> {
   typedef VT type;
 };
+// The presence masks of P consecutive rows in one load.
+template <int BYTES>
+struct DiaMaskLoad;
+template <>
+struct DiaMaskLoad<1> {
+  typedef unsigned char type;
+};
+template <>
+struct DiaMaskLoad<2> {
+  typedef unsigned short type;
+};
+template <>
+struct DiaMaskLoad<4> {
+  typedef unsigned int type;
+};
+template <>
+struct DiaMaskLoad<8> {
+  typedef unsigned long long type;
+};
 
-template <typename VT, int ND, int P>
+template <typename VT, int ND, int P, bool CV, bool EX>
 __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double* x_in, const double* xs_ptr,
                                                      double* wout, int ndots, int normq, int check_breakdown,
                                                      int i0, const double* shift_ptr) {
@@ -440,6 +472,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
   __shared__ double red[kWavesPerBlock * 2];
   typedef typename DiaVec<VT, P>::type VTP;
   typedef typename DiaVec<double, P>::type DP;
+  typedef typename std::conditional<(ND <= 8), unsigned char, unsigned short>::type MT;
+  typedef typename DiaMaskLoad<static_cast<int>(sizeof(MT)) * P>::type MTP;
   const bool has_shift = shift_ptr != nullptr;
   const double sh = has_shift ? *shift_ptr : 0.0;
   const int lb = xcd_block_id();
@@ -450,99 +484,140 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
   const double cs0 = ndots ? b.colscale[i0] : 0.0;
   const double* __restrict__ Vd = b.V + static_cast<long long>(i0) * b.ld;
   const VT* __restrict__ dv = static_cast<const VT*>(b.dia_val);
+  const MT* __restrict__ mp = static_cast<const MT*>(b.dia_mask);
   const long long n = b.n;
   const long long ld = b.dia_ld;
-  const int nd = b.n_diags;
+  const int nd = EX ? ND : b.n_diags;  // EX: the matrix has exactly ND diagonals
   int off[ND];
-  int d0 = -1;  // the main diagonal's slot: its x loads give the shift's x[r]
+  double cv[ND];  // CV: the diagonals' values
+  int d0 = -1;    // the main diagonal's slot: its x loads give the shift's x[r]
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
     off[d] = (d < nd) ? b.dia_off[d] : 0;
+    cv[d] = (CV && d < nd) ? b.dia_cval[d] : 0.0;
     if (d < nd && off[d] == 0) d0 = d;
   }
   double* __restrict__ w = wout;
   const long long ngroups = (n + kWave * P - 1) / (kWave * P);
   const long long nw = static_cast<long long>(gridDim.x) * kWavesPerBlock;
   double acc[2] = {0.0, 0.0};  // q_i0 . w, ||w||^2
-  for (long long g = static_cast<long long>(lb) * kWavesPerBlock + wave; g < ngroups; g += nw) {
-    const long long r0 = (g * kWave + lane) * P;
-    const bool full = r0 + P <= n;  // all P rows exist
+
+  // The operands of one group of 64 P rows: values (or presence masks) and x.
+  struct Tile {
     VT v[ND][P];
     double xv[ND][P];
     double xrr[P];
+    MTP mk;  // CV: the P rows' presence masks, unpacked only when summed (so
+             // that no wait for the load comes before the x loads are issued)
+  };
+  // An interior group (every x[r + off_d] of its rows exists), with no
+  // branch (a branch around a load makes the compiler wait for every load
+  // before the next one into the same registers).
+  auto load_full = [&](long long g, Tile& t) {
+    const long long r0 = (g * kWave + lane) * P;
+    if constexpr (CV)
+      t.mk = __builtin_nontemporal_load(reinterpret_cast<const MTP*>(mp + r0));
+    else
+      t.mk = 0;
+    if (has_shift && d0 < 0) {
+      DP xx;
+      __builtin_memcpy(&xx, x + r0, sizeof(DP));
+      if constexpr (P == 1) {
+        t.xrr[0] = xx;
+      } else {
 #pragma unroll
-    for (int k = 0; k < P; ++k) xrr[k] = 0.0;
-    if (has_shift && !(full && d0 >= 0)) {
-#pragma unroll
-      for (int k = 0; k < P; ++k)
-        if (r0 + k < n) xrr[k] = x[r0 + k];
+        for (int k = 0; k < P; ++k) t.xrr[k] = xx[k];
+      }
     }
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-      const long long c0 = r0 + off[d];
-#pragma unroll
-      for (int k = 0; k < P; ++k) {
-        v[d][k] = dia_marker<VT>();
-        xv[d][k] = 0.0;
-      }
       if (d < nd) {
-        if (full && c0 >= 0 && c0 + P <= n) {
-          const VTP vv = __builtin_nontemporal_load(reinterpret_cast<const VTP*>(dv + static_cast<long long>(d) * ld + r0));
-          DP xx;
-          __builtin_memcpy(&xx, x + c0, sizeof(DP));
+        if constexpr (!CV) {
+          const VTP vv =
+              __builtin_nontemporal_load(reinterpret_cast<const VTP*>(dv + static_cast<long long>(d) * ld + r0));
           if constexpr (P == 1) {
-            v[d][0] = vv;
-            xv[d][0] = xx;
+            t.v[d][0] = vv;
           } else {
 #pragma unroll
-            for (int k = 0; k < P; ++k) {
-              v[d][k] = vv[k];
-              xv[d][k] = xx[k];
-            }
+            for (int k = 0; k < P; ++k) t.v[d][k] = vv[k];
           }
+        }
+        DP xx;
+        __builtin_memcpy(&xx, x + r0 + off[d], sizeof(DP));
+        if constexpr (P == 1) {
+          t.xv[d][0] = xx;
         } else {
 #pragma unroll
-          for (int k = 0; k < P; ++k) {
-            const long long r = r0 + k, c = c0 + k;
-            if (r < n && c >= 0 && c < n) {
-              v[d][k] = dv[static_cast<long long>(d) * ld + r];
-              xv[d][k] = x[c];
-            }
-          }
+          for (int k = 0; k < P; ++k) t.xv[d][k] = xx[k];
+        }
+      } else {  // an unused slot of the 8- or 16-slot kernel
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+          t.v[d][k] = dia_marker<VT>();
+          t.xv[d][k] = 0.0;
         }
       }
     }
-    if (has_shift && full && d0 >= 0) {
+  };
+  // A group near either end of the matrix: one entry at a time.
+  auto load_edge = [&](long long g, Tile& t) {
+    const long long r0 = (g * kWave + lane) * P;
 #pragma unroll
-      for (int d = 0; d < ND; ++d)
-        if (d == d0) {
+    for (int k = 0; k < P; ++k) t.xrr[k] = 0.0;
+    t.mk = 0;
+    if constexpr (CV) {
+      MTP mm = 0;
 #pragma unroll
-          for (int k = 0; k < P; ++k) xrr[k] = xv[d][k];
-        }
+      for (int k = 0; k < P; ++k)
+        if (r0 + k < n) mm |= static_cast<MTP>(static_cast<MTP>(mp[r0 + k]) << (8 * sizeof(MT) * k));
+      t.mk = mm;
     }
-    double a[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+      if (has_shift && r0 + k < n) t.xrr[k] = x[r0 + k];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        t.v[d][k] = dia_marker<VT>();
+        t.xv[d][k] = 0.0;
+        const long long r = r0 + k, c = r + off[d];
+        if (d < nd && r < n && c >= 0 && c < n) {
+          if constexpr (!CV) t.v[d][k] = dv[static_cast<long long>(d) * ld + r];
+          t.xv[d][k] = x[c];
+        }
+      }
+    }
+  };
+  // Row sums of a loaded group (interior: loaded by load_full), then an
+  // empty asm that takes them as operands and clobbers memory: the sums exist
+  // before the next group's loads are issued (the compiler would otherwise
+  // sink them below those loads and keep two groups' operands in registers).
+  auto sums = [&](const Tile& t, bool interior, double (&a)[P]) {
 #pragma unroll
     for (int k = 0; k < P; ++k) {
       a[k] = 0.0;
+      const unsigned mk = static_cast<unsigned>(static_cast<unsigned long long>(t.mk) >> (8 * sizeof(MT) * k));
+      // x[r] for the shift: the main diagonal's x load when the group is
+      // interior and the matrix has one, else the load made for it
+      double xr = 0.0;
+      if (has_shift && !(interior && d0 >= 0)) xr = t.xrr[k];
 #pragma unroll
-      for (int d = 0; d < ND; ++d)
-        if (!dia_absent<VT>(v[d][k])) a[k] += static_cast<double>(v[d][k]) * (xs * xv[d][k]);
-      if (has_shift) a[k] = shift_row(a[k], sh, xs, xrr[k]);
-    }
-    if (full) {
-      if constexpr (P == 1) {
-        __builtin_nontemporal_store(a[0], w + r0);
-      } else {
-        DP o;
-#pragma unroll
-        for (int k = 0; k < P; ++k) o[k] = a[k];
-        __builtin_nontemporal_store(o, reinterpret_cast<DP*>(w + r0));
+      for (int d = 0; d < ND; ++d) {
+        const double xd = t.xv[d][k];
+        if (has_shift && interior && d == d0) xr = xd;
+        if constexpr (CV) {
+          if ((mk >> d) & 1u) a[k] += cv[d] * (xs * xd);
+        } else {
+          if (!dia_absent<VT>(t.v[d][k])) a[k] += static_cast<double>(t.v[d][k]) * (xs * xd);
+        }
       }
-    } else {
-#pragma unroll
-      for (int k = 0; k < P; ++k)
-        if (r0 + k < n) w[r0 + k] = a[k];
+      if (has_shift) a[k] = shift_row(a[k], sh, xs, xr);
     }
+#pragma unroll
+    for (int k = 0; k < P; ++k) asm volatile("" : "+v"(a[k])::"memory");
+  };
+  auto accumulate = [&](long long r0, const double (&a)[P]) {
 #pragma unroll
     for (int k = 0; k < P; ++k) {
       if (r0 + k < n) {
@@ -550,7 +625,69 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
         if (ndots) acc[0] += (cs0 * Vd[r0 + k]) * a[k];
       }
     }
+  };
+  auto store_full = [&](long long r0, const double (&a)[P]) {
+    if constexpr (P == 1) {
+      __builtin_nontemporal_store(a[0], w + r0);
+    } else {
+      DP o;
+#pragma unroll
+      for (int k = 0; k < P; ++k) o[k] = a[k];
+      __builtin_nontemporal_store(o, reinterpret_cast<DP*>(w + r0));
+    }
+  };
+
+  // Interior groups [glo, ghi) (rows whose every x[r + off_d] exists): each
+  // group's sums are formed, the next group's loads issued, then the sums
+  // stored, so that waiting for a group's loads does not also wait for the
+  // previous group's store (loads and stores retire in issue order).  The
+  // first group is peeled so that the loop is entered only with a store
+  // issued after the pending loads, and the store is unconditional, so the
+  // wait at the top of the loop can leave it in flight.  Groups before glo
+  // and from ghi on (the matrix's first and last rows, and a last group that
+  // runs past n) take the entry-by-entry path.
+  const int omin = nd > 0 ? off[0] : 0;
+  int omax = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d)
+    if (d < nd) omax = off[d];
+  const long long gp = static_cast<long long>(kWave) * P;
+  const long long glo = omin < 0 ? (-static_cast<long long>(omin) + gp - 1) / gp : 0;
+  const long long rhi = n - (omax > 0 ? omax : 0);  // rows r < rhi have x[r + omax]
+  const long long ghi = rhi > 0 ? rhi / gp : 0;
+  auto edge_group = [&](long long g) {
+    Tile t;
+    double a[P];
+    const long long r0 = (g * kWave + lane) * P;
+    load_edge(g, t);
+    sums(t, false, a);
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+      if (r0 + k < n) w[r0 + k] = a[k];
+    accumulate(r0, a);
+  };
+  const long long g0 = static_cast<long long>(lb) * kWavesPerBlock + wave;
+  long long g = g0;
+  for (; g < ngroups && g < glo; g += nw) edge_group(g);
+  if (g < ghi) {
+    Tile t;
+    double a[P];
+    load_full(g, t);
+    sums(t, true, a);
+    long long gn = g + nw;
+    if (gn < ghi) load_full(gn, t);
+    long long r0 = (g * kWave + lane) * P;
+    store_full(r0, a);
+    accumulate(r0, a);
+    for (g = gn; g < ghi; g += nw) {
+      sums(t, true, a);
+      r0 = (g * kWave + lane) * P;
+      if (g + nw < ghi) load_full(g + nw, t);
+      store_full(r0, a);
+      accumulate(r0, a);
+    }
   }
+  for (; g < ngroups; g += nw) edge_group(g);
   block_partials<2>(acc, ndots, red, b.partials, gridDim.x, lb);
   __syncthreads();
   double nv[1] = {acc[1]};
@@ -580,6 +717,28 @@ __global__ __launch_bounds__(kBlock) void k_dia_build(const int* __restrict__ rp
           break;
         }
     }
+  }
+}
+
+// The presence masks of the constant-diagonal form: one thread per row sets
+// bit d for each entry on diagonal d; rows n..ld-1 get 0.
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_dia_mask_build(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                           int n, const int* __restrict__ offs, int nd,
+                                                           long long ld, MT* __restrict__ mask) {
+  for (long long r = gtid(); r < ld; r += gstride()) {
+    unsigned m = 0;
+    if (r < n) {
+      for (int e = rp[r]; e < rp[r + 1]; ++e) {
+        const long long o = static_cast<long long>(ci[e]) - r;
+        for (int d = 0; d < nd; ++d)
+          if (offs[d] == o) {
+            m |= 1u << d;
+            break;
+          }
+      }
+    }
+    mask[r] = static_cast<MT>(m);
   }
 }
 
@@ -1018,30 +1177,37 @@ template <int MAXJ>
 static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const double* xs, double* w,
                               int ndots, int normq, int chk, hipStream_t s, int i0 = 0,
                               const double* shift = nullptr) {
-  if (b.dia_val && ndots <= 1) {
-#define MEK_DIA(VT, ND, R)                                                                          \
-  hipLaunchKernelGGL((k_spmv_dia<VT, ND, R>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, ndots, \
-                     normq, chk, i0, shift)
+  if (b.n_diags > 0 && ndots <= 1) {
+#define MEK_DIA(VT, ND, R, CV, EX)                                                                       \
+  hipLaunchKernelGGL((k_spmv_dia<VT, ND, R, CV, EX>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, \
+                     ndots, normq, chk, i0, shift)
+// the 3-, 5- and 7-point stencils get kernels of their exact diagonal count
+// (fewer registers), other counts the 8- or 16-slot kernel
+#define MEK_DIA_ND(VT, R, CV)                                  \
+  do {                                                         \
+    switch (b.n_diags) {                                       \
+      case 3: MEK_DIA(VT, 3, R, CV, true); break;              \
+      case 5: MEK_DIA(VT, 5, R, CV, true); break;              \
+      case 7: MEK_DIA(VT, 7, R, CV, true); break;              \
+      default:                                                 \
+        if (b.n_diags <= 8) MEK_DIA(VT, 8, R, CV, false);      \
+        else MEK_DIA(VT, kDiaDiagCap, R, CV, false);           \
+    }                                                          \
+  } while (0)
     // two rows per lane when w can take 16-byte stores (every buffer of the
     // runner; a caller's output of the plain SpMV may not)
     const bool pair = (reinterpret_cast<uintptr_t>(w) & 15) == 0;
-    if (b.n_diags <= 8) {
-      if (pair) {
-        if (b.vals_f32) MEK_DIA(float, 8, 2);
-        else MEK_DIA(double, 8, 2);
-      } else {
-        if (b.vals_f32) MEK_DIA(float, 8, 1);
-        else MEK_DIA(double, 8, 1);
-      }
+    if (!b.dia_val) {  // constant diagonals
+      if (pair) MEK_DIA_ND(double, 2, true);
+      else MEK_DIA_ND(double, 1, true);
+    } else if (b.vals_f32) {
+      if (pair) MEK_DIA_ND(float, 2, false);
+      else MEK_DIA_ND(float, 1, false);
     } else {
-      if (pair) {
-        if (b.vals_f32) MEK_DIA(float, kDiaDiagCap, 2);
-        else MEK_DIA(double, kDiaDiagCap, 2);
-      } else {
-        if (b.vals_f32) MEK_DIA(float, kDiaDiagCap, 1);
-        else MEK_DIA(double, kDiaDiagCap, 1);
-      }
+      if (pair) MEK_DIA_ND(double, 2, false);
+      else MEK_DIA_ND(double, 1, false);
     }
+#undef MEK_DIA_ND
 #undef MEK_DIA
     return;
   }
@@ -1092,6 +1258,18 @@ void launch_dia_build(const int* row_ptr, const int* col_idx, const void* vals, 
     hipLaunchKernelGGL(k_dia_build<double>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx,
                        static_cast<const double*>(vals), n, offsets, n_diags, ld, static_cast<double*>(dia_val));
   }
+}
+
+void launch_dia_mask_build(const int* row_ptr, const int* col_idx, int n, const int* offsets, int n_diags,
+                           long long ld, void* mask, hipStream_t s) {
+  if (ld <= 0 || n_diags <= 0) return;
+  const int grid = static_cast<int>(std::min<long long>((ld + kBlock - 1) / kBlock, 16384));
+  if (n_diags <= 8)
+    hipLaunchKernelGGL(k_dia_mask_build<unsigned char>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx, n,
+                       offsets, n_diags, ld, static_cast<unsigned char*>(mask));
+  else
+    hipLaunchKernelGGL(k_dia_mask_build<unsigned short>, dim3(grid), dim3(kBlock), 0, s, row_ptr, col_idx, n,
+                       offsets, n_diags, ld, static_cast<unsigned short*>(mask));
 }
 
 // s-step block vector i (columns k + i from k - 1 + i): y = (A - theta_i) p

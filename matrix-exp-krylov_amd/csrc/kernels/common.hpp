@@ -118,8 +118,15 @@ struct ArnoldiBufs {
   // (kDiaAbsent64 / kDiaAbsent32 bits), which the SpMV skips.
   const void* dia_val;        // double, or float when vals_f32
   const int* dia_off;
-  int n_diags;
+  int n_diags;                // > 0 when the SpMV uses a diagonal layout
   long long dia_ld;
+  // Constant-diagonal form (dia_val null): every entry of diagonal d has the
+  // value dia_cval[d] (the stored value, FP32-rounded when vals_f32), and bit
+  // d of row r's presence mask dia_mask[r] (uint8 when n_diags <= 8, else
+  // uint16; dia_ld entries, 0 past the last row) says whether the CSR holds
+  // (r, r + dia_off[d]).
+  const void* dia_mask;
+  const double* dia_cval;
 };
 
 // Marker bits of an absent entry in the diagonal copy (quiet NaNs; the copy

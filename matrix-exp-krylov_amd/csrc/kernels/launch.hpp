@@ -54,6 +54,11 @@ void launch_sell_build(const int* row_ptr, const int* col_idx, const void* vals,
 // fused spmv_dots uses it (before the sliced-ELL copy).
 void launch_dia_build(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32, int n,
                       const int* offsets, int n_diags, long long ld, void* dia_val, hipStream_t s);
+// Presence masks of the constant-diagonal form (ArnoldiBufs::dia_mask): ld
+// entries, uint8 for n_diags <= 8 else uint16; with dia_mask and dia_cval
+// set and dia_val null the diagonal SpMV reads them instead of values.
+void launch_dia_mask_build(const int* row_ptr, const int* col_idx, int n, const int* offsets, int n_diags,
+                           long long ld, void* mask, hipStream_t s);
 
 // ------------------------------------------------------------ s-step blocks
 //

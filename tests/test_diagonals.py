@@ -54,3 +54,46 @@ def test_refusals():
 
 def test_random_rows_have_no_diagonal_form():
     assert _diags(*gen.random_rows(500, 5, 30, seed=1)) == []
+
+
+def _values(rp, ci, va, f32=False):
+    A = mek.CSR.from_arrays(rp, ci, va)
+    v = A.diagonal_values(f32)
+    A.close()
+    return v
+
+
+def test_constant_diagonals_of_stencils():
+    """The constant-diagonal form's values: one per diagonal, in offset order,
+    read from the stencil's coefficients."""
+    assert _values(*gen.laplacian_2d(9)) == [1.0, 1.0, -4.0, 1.0, 1.0]
+    rp, ci, va = gen.convdiff_3d(5)
+    coeffs = _values(rp, ci, va)
+    assert len(coeffs) == 7
+    offs = _diags(rp, ci, va)
+    row = 2 * 25 + 2 * 5 + 2  # an interior row holds every diagonal once
+    got = dict(zip((ci[rp[row]:rp[row + 1]] - row).tolist(), va[rp[row]:rp[row + 1]].tolist()))
+    assert [got[o] for o in offs] == coeffs
+
+
+def test_constant_diagonals_compare_bits():
+    """A diagonal with two bit patterns has no constant form: a different
+    value, or +0.0 beside -0.0 (the products would differ in sign); with
+    f32=True values are compared after rounding to FP32."""
+    rp, ci, va = gen.laplacian_2d(6)
+    va2 = va.copy()
+    va2[rp[10] + 1] = 1.5
+    assert _values(rp, ci, va2) == []
+    zeros = np.where(va == 1.0, 0.0, va)
+    assert _values(rp, ci, zeros) == [0.0, 0.0, -4.0, 0.0, 0.0]
+    mixed = zeros.copy()
+    mixed[rp[10]] = -0.0
+    assert _values(rp, ci, mixed) == []
+    near = va.copy()
+    near[rp[14]] = 1.0 + 2.0 ** -40  # first entry of an interior row: diagonal -N
+    assert _values(rp, ci, near) == []
+    assert _values(rp, ci, near, f32=True) == [1.0, 1.0, -4.0, 1.0, 1.0]
+
+
+def test_no_constant_form_without_a_diagonal_form():
+    assert _values(*gen.random_rows(300, 5, 30, seed=2)) == []

@@ -217,6 +217,88 @@ def test_spmv_dia_refused(gpu, which):
     plan.close()
 
 
+def _constant_on_diagonals(rp, ci, va, seed):
+    """The same pattern with one value per diagonal (drawn U[-1,1], one
+    diagonal +0.0 and one -0.0 when there are at least three)."""
+    n = rp.size - 1
+    offs = ci.astype(np.int64) - np.repeat(np.arange(n), np.diff(rp))
+    uniq = np.unique(offs)
+    vals = np.random.default_rng(seed).uniform(-1, 1, uniq.size)
+    if uniq.size >= 3:
+        vals[0], vals[-1] = 0.0, -0.0
+    return vals[np.searchsorted(uniq, offs)]
+
+
+@pytest.mark.parametrize("which", ["laplacian", "convdiff", "convdiff_f32", "band_holes", "sixteen",
+                                   "sixteen_f32", "n65", "one_row", "zero_x"])
+def test_spmv_dia_const_bitexact(gpu, oracle, which):
+    """The constant-diagonal form (one value per diagonal, a presence mask
+    per row) gives the oracle's CSR-loop bits: stencils with their absent
+    boundary entries, FP32-stored values (the constants rounded as the copy
+    would round them), bands with holes and with diagonals of +0.0 and -0.0,
+    16 diagonals (16-bit masks), row counts that are not a multiple of 64 or
+    128, one row, an x with zeros, and the one-row-per-lane form for an
+    output that is not 16-byte aligned."""
+    print(f"[spmv-dia-const] {which}", flush=True)
+    f32 = which.endswith("_f32")
+    if which == "laplacian":
+        rp, ci, va = gen.laplacian_2d(77)
+    elif which.startswith("convdiff"):
+        rp, ci, va = gen.convdiff_3d(19)
+    elif which == "band_holes":
+        rp, ci, _ = _banded_sparse(1000, [-300, -7, -3, 0, 2, 5, 11, 640], 0.6, seed=3)
+        va = _constant_on_diagonals(rp, ci, None, seed=13)
+    elif which.startswith("sixteen"):
+        rp, ci, _ = _banded_sparse(3001, list(range(-8, 8)), 0.9, seed=5)
+        va = _constant_on_diagonals(rp, ci, None, seed=15)
+    elif which == "n65":
+        rp, ci, _ = _banded_sparse(65, [-1, 0, 1, 64], 0.8, seed=6)
+        va = _constant_on_diagonals(rp, ci, None, seed=16)
+    elif which == "one_row":
+        rp, ci, va = np.array([0, 1], np.int32), np.array([0], np.int32), np.array([-2.5])
+    else:
+        rp, ci, _ = _banded_sparse(500, [-2, -1, 0, 1, 2], 1.0, seed=7)
+        va = -np.abs(_constant_on_diagonals(rp, ci, None, seed=17)) - 0.25
+    n = rp.size - 1
+    x = np.random.default_rng(9).standard_normal(n)
+    if which == "zero_x":
+        x[::3] = 0.0
+    plan = mek.SpmvPlan(rp, ci, va, values_f32=f32)
+    y_csr = plan(x)
+    nd = plan.use_dia_const()
+    assert nd == np.unique(ci.astype(np.int64) - np.repeat(np.arange(n), np.diff(rp))).size
+    y = plan(x)
+    yo = oracle.spmv(rp, ci, va.astype(np.float32) if f32 else va, x)
+    np.testing.assert_array_equal(y.view(np.int64), yo.view(np.int64))
+    np.testing.assert_array_equal(y.view(np.int64), y_csr.view(np.int64))
+    d_x = mek.DeviceBuffer.from_array(x)
+    d_y = mek.DeviceBuffer(8 * (n + 1))
+    mek._check(mek.lib().mek_spmv_execute(plan.h, d_x.ptr, ctypes.c_void_p(d_y.ptr.value + 8), None))
+    mek.synchronize()
+    y1 = d_y.to_array()[1:]
+    np.testing.assert_array_equal(y1.view(np.int64), yo.view(np.int64))
+    plan.close()
+
+
+@pytest.mark.parametrize("which", ["two_values", "signed_zeros", "no_diagonal_form"])
+def test_spmv_dia_const_refused(gpu, which):
+    """No constant-diagonal form when a diagonal holds two values (also +0.0
+    beside -0.0) or the matrix has no diagonal form at all."""
+    rp, ci, va = gen.laplacian_2d(8)
+    va = va.copy()
+    if which == "two_values":
+        va[rp[20] + 1] = 0.75
+    elif which == "signed_zeros":
+        va = np.where(va == 1.0, 0.0, va)
+        va[rp[20]] = -0.0
+    else:
+        rp, ci, va = gen.random_rows(400, 5, 30, seed=2)
+    plan = mek.SpmvPlan(rp, ci, va)
+    with pytest.raises(mek.MekError):
+        plan.use_dia_const()
+    plan.close()
+
+
 @pytest.mark.parametrize("k", [1, 2, 5, 17, 40, 58, 62, 63, 80])
 def test_expm_dense(gpu, oracle, k):
     rng = np.random.default_rng(100 + k)

@@ -439,23 +439,27 @@ def test_odd_row_count(gpu, oracle, gs_mode):
 
 @pytest.mark.parametrize("sstep", [0, 12])
 def test_spmv_formats_agree(gpu, sstep):
-    """The automatic SpMV layout is the diagonal copy for a stencil, the
-    sliced-ELL copy for a band of more than 16 diagonals, and the CSR row
-    blocks for rows of 5-30 random nonzeros (slices padded to their longest
-    row would carry ~70 % padding); forcing any layout gives the same solve to
-    rounding, within 1e-8 of the DST-I closed form."""
+    """The automatic SpMV layout is the constant-diagonal form for a stencil
+    with constant coefficients, the diagonal copy for one whose diagonals
+    vary, the sliced-ELL copy for a band of more than 16 diagonals, and the
+    CSR row blocks for rows of 5-30 random nonzeros (slices padded to their
+    longest row would carry ~70 % padding); forcing any layout gives the same
+    solve to rounding, within 1e-8 of the DST-I closed form, and the two
+    diagonal forms give the same bits."""
     N, t = 80, 10.0
     rp, ci, va = gen.laplacian_2d(N)
     v = gen.normal_vector(N * N, seed=4)
     kw = dict(m=40, t=t, tol=1e-10, max_restarts=0, step_control="expokit", gs_mode=2, sstep=sstep)
     ys, infos = {}, {}
-    for fmt in ("auto", "csr", "sell", "dia"):
+    for fmt in ("auto", "csr", "sell", "dia", "dia_const"):
         R = _runner(rp, ci, va, spmv_format=fmt, **kw)
         infos[fmt] = R.spmv_format()
         ys[fmt] = R.expmv(v)
         R.close()
-    assert infos["auto"]["format"] == "dia" and infos["csr"]["format"] == "csr"
+    assert infos["auto"]["format"] == "dia_const" and infos["csr"]["format"] == "csr"
     assert infos["sell"]["format"] == "sell" and infos["dia"]["format"] == "dia"
+    assert infos["dia_const"]["format"] == "dia_const" and infos["dia_const"]["width"] == 5
+    np.testing.assert_array_equal(ys["dia_const"], ys["dia"])
     lens = np.diff(rp)
     assert infos["sell"]["width"] == 5
     assert infos["sell"]["entries"] == sum(64 * int(lens[64 * q:64 * (q + 1)].max())
@@ -467,6 +471,14 @@ def test_spmv_formats_agree(gpu, sstep):
     assert rel_err(ys["sell"], ys["csr"]) <= 1e-12
     assert rel_err(ys["dia"], ys["csr"]) <= 1e-12
     np.testing.assert_array_equal(ys["auto"], ys["dia"])
+    # a varying coefficient on one diagonal: the automatic layout keeps the values
+    va2 = va.copy()
+    va2[rp[N + 5]] = 1.0 + 2.0 ** -20
+    R = _runner(rp, ci, va2, m=20, t=0.5, tol=1e-10, max_restarts=0, step_control="expokit")
+    assert R.spmv_format()["format"] == "dia"
+    R.close()
+    with pytest.raises(mek.MekError):
+        _runner(rp, ci, va2, m=20, t=0.5, tol=1e-10, max_restarts=0, spmv_format="dia_const")
     rp, ci, va = gen.random_rows(4000, 5, 30, seed=3)
     R = _runner(rp, ci, va, m=20, t=0.5, tol=1e-10, max_restarts=0, step_control="expokit")
     assert R.spmv_format()["format"] == "csr"
