@@ -756,7 +756,8 @@ def run_single(args, mek, gen):
             r["csr_equivalent_gbs"] = round(csr_bytes / (avg_us * 1e-6) / 1e9, 1)
         return r
 
-    streaming = [k for k in ("spmv", "update_dot", "update_norm", "dots", "lift") if kern[k]["launches"]]
+    streaming = [k for k in ("spmv", "sstep_update", "sstep_dots", "update_dot", "update_norm", "dots", "lift")
+                 if kern[k]["launches"]]
     dominant = max(streaming, key=lambda k: kern[k]["ms"])
     roofline = roof(dominant)
     spmv_roof = roof("spmv")

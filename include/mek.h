@@ -168,8 +168,10 @@ int mek_runner_stats(const mek_runner* R, mek_solve_stats* out);
  * m_used, accepts, rejects (3 ints) per record; returns the count in *n. */
 int mek_runner_steps(const mek_runner* R, double* tde, int* mar, int cap, int* n);
 /* Event-timed kernel classes of the last solve (options.timing):
- * kind 0 SpMV(+dots), 1 update+dot, 2 update+norm, 3 dots, 4 finalisers,
- * 5 expm, 6 lift.  Summed ms, launches and algorithmic bytes. */
+ * kind 0 SpMV(+dots), 1 update+dot, 2 update+norm (and the s-step blocks'
+ * conditional passes, counted with no bytes), 3 dots, 4 finalisers, 5 expm,
+ * 6 lift, 7 other, 8 s-step block dots, 9 s-step block update.  Summed ms,
+ * launches and algorithmic bytes. */
 int mek_runner_kernel_time(const mek_runner* R, int kind, double* ms, long long* launches,
                            double* bytes);
 /* One Arnoldi decomposition from v: H ((m+1) x m, column-major).  Returns the

@@ -3,6 +3,7 @@
 // main.cu:1220-2081.
 #pragma once
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -99,7 +100,12 @@ struct StepRecord {
 };
 
 // Kernel classes for event timing.
-enum KernelKind : int { kKSpmv = 0, kKUpdateDot, kKUpdateNorm, kKDots, kKFinal, kKExpm, kKLift, kKOther, kNumKinds };
+// kKUpdateNorm also holds the conditional passes of the s-step blocks (second
+// update, normalising sweeps), which count no bytes since they may return at
+// once; kKSsDots / kKSsUpdate are the s-step blocks' dots and first update.
+enum KernelKind : int {
+  kKSpmv = 0, kKUpdateDot, kKUpdateNorm, kKDots, kKFinal, kKExpm, kKLift, kKOther, kKSsDots, kKSsUpdate, kNumKinds
+};
 
 struct KernelTiming {
   double ms = 0.0;        // summed event time
@@ -196,8 +202,15 @@ struct ArnoldiRunner {
   std::vector<double> ss_shifts_;
   int ss_blocks_before_ = 0, ss_second_before_ = 0, ss_fused_before_ = 0;
   void stage_copy(double* dst, const double* src, size_t n);
+  void stage_pipelined(double* dst, const double* src, size_t n, size_t nchunk,
+                       const std::function<void(size_t, size_t)>& after, const hipEvent_t* ready);
   double* h_stage_ = nullptr;  // pinned staging buffer for host v and y
   size_t h_stage_n_ = 0;
+  std::vector<hipEvent_t> stage_ev_;  // one per chunk of the staged copy of y
+  // host v and y cross in chunks (copy of one chunk overlapping the DMA of
+  // the one before) when n holds at least two chunks of kStageChunkMin
+  static constexpr size_t kStageChunks = 8;
+  static constexpr size_t kStageChunkMin = size_t(1) << 20;  // doubles
   struct EventPair { hipEvent_t a, b; int kind; double bytes; };
   std::vector<EventPair> ev_pool_;
   size_t ev_used_ = 0;

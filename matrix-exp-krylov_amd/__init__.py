@@ -22,7 +22,7 @@ CLI_PATH = os.path.join(HERE, "bin", "matrix_exp")
 
 # kernel classes of mek_runner_kernel_time
 KERNEL_KINDS = {"spmv": 0, "update_dot": 1, "update_norm": 2, "dots": 3, "final": 4,
-                "expm": 5, "lift": 6}
+                "expm": 5, "lift": 6, "other": 7, "sstep_dots": 8, "sstep_update": 9}
 
 
 STEP_CONTROL = {"reference": 0, "expokit": 1, "expokit_rel": 2}

@@ -11,9 +11,11 @@
 #include "mek/arnoldi_runner.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <exception>
 #include <iostream>
 #include <limits>
 #include <thread>
@@ -318,8 +320,13 @@ void ArnoldiRunner::init_q1_device(const double* d_v, const double* h_v) {
   const mek::ArnoldiBufs b = dc.bufs();
   hipStream_t s = dc.stream_compute;
   const size_t bytes = sizeof(double) * dc.local_rows;
-  if (d_v) MEK_HIP(hipMemcpyAsync(dc.d_V_m, d_v, bytes, hipMemcpyDeviceToDevice, s));
-  else MEK_HIP(hipMemcpyAsync(dc.d_V_m, h_v, bytes, hipMemcpyHostToDevice, s));
+  if (d_v == dc.d_V_m) {
+    // v is already in V[:, 0] (the pipelined host staging puts it there)
+  } else if (d_v) {
+    MEK_HIP(hipMemcpyAsync(dc.d_V_m, d_v, bytes, hipMemcpyDeviceToDevice, s));
+  } else {
+    MEK_HIP(hipMemcpyAsync(dc.d_V_m, h_v, bytes, hipMemcpyHostToDevice, s));
+  }
   mek::launch_norm2(b, dc.d_V_m, s);
   mek::launch_fin_start(b, s);
   MEK_HIP_LAUNCHED();
@@ -379,14 +386,14 @@ void ArnoldiRunner::run_sstep_block(int k, int ss) {
     end_timed(mek::kKSpmv, bspmv);
     mek::launch_ss_sigma(b, a, k, i, s);
   }
-  begin_timed(mek::kKDots);
+  begin_timed(mek::kKSsDots);
   mek::launch_ss_dots(a, k, ss, s);
-  end_timed(mek::kKDots, 8.0 * rows * (k + ss));
+  end_timed(mek::kKSsDots, 8.0 * rows * (k + ss));
   mek::launch_ss_reduce(a, k + ss, ss, -1, s);
   mek::launch_ss_plan(a, mek::kSsPhaseA, 0, k, ss, force2, -1, s);
-  begin_timed(mek::kKUpdateDot);
+  begin_timed(mek::kKSsUpdate);
   mek::launch_ss_update(a, 0, k + ss, k, ss, -1, s);
-  end_timed(mek::kKUpdateDot, 8.0 * rows * (k + 2 * ss));
+  end_timed(mek::kKSsUpdate, 8.0 * rows * (k + 2 * ss));
   mek::launch_ss_reduce(a, k + ss, ss, -1, s);
   mek::launch_ss_plan(a, mek::kSsPhaseB, 0, k, ss, force2, -1, s);
   // second update, only when the side dots asked for it (no bytes counted)
@@ -397,9 +404,9 @@ void ArnoldiRunner::run_sstep_block(int k, int ss) {
   mek::launch_ss_plan(a, mek::kSsPhaseB, 1, k, ss, force2, mek::kSsPass2, s);
   // normalising sweep over the block's own columns, only when the fused
   // update did not already write Q_new (no bytes counted: it may return at once)
-  begin_timed(mek::kKLift);
+  begin_timed(mek::kKUpdateNorm);
   mek::launch_ss_update(a, k, ss, k, ss, mek::kSsNormReady, s);
-  end_timed(mek::kKLift, 0.0);
+  end_timed(mek::kKUpdateNorm, 0.0);
   mek::launch_ss_reduce(a, ss, ss, mek::kSsNormReady, s);
   mek::launch_ss_plan(a, mek::kSsPhaseD, 0, k, ss, force2, mek::kSsNormReady, s);
   begin_timed(mek::kKUpdateNorm);
@@ -779,6 +786,48 @@ void ArnoldiRunner::stage_copy(double* dst, const double* src, size_t n) {
   for (auto& x : th) x.join();
 }
 
+// Copies src to dst in nchunk chunks with up to 8 host threads, each thread
+// taking its slice of every chunk.  With `after`, the calling thread waits
+// until every thread has copied chunk c and then calls after(begin, end) (the
+// DMA of that chunk) while the threads go on with the next chunks; with
+// `ready`, each thread waits for event ready[c] before copying chunk c.
+void ArnoldiRunner::stage_pipelined(double* dst, const double* src, size_t n, size_t nchunk,
+                                    const std::function<void(size_t, size_t)>& after,
+                                    const hipEvent_t* ready) {
+  const size_t nt = std::min<size_t>(8, std::max<size_t>(1, n / nchunk / (size_t(1) << 18)));
+  std::vector<std::atomic<int>> done(nchunk);
+  for (auto& d : done) d.store(0, std::memory_order_relaxed);
+  std::atomic<int> failed{0};
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t) {
+    th.emplace_back([&, t] {
+      for (size_t c = 0; c < nchunk; ++c) {
+        const size_t a = n * c / nchunk, b = n * (c + 1) / nchunk;
+        const size_t lo = a + (b - a) * t / nt, hi = a + (b - a) * (t + 1) / nt;
+        if (ready && hipEventSynchronize(ready[c]) != hipSuccess) failed.store(1);
+        if (!failed.load()) std::copy(src + lo, src + hi, dst + lo);
+        done[c].fetch_add(1, std::memory_order_release);
+      }
+    });
+  }
+  std::exception_ptr err;
+  if (after) {
+    for (size_t c = 0; c < nchunk; ++c) {
+      while (done[c].load(std::memory_order_acquire) < static_cast<int>(nt)) std::this_thread::yield();
+      if (!err) {
+        try {
+          after(n * c / nchunk, n * (c + 1) / nchunk);
+        } catch (...) {
+          err = std::current_exception();
+        }
+      }
+    }
+  }
+  for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
+  if (failed.load()) throw mek::Error("ERROR: waiting for a staged copy of y failed");
+}
+
 void ArnoldiRunner::compute_expmv(const std::vector<double>& v_host, std::vector<double>& y_host) {
   DeviceContext& dc = device_contexts[0];
   if (static_cast<int>(v_host.size()) < dc.local_rows || static_cast<int>(y_host.size()) < dc.local_rows)
@@ -798,11 +847,36 @@ void ArnoldiRunner::compute_expmv_host(const double* v_host, double* y_host) {
     MEK_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_stage_), sizeof(double) * std::max<size_t>(n, 1)));
     h_stage_n_ = n;
   }
-  stage_copy(h_stage_, v_host, n);
-  solve_on_device(nullptr, h_stage_);
-  MEK_HIP(hipMemcpyAsync(h_stage_, dc.d_y, sizeof(double) * n, hipMemcpyDeviceToHost, dc.stream_compute));
-  MEK_HIP(hipStreamSynchronize(dc.stream_compute));
-  stage_copy(y_host, h_stage_, n);
+  hipStream_t s = dc.stream_compute;
+  const size_t nchunk = n >= kStageChunkMin * 2 ? std::min<size_t>(kStageChunks, n / kStageChunkMin) : 1;
+  if (nchunk == 1) {
+    stage_copy(h_stage_, v_host, n);
+    solve_on_device(nullptr, h_stage_);
+    MEK_HIP(hipMemcpyAsync(h_stage_, dc.d_y, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    MEK_HIP(hipStreamSynchronize(s));
+    stage_copy(y_host, h_stage_, n);
+  } else {
+    // v: the host threads fill the pinned buffer chunk by chunk while the DMA
+    // engine moves the chunks already filled into V[:, 0]
+    stage_pipelined(h_stage_, v_host, n, nchunk, [&](size_t a, size_t b) {
+      MEK_HIP(hipMemcpyAsync(dc.d_V_m + a, h_stage_ + a, sizeof(double) * (b - a), hipMemcpyHostToDevice, s));
+    }, nullptr);
+    solve_on_device(dc.d_V_m, nullptr);
+    // y: every chunk's DMA is enqueued with an event after it; the host
+    // threads empty each chunk of the pinned buffer as soon as it has landed
+    if (stage_ev_.size() < nchunk) {
+      const size_t have = stage_ev_.size();
+      stage_ev_.resize(nchunk);
+      for (size_t c = have; c < nchunk; ++c)
+        MEK_HIP(hipEventCreateWithFlags(&stage_ev_[c], hipEventDisableTiming));
+    }
+    for (size_t c = 0; c < nchunk; ++c) {
+      const size_t a = n * c / nchunk, b = n * (c + 1) / nchunk;
+      MEK_HIP(hipMemcpyAsync(h_stage_ + a, dc.d_y + a, sizeof(double) * (b - a), hipMemcpyDeviceToHost, s));
+      MEK_HIP(hipEventRecord(stage_ev_[c], s));
+    }
+    stage_pipelined(y_host, h_stage_, n, nchunk, nullptr, stage_ev_.data());
+  }
   if (options.verbose) {
     // reference's analytic FLOP model (main.cu:1538-1565)
     const double N = global_rows, nnz = static_cast<double>(global_nnz), mv = params.m,
@@ -852,6 +926,8 @@ void ArnoldiRunner::destroy() {
   if (h_stage_) (void)hipHostFree(h_stage_);
   h_stage_ = nullptr;
   h_stage_n_ = 0;
+  for (auto& e : stage_ev_) (void)hipEventDestroy(e);
+  stage_ev_.clear();
   for (auto& p : ev_pool_) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);

@@ -507,6 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
     VT v[ND][P];
     double xv[ND][P];
     double xrr[P];
+    double vq[P];  // ndots: the rows of column i0, for the fused dot
     MTP mk;  // CV: the P rows' presence masks, unpacked only when summed (so
              // that no wait for the load comes before the x loads are issued)
   };
@@ -527,6 +528,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
       } else {
 #pragma unroll
         for (int k = 0; k < P; ++k) t.xrr[k] = xx[k];
+      }
+    }
+    if (ndots) {
+      DP qq;
+      __builtin_memcpy(&qq, Vd + r0, sizeof(DP));
+      if constexpr (P == 1) {
+        t.vq[0] = qq;
+      } else {
+#pragma unroll
+        for (int k = 0; k < P; ++k) t.vq[k] = qq[k];
       }
     }
 #pragma unroll
@@ -573,8 +584,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
       t.mk = mm;
     }
 #pragma unroll
-    for (int k = 0; k < P; ++k)
+    for (int k = 0; k < P; ++k) {
+      t.vq[k] = 0.0;
       if (has_shift && r0 + k < n) t.xrr[k] = x[r0 + k];
+      if (ndots && r0 + k < n) t.vq[k] = Vd[r0 + k];
+    }
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
 #pragma unroll
@@ -593,7 +607,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
   // empty asm that takes them as operands and clobbers memory: the sums exist
   // before the next group's loads are issued (the compiler would otherwise
   // sink them below those loads and keep two groups' operands in registers).
-  auto sums = [&](const Tile& t, bool interior, double (&a)[P]) {
+  // The row sums also go into the partial ||w||^2 and, with ndots, q_i0 . w.
+  auto sums = [&](long long r0, const Tile& t, bool interior, double (&a)[P]) {
 #pragma unroll
     for (int k = 0; k < P; ++k) {
       a[k] = 0.0;
@@ -613,18 +628,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
         }
       }
       if (has_shift) a[k] = shift_row(a[k], sh, xs, xr);
+      if (interior || r0 + k < n) {
+        acc[1] += a[k] * a[k];
+        if (ndots) acc[0] += (cs0 * t.vq[k]) * a[k];
+      }
     }
 #pragma unroll
     for (int k = 0; k < P; ++k) asm volatile("" : "+v"(a[k])::"memory");
-  };
-  auto accumulate = [&](long long r0, const double (&a)[P]) {
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-      if (r0 + k < n) {
-        acc[1] += a[k] * a[k];
-        if (ndots) acc[0] += (cs0 * Vd[r0 + k]) * a[k];
-      }
-    }
   };
   auto store_full = [&](long long r0, const double (&a)[P]) {
     if constexpr (P == 1) {
@@ -660,11 +670,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
     double a[P];
     const long long r0 = (g * kWave + lane) * P;
     load_edge(g, t);
-    sums(t, false, a);
+    sums(r0, t, false, a);
 #pragma unroll
     for (int k = 0; k < P; ++k)
       if (r0 + k < n) w[r0 + k] = a[k];
-    accumulate(r0, a);
   };
   const long long g0 = static_cast<long long>(lb) * kWavesPerBlock + wave;
   long long g = g0;
@@ -673,18 +682,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
     Tile t;
     double a[P];
     load_full(g, t);
-    sums(t, true, a);
+    long long r0 = (g * kWave + lane) * P;
+    sums(r0, t, true, a);
     long long gn = g + nw;
     if (gn < ghi) load_full(gn, t);
-    long long r0 = (g * kWave + lane) * P;
     store_full(r0, a);
-    accumulate(r0, a);
     for (g = gn; g < ghi; g += nw) {
-      sums(t, true, a);
       r0 = (g * kWave + lane) * P;
+      sums(r0, t, true, a);
       if (g + nw < ghi) load_full(g + nw, t);
       store_full(r0, a);
-      accumulate(r0, a);
     }
   }
   for (; g < ngroups; g += nw) edge_group(g);
