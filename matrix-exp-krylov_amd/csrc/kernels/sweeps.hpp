@@ -165,7 +165,13 @@ __device__ __forceinline__ void sweep_dots(const double* __restrict__ V, long lo
 
 // out = x - sum_{i<nq} c_i V[:, i] with c_i = h_i cs_i; partials of
 // q_i . out (quantity i) and ||out||^2 (quantity nq).  out is not a column
-// in [0, nq).
+// in [0, nq).  Two tiles are in flight per wave: tile t + 2 ts is loaded as
+// soon as tile t is done (register sets A and B alternate, so nothing is
+// copied), a load is never inside a branch (a tile past the range reloads
+// the last one, whose lines are in L2), and each tile's result is stored
+// before the next loads are issued.  On the 4096^2 Laplacian this took the
+// sweep from 4.30 to 3.85 ms per solve; the same overlap in sweep_dots made
+// it slower (3.33 to 3.90 ms), so that sweep loads one tile at a time.
 template <int MAXK, int R = 1>
 __device__ __forceinline__ void sweep_update_dot(const double* __restrict__ V, long long ld, int nq,
                                                  const double* h, const double* cs,
@@ -189,27 +195,32 @@ __device__ __forceinline__ void sweep_update_dot(const double* __restrict__ V, l
   for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
   double nrm = 0.0;
   int buf = 0;
-  for (long long t = tr.t0; t < tr.ntiles; t += tr.ts) {
-    const long long r = tile_row<R>(tr, t, lane);
-    const long long rc = clamp_row<R>(tr, r);
+  struct Tile {
     double v[MAXK][R];
+    double x[R];
+  };
+  auto load = [&](long long t, Tile& T) {
+    const long long tc = t < tr.ntiles ? t : tr.ntiles - 1;
+    const long long rc = clamp_row<R>(tr, tile_row<R>(tr, tc, lane));
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k) load_rows_nt<R>(V + col[k] + rc, v[k]);
-    double xr[R];
-    load_rows<R>(x + rc, xr);
+    for (int k = 0; k < MAXK; ++k) load_rows_nt<R>(V + col[k] + rc, T.v[k]);
+    load_rows<R>(x + rc, T.x);
+  };
+  auto use = [&](long long t, const Tile& T) {
+    const long long r = tile_row<R>(tr, t, lane);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       double p = 0.0;
 #pragma unroll
-      for (int k = 0; k < MAXK; ++k) p += c[k] * v[k][q];
+      for (int k = 0; k < MAXK; ++k) p += c[k] * T.v[k][q];
       part[buf][wave][R * lane + q] = p;
     }
     __syncthreads();
     double a[R];
 #pragma unroll
     for (int q = 0; q < R; ++q)
-      a[q] = xr[q] - ((part[buf][0][R * lane + q] + part[buf][1][R * lane + q]) +
-                      (part[buf][2][R * lane + q] + part[buf][3][R * lane + q]));
+      a[q] = T.x[q] - ((part[buf][0][R * lane + q] + part[buf][1][R * lane + q]) +
+                       (part[buf][2][R * lane + q] + part[buf][3][R * lane + q]));
     if (r < tr.rend) {
       const bool all = (r + R - 1) < tr.rend;
       if (!all) a[R - 1] = 0.0;
@@ -221,9 +232,25 @@ __device__ __forceinline__ void sweep_update_dot(const double* __restrict__ V, l
 #pragma unroll
       for (int k = 0; k < MAXK; ++k)
 #pragma unroll
-        for (int q = 0; q < R; ++q) acc[k] += v[k][q] * a[q];
+        for (int q = 0; q < R; ++q) acc[k] += T.v[k][q] * a[q];
     }
     buf ^= 1;
+  };
+  if (tr.t0 < tr.ntiles) {
+    Tile A, B;
+    long long t = tr.t0;
+    load(t, A);
+    load(t + tr.ts, B);
+    for (;;) {
+      use(t, A);
+      load(t + 2 * tr.ts, A);
+      t += tr.ts;
+      if (t >= tr.ntiles) break;
+      use(t, B);
+      load(t + 2 * tr.ts, B);
+      t += tr.ts;
+      if (t >= tr.ntiles) break;
+    }
   }
   write_column_partials<MAXK>(acc, nrm, nq, true, cs, red, partials, pstride, pidx);
 }
