@@ -437,6 +437,13 @@ void ArnoldiRunner::run_columns_range(int j0, int j1) {
       mek::launch_spmv_dots(b, j, s);
       end_timed(mek::kKSpmv, bspmv_csr + 8.0 * n * nq);
       mek::launch_fin_h1(b, nq, b.spmv_grid, s);
+    } else if (mek::spmv_dia_fuses_dots(b, nq)) {
+      // stencils: the dots against q_0 .. q_j formed while w is in registers
+      // (q_j is the SpMV's x, already counted: j more columns read)
+      begin_timed(mek::kKSpmv);
+      mek::launch_spmv_dia_col_dots(b, j, s);
+      end_timed(mek::kKSpmv, bspmv + 8.0 * n * j);
+      mek::launch_fin_h1(b, nq, b.spmv_grid, s);
     } else {
       begin_timed(mek::kKSpmv);
       mek::launch_spmv_col(b, j, spare_q, s);

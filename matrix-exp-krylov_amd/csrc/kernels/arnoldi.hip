@@ -463,13 +463,13 @@ struct DiaMaskLoad<8> {
   typedef unsigned long long type;
 };
 
-template <typename VT, int ND, int P, bool CV, bool EX>
+template <typename VT, int ND, int P, bool CV, bool EX, int FD>
 __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double* x_in, const double* xs_ptr,
                                                      double* wout, int ndots, int normq, int check_breakdown,
                                                      int i0, const double* shift_ptr) {
 #pragma clang fp contract(off)
   if (check_breakdown && (b.iscal[kBreakdown] || (check_breakdown == 2 && b.iscal[kSStepFail]))) return;
-  __shared__ double red[kWavesPerBlock * 2];
+  __shared__ double red[kWavesPerBlock * (FD + 1)];
   typedef typename DiaVec<VT, P>::type VTP;
   typedef typename DiaVec<double, P>::type DP;
   typedef typename std::conditional<(ND <= 8), unsigned char, unsigned short>::type MT;
@@ -500,14 +500,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
   double* __restrict__ w = wout;
   const long long ngroups = (n + kWave * P - 1) / (kWave * P);
   const long long nw = static_cast<long long>(gridDim.x) * kWavesPerBlock;
-  double acc[2] = {0.0, 0.0};  // q_i0 . w, ||w||^2
+  double acc[2] = {0.0, 0.0};  // q_i0 . w (FD = 1), ||w||^2
+  double accd[FD];             // FD > 1: sum of V[r, i0 + q] w[r], scaled by colscale at the end
+#pragma unroll
+  for (int q = 0; q < FD; ++q) accd[q] = 0.0;
 
   // The operands of one group of 64 P rows: values (or presence masks) and x.
   struct Tile {
     VT v[ND][P];
     double xv[ND][P];
     double xrr[P];
-    double vq[P];  // ndots: the rows of column i0, for the fused dot
+    double vq[FD][P];  // ndots: the rows of columns i0 .. i0 + ndots - 1, for the fused dots
     MTP mk;  // CV: the P rows' presence masks, unpacked only when summed (so
              // that no wait for the load comes before the x loads are issued)
   };
@@ -530,14 +533,20 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
         for (int k = 0; k < P; ++k) t.xrr[k] = xx[k];
       }
     }
-    if (ndots) {
-      DP qq;
-      __builtin_memcpy(&qq, Vd + r0, sizeof(DP));
-      if constexpr (P == 1) {
-        t.vq[0] = qq;
-      } else {
+    if (FD > 1 || ndots) {
+      // FD > 1: all FD columns are loaded (surplus ones repeat the last
+      // column, a cache hit), so no load is inside a branch
 #pragma unroll
-        for (int k = 0; k < P; ++k) t.vq[k] = qq[k];
+      for (int q = 0; q < FD; ++q) {
+        const int qc = q < ndots ? q : ndots - 1;
+        DP qq;
+        __builtin_memcpy(&qq, Vd + static_cast<long long>(qc) * b.ld + r0, sizeof(DP));
+        if constexpr (P == 1) {
+          t.vq[q][0] = qq;
+        } else {
+#pragma unroll
+          for (int k = 0; k < P; ++k) t.vq[q][k] = qq[k];
+        }
       }
     }
 #pragma unroll
@@ -585,9 +594,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
     }
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      t.vq[k] = 0.0;
       if (has_shift && r0 + k < n) t.xrr[k] = x[r0 + k];
-      if (ndots && r0 + k < n) t.vq[k] = Vd[r0 + k];
+#pragma unroll
+      for (int q = 0; q < FD; ++q) {
+        t.vq[q][k] = 0.0;
+        if (q < ndots && r0 + k < n) t.vq[q][k] = Vd[static_cast<long long>(q) * b.ld + r0 + k];
+      }
     }
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -630,7 +642,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
       if (has_shift) a[k] = shift_row(a[k], sh, xs, xr);
       if (interior || r0 + k < n) {
         acc[1] += a[k] * a[k];
-        if (ndots) acc[0] += (cs0 * t.vq[k]) * a[k];
+        if constexpr (FD == 1) {
+          if (ndots) acc[0] += (cs0 * t.vq[0][k]) * a[k];
+        } else {
+#pragma unroll
+          for (int q = 0; q < FD; ++q)
+            if (q < ndots) accd[q] += t.vq[q][k] * a[k];
+        }
       }
     }
 #pragma unroll
@@ -695,7 +713,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dia(ArnoldiBufs b, const double
     }
   }
   for (; g < ngroups; g += nw) edge_group(g);
-  block_partials<2>(acc, ndots, red, b.partials, gridDim.x, lb);
+  if constexpr (FD == 1) {
+    block_partials<2>(acc, ndots, red, b.partials, gridDim.x, lb);
+  } else {
+#pragma unroll
+    for (int q = 0; q < FD; ++q) accd[q] *= (q < ndots) ? b.colscale[i0 + q] : 0.0;
+    block_partials<FD>(accd, ndots, red, b.partials, gridDim.x, lb);
+  }
   __syncthreads();
   double nv[1] = {acc[1]};
   block_partials<1>(nv, 1, red, b.partials + static_cast<long long>(normq) * gridDim.x, gridDim.x, lb);
@@ -1185,8 +1209,8 @@ static void launch_spmv_dot_t(const ArnoldiBufs& b, const double* x, const doubl
                               int ndots, int normq, int chk, hipStream_t s, int i0 = 0,
                               const double* shift = nullptr) {
   if (b.n_diags > 0 && ndots <= 1) {
-#define MEK_DIA(VT, ND, R, CV, EX)                                                                       \
-  hipLaunchKernelGGL((k_spmv_dia<VT, ND, R, CV, EX>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, \
+#define MEK_DIA(VT, ND, R, CV, EX)                                                                          \
+  hipLaunchKernelGGL((k_spmv_dia<VT, ND, R, CV, EX, 1>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, w, \
                      ndots, normq, chk, i0, shift)
 // the 3-, 5- and 7-point stencils get kernels of their exact diagonal count
 // (fewer registers), other counts the 8- or 16-slot kernel
@@ -1294,6 +1318,51 @@ void launch_spmv_col(const ArnoldiBufs& b, int j, int normq, hipStream_t s) {
   const double* x = b.V + static_cast<long long>(j) * b.ld;
   const double* xs = b.colscale + j;
   launch_spmv_dot_t<1>(b, x, xs, b.w, 0, normq, 1, s);
+}
+
+bool spmv_dia_fuses_dots(const ArnoldiBufs& b, int nq) {
+  const int nd = b.n_diags;
+  return (nd == 3 || nd == 5 || nd == 7) && nq >= 1 && nq <= kDiaFusedDots &&
+         (reinterpret_cast<uintptr_t>(b.w) & 15) == 0;
+}
+
+// Column j on a diagonal layout with its Gram-Schmidt dots fused into the
+// SpMV (spmv_dia_fuses_dots): partials of q_i . w (i <= j) and ||w||^2
+// (quantity j + 1), spmv_grid blocks.
+void launch_spmv_dia_col_dots(const ArnoldiBufs& b, int j, hipStream_t s) {
+  const double* x = b.V + static_cast<long long>(j) * b.ld;
+  const double* xs = b.colscale + j;
+  const int nq = j + 1;
+#define MEK_DD(VT, ND, CV, FD)                                                                              \
+  hipLaunchKernelGGL((k_spmv_dia<VT, ND, 2, CV, true, FD>), dim3(b.spmv_grid), dim3(kBlock), 0, s, b, x, xs, \
+                     b.w, nq, nq, 1, 0, static_cast<const double*>(nullptr))
+#define MEK_DD_FD(VT, ND, CV) \
+  do {                        \
+    if (nq <= 2)              \
+      MEK_DD(VT, ND, CV, 2);  \
+    else if (nq <= 4)         \
+      MEK_DD(VT, ND, CV, 4);  \
+    else                      \
+      MEK_DD(VT, ND, CV, 8);  \
+  } while (0)
+#define MEK_DD_ND(VT, CV)                       \
+  do {                                          \
+    if (b.n_diags == 3)                         \
+      MEK_DD_FD(VT, 3, CV);                     \
+    else if (b.n_diags == 5)                    \
+      MEK_DD_FD(VT, 5, CV);                     \
+    else                                        \
+      MEK_DD_FD(VT, 7, CV);                     \
+  } while (0)
+  if (!b.dia_val)
+    MEK_DD_ND(double, true);
+  else if (b.vals_f32)
+    MEK_DD_ND(float, false);
+  else
+    MEK_DD_ND(double, false);
+#undef MEK_DD_ND
+#undef MEK_DD_FD
+#undef MEK_DD
 }
 
 // Column j with the first Gram-Schmidt pass fused into the SpMV (nq <= 64):

@@ -54,6 +54,12 @@ void launch_sell_build(const int* row_ptr, const int* col_idx, const void* vals,
 // fused spmv_dots uses it (before the sliced-ELL copy).
 void launch_dia_build(const int* row_ptr, const int* col_idx, const void* vals, int vals_f32, int n,
                       const int* offsets, int n_diags, long long ld, void* dia_val, hipStream_t s);
+// Column j on a 3-, 5- or 7-diagonal layout with its first Gram-Schmidt pass
+// (q_i . w for i <= j, and ||w||^2 at quantity j + 1, spmv_grid blocks)
+// fused into the SpMV, for j + 1 <= kDiaFusedDots (spmv_dia_fuses_dots).
+constexpr int kDiaFusedDots = 8;
+bool spmv_dia_fuses_dots(const ArnoldiBufs& b, int nq);
+void launch_spmv_dia_col_dots(const ArnoldiBufs& b, int j, hipStream_t s);
 // Presence masks of the constant-diagonal form (ArnoldiBufs::dia_mask): ld
 // entries, uint8 for n_diags <= 8 else uint16; with dia_mask and dia_cval
 // set and dia_val null the diagonal SpMV reads them instead of values.
