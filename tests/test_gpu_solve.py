@@ -6,6 +6,7 @@ import os
 import subprocess
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 from closed_forms import breakdown_trace, diag2_m1_trace, kron3_expmv, laplacian_2d_eigvec, laplacian_2d_expmv, rel_err
@@ -534,3 +535,37 @@ def test_host_staging_matches_device_solve(gpu, N):
         np.testing.assert_array_equal(y_host.view(np.int64), d_y.to_array().view(np.int64))
         assert rel_err(y_host, laplacian_2d_expmv(N, 0.05, v)) <= TOL
     R.close()
+
+
+@pytest.mark.parametrize("which", ["laplacian", "convdiff"])
+def test_fused_first_column_dots_give_the_hessenberg_matrix(gpu, which):
+    """On a 5- or 7-diagonal matrix the column path forms the Gram-Schmidt
+    dots of columns j <= 7 inside the diagonal SpMV (8-, 4- and 2-dot
+    kernels); later columns and the CSR layout use the separate dot sweep.
+    The Hessenberg matrices of the two layouts agree to rounding over 12
+    columns, and both satisfy the Arnoldi relation."""
+    if which == "laplacian":
+        rp, ci, va = gen.laplacian_2d(48)
+    else:
+        rp, ci, va = gen.convdiff_3d(13)
+    n = rp.size - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    v = gen.normal_vector(n, seed=7)
+    Hs = {}
+    for fmt in ("dia_const", "csr"):
+        R = _runner(rp, ci, va, m=12, t=1.0, gs_mode=2, spmv_format=fmt)
+        assert R.spmv_format()["format"] == fmt
+        H, k = R.arnoldi(v)
+        R.close()
+        assert k == 12
+        Hs[fmt] = H[:13, :12]
+    scale = np.abs(Hs["csr"]).max()
+    np.testing.assert_allclose(Hs["dia_const"], Hs["csr"], rtol=0, atol=1e-12 * scale)
+    # Arnoldi relation with the column path's basis rebuilt from H: A V_12 = V_13 H
+    V = np.zeros((n, 13))
+    V[:, 0] = v / np.linalg.norm(v)
+    H = Hs["dia_const"]
+    for j in range(12):
+        w = A @ V[:, j] - V[:, :j + 1] @ H[:j + 1, j]
+        V[:, j + 1] = w / H[j + 1, j]
+    assert np.abs(V.T @ V - np.eye(13)).max() < 1e-9
