@@ -138,7 +138,7 @@ def _banded_sparse(n, offsets, keep, seed):
 
 
 @pytest.mark.parametrize("which", ["laplacian", "convdiff", "convdiff_f32", "band_holes", "nine", "sixteen",
-                                   "sixteen_f32", "n65", "one_row", "zero_x"])
+                                   "sixteen_f32", "n65", "one_row", "zero_x", "tridiag", "tridiag_holes"])
 def test_spmv_dia_bitexact(gpu, oracle, which):
     """The diagonal copy gives the oracle's CSR-loop bits: stencils with their
     absent boundary entries, FP32 values, bands with holes, explicit (and
@@ -169,6 +169,9 @@ def test_spmv_dia_bitexact(gpu, oracle, which):
     elif which == "one_row":
         rp, ci, va = np.array([0, 1], np.int32), np.array([0], np.int32), np.array([-2.5])
         n_diags = 1
+    elif which.startswith("tridiag"):  # the exact 3-diagonal kernels
+        rp, ci, va = _banded_sparse(777, [-1, 0, 1], 0.7 if which.endswith("holes") else 1.0, seed=21)
+        n_diags = 3
     else:
         rp, ci, va = _banded_sparse(500, [-2, -1, 0, 1, 2], 1.0, seed=7)
         va = -np.abs(va)
@@ -230,7 +233,7 @@ def _constant_on_diagonals(rp, ci, va, seed):
 
 
 @pytest.mark.parametrize("which", ["laplacian", "convdiff", "convdiff_f32", "band_holes", "sixteen",
-                                   "sixteen_f32", "n65", "one_row", "zero_x"])
+                                   "sixteen_f32", "n65", "one_row", "zero_x", "tridiag_holes"])
 def test_spmv_dia_const_bitexact(gpu, oracle, which):
     """The constant-diagonal form (one value per diagonal, a presence mask
     per row) gives the oracle's CSR-loop bits: stencils with their absent
@@ -256,6 +259,9 @@ def test_spmv_dia_const_bitexact(gpu, oracle, which):
         va = _constant_on_diagonals(rp, ci, None, seed=16)
     elif which == "one_row":
         rp, ci, va = np.array([0, 1], np.int32), np.array([0], np.int32), np.array([-2.5])
+    elif which == "tridiag_holes":  # the exact 3-diagonal kernels
+        rp, ci, _ = _banded_sparse(777, [-1, 0, 1], 0.7, seed=21)
+        va = _constant_on_diagonals(rp, ci, None, seed=22)
     else:
         rp, ci, _ = _banded_sparse(500, [-2, -1, 0, 1, 2], 1.0, seed=7)
         va = -np.abs(_constant_on_diagonals(rp, ci, None, seed=17)) - 0.25
