@@ -537,17 +537,22 @@ def test_host_staging_matches_device_solve(gpu, N):
     R.close()
 
 
-@pytest.mark.parametrize("which", ["laplacian", "convdiff"])
+@pytest.mark.parametrize("which", ["laplacian", "convdiff", "tridiag"])
 def test_fused_first_column_dots_give_the_hessenberg_matrix(gpu, which):
-    """On a 5- or 7-diagonal matrix the column path forms the Gram-Schmidt
+    """On a 3-, 5- or 7-diagonal matrix the column path forms the Gram-Schmidt
     dots of columns j <= 7 inside the diagonal SpMV (8-, 4- and 2-dot
     kernels); later columns and the CSR layout use the separate dot sweep.
     The Hessenberg matrices of the two layouts agree to rounding over 12
     columns, and both satisfy the Arnoldi relation."""
     if which == "laplacian":
         rp, ci, va = gen.laplacian_2d(48)
-    else:
+    elif which == "convdiff":
         rp, ci, va = gen.convdiff_3d(13)
+    else:  # a 1-D convection-diffusion operator, nonsymmetric
+        n = 3001
+        T = sp.diags([np.full(n - 1, 1.3), np.full(n, -2.0), np.full(n - 1, 0.7)], [-1, 0, 1], format="csr")
+        T.sort_indices()
+        rp, ci, va = T.indptr.astype(np.int32), T.indices.astype(np.int32), T.data.astype(np.float64)
     n = rp.size - 1
     A = sp.csr_matrix((va, ci, rp), shape=(n, n))
     v = gen.normal_vector(n, seed=7)
