@@ -646,6 +646,55 @@ def run_block(args, mek, gen):
     print(json.dumps(out), flush=True)
 
 
+def spmv_layout_comparison(mek, rp, ci, va, f32):
+    """The same SpMV (w = A x, one x of N(0,1) entries) in each layout the
+    matrix has, through the plan API: mean wall time per launch over 5
+    batches of 20 back-to-back launches after a synchronisation (median of
+    the batches), the layout's bytes, and whether every layout gave the CSR
+    row blocks' bits."""
+    n = rp.size - 1
+    x = np.random.default_rng(5).standard_normal(n)
+    d_x = mek.DeviceBuffer.from_array(x)
+    d_y = mek.DeviceBuffer(8 * n)
+    vb = 4 if f32 else 8
+    nnz = int(rp[-1])
+    out, ref = {}, None
+    for name, switch in (("csr", None), ("dia", "use_dia"), ("dia_const", "use_dia_const")):
+        plan = mek.SpmvPlan(rp, ci, va, values_f32=f32)
+        try:
+            nd = getattr(plan, switch)() if switch else 0
+        except mek.MekError:
+            plan.close()
+            continue
+        for _ in range(5):
+            plan.execute(d_x, d_y)
+        mek.synchronize()
+        per = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            for _ in range(20):
+                plan.execute(d_x, d_y)
+            mek.synchronize()
+            per.append((time.perf_counter() - t0) / 20)
+        y = d_y.to_array()
+        plan.close()
+        if name == "csr":
+            nbytes = nnz * (vb + 4) + 4.0 * (n + 1) + 16.0 * n
+            ref = y
+        elif name == "dia":
+            nbytes = nd * n * vb + 16.0 * n
+        else:
+            nbytes = (1 if nd <= 8 else 2) * n + 16.0 * n
+        us = statistics.median(per) * 1e6
+        out[name] = {"us_per_launch": round(us, 1), "layout_bytes": nbytes,
+                     "layout_gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
+                     "same_bits_as_csr": bool(ref is not None and np.array_equal(y.view(np.int64),
+                                                                              ref.view(np.int64)))}
+    out["note"] = ("wall time per launch of 20 back-to-back launches (median of 5), the plan API on the "
+                   "default stream, x ~ N(0,1); layout bytes as in the roofline")
+    return out
+
+
 def run_single(args, mek, gen):
     w = WORKLOADS[args.workload]
     log(f"building {args.workload}")
@@ -810,6 +859,9 @@ def run_single(args, mek, gen):
         "launch_shapes": tuning,
         "cpu_baseline": None,
     }
+    if not (args.fuse or args.lanczos):
+        log("SpMV layouts side by side")
+        out["spmv_layouts"] = spmv_layout_comparison(mek, rp, ci, va, f32_used)
     if args.cpu_seconds > 0:
         log("CPU baseline (oracle, one thread, one full sub-step)")
         out["cpu_baseline"] = cpu_baseline_substep(rp, ci, va, v, w["m"], steps[0]["dt"], len(steps))
