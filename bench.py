@@ -801,8 +801,22 @@ def run_single(args, mek, gen):
              "avg_launch_us": avg_us}
         if k == "spmv":
             r["layout"] = layout
-            r["csr_bytes_per_launch"] = csr_bytes
-            r["csr_equivalent_gbs"] = round(csr_bytes / (avg_us * 1e-6) / 1e9, 1)
+            # the plain launch's layout bytes; what a launch moves beyond them
+            # (the first columns' fused dots read q columns) is added to the
+            # CSR figure too
+            vbl = 4 if f32_used else 8
+            fmt, ent, wid = layout["format"], layout["entries"], layout["width"]
+            if fmt == "dia_const":
+                plain = (1 if wid <= 8 else 2) * n + 16.0 * n
+            elif fmt == "dia":
+                plain = wid * n * vbl + 16.0 * n
+            elif fmt == "sell":
+                plain = ent * (vbl + 4) + 8.0 * ((n + 63) // 64 + 1) + 16.0 * n
+            else:
+                plain = csr_bytes
+            csr_eq = csr_bytes + max(0.0, per_launch - plain)
+            r["csr_bytes_per_launch"] = csr_eq
+            r["csr_equivalent_gbs"] = round(csr_eq / (avg_us * 1e-6) / 1e9, 1)
         return r
 
     streaming = [k for k in ("spmv", "sstep_update", "sstep_dots", "update_dot", "update_norm", "dots", "lift")
